@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""LDE + Merkle benchmark of the stwo commitment hot path on one MI355X.
+
+Workload (one step): 256 M31 columns of 2^20 bit-reversed evaluations on
+CanonicCoset(20), resident in HBM -> interpolate_columns (circle IFFT) ->
+evaluate_polynomials at blowup 4 onto CanonicCoset(22) (RFFT) -> Blake2s
+Merkle commit of the 256 evaluations.  Then, once, the FRI chain on one QM31
+evaluation of 2^22 points (fold_circle_into_line, fold_line down to 2^4, a
+Merkle commit per layer, Blake2s channel), reported beside.
+
+value = input trace elements (256 * 2^20) / step time, in Melem/s.  Inputs
+are synthetic (seeded SplitMix64, u64 % P) and generated on the device.
+Interpolation runs in place, so each step first restores the evaluations
+from a pristine device copy; that copy is outside the timed span.
+
+Prints ONE JSON line.  --gpus must be 1 (one GPU per box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "stwo-metal_amd"))
+
+import numpy as np  # noqa: E402
+
+import stwo_hip as sh  # noqa: E402
+
+N_COLS, LOG_N, LOG_BLOWUP = 256, 20, 2
+FRI_LOG = 22
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_step(ctx, work, pristine, outs, layers, tw, logs):
+    for w, p in zip(work, pristine):
+        ctx._check(sh.lib().stwo_hip_copy(ctx.h, w.ptr, p.ptr, w.n))
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.interpolate_columns(work, logs, tw)
+    ctx.evaluate_polynomials(work, logs, LOG_BLOWUP, tw, outs)
+    ctx.merkle_commit(outs, [LOG_N + LOG_BLOWUP] * N_COLS, layers)
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
+def fri_chain(ctx, tw):
+    """FRI on a low-degree QM31 evaluation of 2^22 points (degree < 2^21, blowup 2)."""
+    cols = []
+    for k in range(4):
+        co = ctx.alloc(1 << (FRI_LOG - 1))
+        ctx.fill_random(co, 1 << (FRI_LOG - 1), 2, 1000 + k)
+        out = ctx.alloc(1 << FRI_LOG)
+        ctx.evaluate(co, FRI_LOG - 1, sh.canonic_domain(FRI_LOG), tw, out)
+        co.free()
+        cols.append(out)
+    cfg = sh.FriConfig(1, 3, 3, 1)  # last layer 2^4
+    ctx.sync()
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        roots, last, digest, nd = ctx.fri_commit(bytes(32), 0, cfg, cols, FRI_LOG, tw)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    for c in cols:
+        c.free()
+    return best * 1e3, int(roots.shape[0])
+
+
+def cpu_baseline(sample_cols=16):
+    """The oracle (C, OpenMP over columns/nodes) on a sample of the same workload."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    init = oracle.half_odds_initial(LOG_N + LOG_BLOWUP - 1)
+    tw, itw = oracle.precompute_twiddles(init, LOG_N + LOG_BLOWUP - 1)
+    evs = [oracle.fill_random(1 << LOG_N, 1, c) for c in range(sample_cols)]
+    t0 = time.perf_counter()
+    oracle.interpolate_columns(evs, itw)
+    outs = oracle.evaluate_polynomials(evs, LOG_BLOWUP, tw)
+    oracle.merkle_commit(outs)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(sample_cols * (1 << LOG_N) / dt / 1e6, 3),
+        "unit": "Melem/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": "%d of the 256 columns (2^20 -> blowup 4 -> 2^22) + Blake2s Merkle commit of those columns; "
+                  "%.2f s on the host" % (sample_cols, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fri", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus %d requested); "
+              "the multi-GPU layout is a design note, not a measured path." % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    ctx = sh.Context(0)
+    root = sh.half_odds(LOG_N + LOG_BLOWUP - 1)
+    tw = ctx.precompute_twiddles(root)
+    logs = [LOG_N] * N_COLS
+    log("allocating %d x 2^%d inputs, %d x 2^%d evaluations" % (N_COLS, LOG_N, N_COLS, LOG_N + LOG_BLOWUP))
+    pristine = [ctx.alloc(1 << LOG_N) for _ in range(N_COLS)]
+    for c, p in enumerate(pristine):
+        ctx.fill_random(p, 1 << LOG_N, 2, c)
+    work = [ctx.alloc(1 << LOG_N) for _ in range(N_COLS)]
+    outs = [ctx.alloc(1 << (LOG_N + LOG_BLOWUP)) for _ in range(N_COLS)]
+    layers = ctx.alloc(sh.tree_words(LOG_N + LOG_BLOWUP))
+    ctx.sync()
+
+    for i in range(args.warmup):
+        dt = run_step(ctx, work, pristine, outs, layers, tw, logs)
+        log("warmup %d: %.2f ms" % (i, dt * 1e3))
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    times = []
+    for i in range(args.steps):
+        dt = run_step(ctx, work, pristine, outs, layers, tw, logs)
+        times.append(dt)
+        log("step %d: %.2f ms" % (i, dt * 1e3))
+    prof = ctx.prof_read()
+    ctx.prof_enable(False)
+    total = sum(times)
+    ms_per_step = total / args.steps * 1e3
+    elems = N_COLS * (1 << LOG_N)
+    value = elems / (total / args.steps) / 1e6
+
+    # Per-kernel device time (HIP events around every launch on the context stream).
+    kernels = {k: {"launches": v[0], "ms_total": round(v[1], 3)} for k, v in prof.items() if k != "twiddle_tree"}
+    # Algorithmic bytes per step for each kernel class (twiddle reads excluded):
+    #   IFFT 8 B per input element; RFFT 4 B in + 4 B out per output element side;
+    #   Merkle leaves read 4 B per element per column + 32 B per leaf; nodes 96 B per node.
+    n_out = 1 << (LOG_N + LOG_BLOWUP)
+    alg = {
+        "ifft_tile": 8.0 * elems,
+        "rfft_tile": 4.0 * elems + 4.0 * N_COLS * n_out,
+        "merkle_commit_on_layer": 4.0 * N_COLS * n_out + 32.0 * n_out,
+        "merkle_nodes": 96.0 * (n_out - 1),
+    }
+    dom_name = max((k for k in kernels if k in alg), key=lambda k: kernels[k]["ms_total"])
+    dom_ms = kernels[dom_name]["ms_total"] / args.steps
+    achieved = alg[dom_name] / (dom_ms * 1e-3) / 1e9
+    for k in kernels:
+        if k in alg:
+            kernels[k]["algorithmic_GBps"] = round(alg[k] / (kernels[k]["ms_total"] / args.steps * 1e-3) / 1e9, 1)
+    log("kernels: %s" % json.dumps(kernels))
+
+    fri_ms, fri_layers = (None, None)
+    if not args.no_fri:
+        fri_ms, fri_layers = fri_chain(ctx, tw)
+        log("fri chain: %.2f ms, %d committed layers" % (fri_ms, fri_layers))
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline()
+        log("cpu: %s" % json.dumps(cpu))
+
+    for x in pristine + work + outs + [layers]:
+        x.free()
+    tw.destroy()
+    ctx.close()
+
+    line = {
+        "metric": "LDE+Merkle Melem/s (interpolate + evaluate + Blake2s Merkle commit, bit-exact)",
+        "value": round(value, 2),
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (M31)",
+        "data": "synthetic (seeded SplitMix64 u64 % P, generated on device)",
+        "config": {
+            "workload": "plain method, large: 256 M31 columns x 2^20 -> LDE blowup 4 (2^22 rows) -> Blake2s "
+                        "Merkle commit; FRI chain on a 2^22 QM31 evaluation reported beside",
+            "n_cols": N_COLS, "log_size": LOG_N, "log_blowup": LOG_BLOWUP,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "fri_chain_ms": round(fri_ms, 3) if fri_ms is not None else None,
+        "fri_committed_layers": fri_layers,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

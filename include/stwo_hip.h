@@ -1,0 +1,228 @@
+/*
+ * stwo_hip.h -- C ABI of the MI355X (gfx950) implementation of the stwo
+ * Circle-STARK commitment hot path.
+ *
+ * Each entry point replaces one operation of stwo's Backend traits; the
+ * comment on each names the trait method it stands in for and the reference
+ * interface it replaces.  Paths are relative to the reference checkout:
+ *   STWO = vendor/stwo-upstream-dev-62b228e/crates/stwo/src
+ *   SYS  = crates/stwo-metal-sys      (the reference's FFI crate)
+ *   MB   = crates/stwo-metal/src/backend/metal
+ *
+ * Conventions
+ *   - Device memory is plain `uint32_t *` (one M31 element or one hash word per
+ *     u32).  A BaseField column of log size n is 2^n words in bit-reversed
+ *     domain order, as in CircleEvaluation<_, BaseField, BitReversedOrder>.
+ *     A SecureField (QM31) column is 4 separate coordinate columns
+ *     (SecureColumnByCoords, STWO/prover/secure_column.rs).
+ *   - A Blake2s hash is 8 words (its 32 bytes, little endian).  A Merkle tree
+ *     is returned as packed layers: layer l (2^l hashes) starts at word
+ *     (2^l - 1) * 8, so the root is words [0, 8) -- the reference's
+ *     `layers[0]` -- and the tree of max layer L has (2^(L+1) - 1) * 8 words.
+ *   - Arrays of columns are host arrays of device pointers.
+ *   - Every call is enqueued on the context's stream and returns without
+ *     waiting unless its comment says it synchronizes.  Results are ready
+ *     after stwo_hip_sync().
+ *   - Errors: every function returns STWO_HIP_OK (0) or a negative status;
+ *     the message is kept per context (stwo_hip_last_error).  The library
+ *     never aborts.  This replaces the reference's `bool` + 512-byte error
+ *     buffer convention (SYS/src/metal.rs:8,4477-4485).
+ *   - Values must be canonical M31 ([0, P), P = 2^31 - 1); every output is
+ *     canonical.
+ */
+#ifndef STWO_HIP_H
+#define STWO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define STWO_HIP_OK 0
+#define STWO_HIP_EINVAL (-1)   /* bad argument (size, order, NULL, domain)  */
+#define STWO_HIP_ENOMEM (-2)   /* device or host allocation failed          */
+#define STWO_HIP_EDEVICE (-3)  /* HIP runtime error                         */
+#define STWO_HIP_EDEGREE (-4)  /* FRI last layer is not of low degree       */
+#define STWO_HIP_EBUSY (-5)    /* serving queue full / budget exceeded      */
+
+/* Hash output mode: Blake2sMerkleHasher or Blake2sM31MerkleHasher
+ * (STWO/core/vcs/blake2_merkle.rs:8-10): the M31 variant reduces each output
+ * word mod P (reduce_to_m31, STWO/core/vcs/blake2_hash.rs:111). */
+#define STWO_HIP_HASH_BLAKE2S 0
+#define STWO_HIP_HASH_BLAKE2S_M31 1
+
+typedef struct stwo_hip_ctx stwo_hip_ctx;
+typedef struct stwo_hip_twiddles stwo_hip_twiddles;
+
+/* A coset `initial_index + <2^(31 - log_size)>` of the circle group, by its
+ * CirclePointIndex (STWO/core/circle.rs:286-305).  Circle domains are given
+ * by their half coset (CircleDomain, STWO/core/poly/circle/domain.rs:22-32);
+ * CanonicCoset(n).circle_domain() has half coset (2^(30-n), n-1). */
+typedef struct {
+  uint32_t initial_index;
+  uint32_t log_size;
+} stwo_hip_coset;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the process-global Metal runtime (SYS/src/metal.rs:4442-4448,
+ * runtime creation SH/runtime.m:476-512).  One context = one device + one
+ * stream; create several to run independent work concurrently. */
+int stwo_hip_ctx_create(int device, stwo_hip_ctx **out);
+int stwo_hip_ctx_destroy(stwo_hip_ctx *ctx);
+/* Copies the last error message (NUL-terminated, truncated to len). */
+int stwo_hip_last_error(const stwo_hip_ctx *ctx, char *buf, size_t len);
+/* Blocks until all work enqueued on the context's stream is done. */
+int stwo_hip_sync(stwo_hip_ctx *ctx);
+/* The context's hipStream_t, for callers that order their own work. */
+void *stwo_hip_stream(stwo_hip_ctx *ctx);
+
+/* ---- device memory ------------------------------------------------------
+ * Replaces stwo_metal_u32_buffer_{alloc_zeroed,alloc_uninitialized,destroy,
+ * from_host,read,copy} (SH/runtime.m:523-916). */
+int stwo_hip_alloc(stwo_hip_ctx *ctx, size_t n_words, uint32_t **out);
+int stwo_hip_free(stwo_hip_ctx *ctx, uint32_t *ptr);
+/* Synchronous host<->device copies (they wait for the stream). */
+int stwo_hip_upload(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words);
+int stwo_hip_download(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words);
+int stwo_hip_copy(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words);
+int stwo_hip_memset_zero(stwo_hip_ctx *ctx, uint32_t *dst, size_t n_words);
+/* Fills `n` words with seeded uniform values in [0, P) (SplitMix64 of
+ * (seed, stream, index), u64 % P).  Synthetic benchmark inputs. */
+int stwo_hip_fill_random(stwo_hip_ctx *ctx, uint32_t *dst, size_t n, uint64_t seed, uint64_t stream);
+
+/* ---- twiddles -----------------------------------------------------------
+ * PolyOps::precompute_twiddles(coset) (STWO/prover/poly/circle/ops.rs:144;
+ * CpuBackend layout CPUB/circle.rs:294-311): for each doubling of the root
+ * coset, the x-coordinates of its first half in bit-reversed order, then a
+ * pad of 1; plus the elementwise inverses.  Built on the device and kept
+ * resident (replaces MB/poly.rs:185-224 and SH/runtime.m:1026).  A tree may
+ * be shared by every context on its device. */
+int stwo_hip_twiddles_create(stwo_hip_ctx *ctx, stwo_hip_coset root, stwo_hip_twiddles **out);
+int stwo_hip_twiddles_destroy(stwo_hip_ctx *ctx, stwo_hip_twiddles *tw);
+/* Device pointers and length (= root size) of the tree and its inverse. */
+int stwo_hip_twiddles_buffers(const stwo_hip_twiddles *tw, const uint32_t **twiddles, const uint32_t **itwiddles,
+                              size_t *len);
+
+/* ---- ColumnOps ------------------------------------------------------------
+ * ColumnOps<BaseField>::bit_reverse_column (STWO/prover/backend/mod.rs:37-40;
+ * Metal MB/column.rs:24-32 -> SH/bit_reverse.metal:13). In place. */
+int stwo_hip_bit_reverse_column(stwo_hip_ctx *ctx, uint32_t *col, uint32_t log_size);
+/* ColumnOps<SecureField>::bit_reverse_column on the 4 coordinate columns
+ * (MB/column.rs:34-42 -> SH/bit_reverse.metal:31). */
+int stwo_hip_bit_reverse_secure_column(stwo_hip_ctx *ctx, uint32_t *const coords[4], uint32_t log_size);
+
+/* ---- PolyOps ------------------------------------------------------------
+ * PolyOps::interpolate (STWO/prover/poly/circle/ops.rs:24-27; CPUB/circle.rs:28-80;
+ * Metal MB/poly.rs:364-391 -> stwo_metal_ifft_interpolate_u32,
+ * SH/runtime.m:1572).  `col` holds the bit-reversed evaluation on the circle
+ * domain with half coset `domain`; it is replaced in place by the
+ * coefficients (natural FFT-basis order).  The domain must be a doubling of
+ * the tree's root coset (or the root itself). */
+int stwo_hip_interpolate(stwo_hip_ctx *ctx, uint32_t *col, stwo_hip_coset domain, const stwo_hip_twiddles *tw);
+/* PolyOps::interpolate_columns (ops.rs:29-40) over canonic domains
+ * CanonicCoset(log_sizes[i]).circle_domain(); one batched launch per pass. */
+int stwo_hip_interpolate_columns(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                                 const stwo_hip_twiddles *tw);
+/* PolyOps::evaluate / evaluate_into (ops.rs:90-108; CPUB/circle.rs:190-250;
+ * Metal MB/poly.rs:332-362 -> stwo_metal_rfft_evaluate_u32, SH/runtime.m:1218).
+ * Coefficients of log size `coeffs_log_size` are evaluated on the circle
+ * domain with half coset `domain` (log size domain.log_size + 1 >=
+ * coeffs_log_size); `out` gets 2^(domain.log_size+1) bit-reversed values. */
+int stwo_hip_evaluate(stwo_hip_ctx *ctx, const uint32_t *coeffs, uint32_t coeffs_log_size, stwo_hip_coset domain,
+                      const stwo_hip_twiddles *tw, uint32_t *out);
+/* PolyOps::evaluate_polynomials (ops.rs:110-141; Metal MB/poly.rs:658-793):
+ * column i (log size log_sizes[i]) onto CanonicCoset(log_sizes[i] + log_blowup). */
+int stwo_hip_evaluate_polynomials(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes,
+                                  size_t n_cols, uint32_t log_blowup, const stwo_hip_twiddles *tw,
+                                  uint32_t *const *outs);
+
+/* ---- MerkleOps<Blake2sMerkleHasher> (mixed degree) -----------------------
+ * MerkleOps::commit_on_layer (STWO/prover/vcs/ops.rs:24-28; SimdBackend
+ * SIMD/blake2s.rs:60-144; no Metal implementation exists).  Node i =
+ * Blake2s([prev[2i] || prev[2i+1]] || le32(cols[0][i]) || ...).  `prev` is
+ * NULL for the leaf layer, else 2^(log_size+1) hashes; each column has
+ * 2^log_size values; `out` receives 2^log_size hashes. */
+int stwo_hip_merkle_commit_on_layer(stwo_hip_ctx *ctx, uint32_t log_size, const uint32_t *prev,
+                                    const uint32_t *const *cols, size_t n_cols, uint32_t *out, int hash_mode);
+/* MerkleProver::commit (STWO/prover/vcs/prover.rs:39-65): columns of any
+ * power-of-two sizes, each injected at its own layer (stable descending
+ * order).  `layers` receives the packed tree of max layer = max log size
+ * (root first).  With n_cols == 0 the root is Blake2s("") (8 words). */
+int stwo_hip_merkle_commit(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                           uint32_t *layers, int hash_mode);
+
+/* ---- MerkleOpsLifted<Blake2sMerkleHasher> ----------------------------------
+ * MerkleOpsLifted::build_leaves (STWO/prover/vcs_lifted/ops.rs:13-15;
+ * CPUB/merkle_lifted.rs:44-78; Metal MB/blake2s.rs:203-392 ->
+ * SH/blake2s.metal:282-370).  Columns are hashed in stable ascending length
+ * order; column c of log size s contributes c[lift(i, L - s)] to leaf i.
+ * `out` receives 2^lifting_log_size hashes. */
+int stwo_hip_merkle_lifted_build_leaves(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes,
+                                        size_t n_cols, uint32_t lifting_log_size, uint32_t *out, int hash_mode);
+/* build_next_layer (ops.rs:19-20; SH/blake2s.metal:459): out[i] = H(prev[2i] || prev[2i+1]);
+ * `out_log_size` is the log size of the output layer. */
+int stwo_hip_merkle_lifted_build_next_layer(stwo_hip_ctx *ctx, const uint32_t *prev, uint32_t out_log_size,
+                                            uint32_t *out, int hash_mode);
+/* build_merkle_layers (ops.rs:29-45) -> MerkleProverLifted::commit
+ * (STWO/prover/vcs_lifted/prover.rs:33-44): packed tree of max layer
+ * lifting_log_size. */
+int stwo_hip_merkle_lifted_commit(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes,
+                                  size_t n_cols, uint32_t lifting_log_size, uint32_t *layers, int hash_mode);
+
+/* ---- FriOps -----------------------------------------------------------------
+ * FriOps::fold_circle_into_line (STWO/prover/fri.rs:61-66; CPUB/fri.rs:95-123;
+ * Metal MB/fri.rs:347-365 -> SH/fold_circle_into_line.metal): for i < N/2,
+ * dst[i] = dst[i] * alpha^2 + f0 + alpha * f1 with (f0, f1) =
+ * ibutterfly(src[2i], src[2i+1], 1/p.y), p = domain.at(bitrev(2i)).
+ * `src_domain` is the half coset of src's circle domain (log N - 1); dst has
+ * N/2 values.  `alpha` is a QM31 as 4 limbs. */
+int stwo_hip_fold_circle_into_line(stwo_hip_ctx *ctx, uint32_t *const dst[4], const uint32_t *const src[4],
+                                   stwo_hip_coset src_domain, const uint32_t alpha[4], const stwo_hip_twiddles *tw);
+/* FriOps::fold_line (fri.rs:39-44; CPUB/fri.rs:15-30,67-92; SimdBackend
+ * SIMD/fri.rs:32-122; Metal MB/fri.rs:295-345): `fold_step` folds with alpha,
+ * alpha^2, ... of the line evaluation on LineDomain(`line_domain`);
+ * dst has 2^(line_domain.log_size - fold_step) values. */
+int stwo_hip_fold_line(stwo_hip_ctx *ctx, const uint32_t *const src[4], stwo_hip_coset line_domain,
+                       const uint32_t alpha[4], uint32_t fold_step, const stwo_hip_twiddles *tw,
+                       uint32_t *const dst[4]);
+
+/* ---- FriProver::commit --------------------------------------------------------
+ * Runs the FRI commitment phase (STWO/prover/fri.rs:133-291) for one QM31
+ * evaluation on CanonicCoset(log_size).circle_domain() with a Blake2s
+ * channel (STWO/core/channel/blake2s.rs): commit the first layer (lifted
+ * Merkle of the 4 coordinate columns), then per inner layer: mix root, draw
+ * alpha, fold, commit; the last layer is interpolated on the host, its
+ * degree checked and its coefficients mixed into the channel.
+ * channel_digest (32 bytes) and channel_n_draws are read and updated.
+ * roots: (number of committed layers) * 8 words, first layer first;
+ * n_roots receives that number.  last_layer_poly: 4 words per coefficient,
+ * 2^log_last_layer_degree_bound coefficients in LinePoly order.
+ * Synchronizes.  Returns STWO_HIP_EDEGREE if the degree check fails. */
+typedef struct {
+  uint32_t log_blowup_factor;
+  uint32_t log_last_layer_degree_bound;
+  uint32_t n_queries;
+  uint32_t line_fold_step;
+} stwo_hip_fri_config;
+
+int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                        stwo_hip_fri_config config, const uint32_t *const column[4], uint32_t log_size,
+                        const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly);
+
+/* ---- profiling ------------------------------------------------------------
+ * When enabled, every kernel launch of the context is bracketed by HIP
+ * events on the context's stream; stwo_hip_prof_read synchronizes and
+ * reports, per kernel name, the number of launches and their summed device
+ * time.  Names are reported through `name_buf` (NUL-separated). */
+int stwo_hip_prof_enable(stwo_hip_ctx *ctx, int enable);
+int stwo_hip_prof_reset(stwo_hip_ctx *ctx);
+int stwo_hip_prof_read(stwo_hip_ctx *ctx, char *name_buf, size_t name_buf_len, uint64_t *counts, double *total_ms,
+                       size_t max_entries, size_t *n_entries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STWO_HIP_H */

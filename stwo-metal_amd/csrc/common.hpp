@@ -1,0 +1,112 @@
+// Context, error handling, per-call table arena and launch profiling shared
+// by every translation unit of libstwo_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "stwo_hip.h"
+
+struct stwo_hip_twiddles {
+  int device;
+  stwo_hip_coset root;
+  size_t len;          // root size (2^root.log_size)
+  uint32_t *d_tw;      // twiddle tree, CpuBackend layout
+  uint32_t *d_itw;     // elementwise inverses
+};
+
+namespace stwo_hip {
+
+struct ProfPending {
+  const char *name;
+  hipEvent_t start, stop;
+};
+
+}  // namespace stwo_hip
+
+struct stwo_hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  std::mutex mu;
+
+  // Ring arena for small per-call tables (column pointer lists, per-column
+  // parameters).  Host side is pinned so the H2D copy is a true async DMA;
+  // the device side is read by kernels on the same stream, so stream order
+  // alone protects it.  `inflight` keeps, per upload, its host byte range and
+  // an event after its copy; host bytes are only rewritten once the copy that
+  // read them has completed.
+  uint8_t *h_arena = nullptr;
+  uint8_t *d_arena = nullptr;
+  size_t arena_cap = 0;
+  size_t arena_head = 0;
+  struct Inflight {
+    size_t begin, end;
+    hipEvent_t ev;
+  };
+  std::deque<Inflight> inflight;
+  std::vector<hipEvent_t> event_pool;
+
+  // Launch profiling.
+  bool prof = false;
+  std::vector<stwo_hip::ProfPending> prof_pending;
+  std::map<std::string, std::pair<uint64_t, double>> prof_totals;
+
+  // Scratch device buffers kept across calls, keyed by purpose.
+  std::map<std::string, std::pair<void *, size_t>> scratch;
+};
+
+namespace stwo_hip {
+
+int set_error(stwo_hip_ctx *ctx, int code, const char *fmt, ...);
+
+#define STWO_HIP_TRY(ctx, expr)                                                                   \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess)                                                                         \
+      return ::stwo_hip::set_error((ctx), STWO_HIP_EDEVICE, "%s: %s (%s:%d)", #expr,              \
+                                   hipGetErrorString(_e), __FILE__, __LINE__);                    \
+  } while (0)
+
+#define STWO_HIP_CHECK_ARG(ctx, cond, ...)                                   \
+  do {                                                                       \
+    if (!(cond)) return ::stwo_hip::set_error((ctx), STWO_HIP_EINVAL, __VA_ARGS__); \
+  } while (0)
+
+// Copies `bytes` of host data into the arena and returns the device address
+// (16-byte aligned).  The copy is enqueued on the context stream.
+int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr);
+
+// Returns a device scratch buffer of at least `bytes` for `key` (grown on
+// demand; growing synchronizes the stream before freeing the old buffer).
+int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr);
+
+hipEvent_t take_event(stwo_hip_ctx *ctx);
+void prof_begin(stwo_hip_ctx *ctx, const char *name, hipEvent_t *start);
+void prof_end(stwo_hip_ctx *ctx, const char *name, hipEvent_t start);
+int check_launch(stwo_hip_ctx *ctx, const char *name);
+
+// Launches `kernel` on the context stream, bracketed by profiling events
+// when profiling is enabled, and reports launch errors.
+template <typename... KArgs, typename... Args>
+int launch(stwo_hip_ctx *ctx, const char *name, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem,
+           Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return STWO_HIP_OK;
+  hipEvent_t start = nullptr;
+  prof_begin(ctx, name, &start);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, static_cast<KArgs>(args)...);
+  int rc = check_launch(ctx, name);
+  prof_end(ctx, name, start);
+  return rc;
+}
+
+const stwo_hip_twiddles *check_twiddles(stwo_hip_ctx *ctx, const stwo_hip_twiddles *tw);
+
+}  // namespace stwo_hip

@@ -1,0 +1,436 @@
+// Circle IFFT / RFFT (PolyOps::interpolate / evaluate) and bit reversal.
+//
+// Semantics (CpuBackend, CPUB/circle.rs:28-80, 199-250, 313-354): a column of
+// log size N has N butterfly layers.  Layer s >= 1 (a "line" layer) pairs
+// elements 2^s apart; the pair starting at index i0 uses twiddle
+// line_tw[s-1][i0 >> (s+1)], where line layer j of the domain's twiddles is
+// the tree slice [len - 2^(N-1-j), len - 2^(N-2-j)) (domain_line_twiddles_from_tree,
+// STWO/core/poly/utils.rs:98-114).  Layer 0 (the "circle" layer) pairs
+// neighbours with twiddles [y, -y, -x, x] built from consecutive pairs (x, y)
+// of line layer 0 (circle_twiddles_from_line_twiddles).  RFFT runs layers
+// N-1 .. 0 with `butterfly`; IFFT runs 0 .. N-1 with `ibutterfly` on the
+// inverse tree and scales by 1/2^N (STWO/core/fft.rs:5-21).
+//
+// Evaluating 2^n coefficients on a domain of 2^N points zero-pads them; the
+// top N-n layers then only copy (butterfly(v, 0, t) = (v, v)), so the first
+// pass reads coefficient index (g mod 2^n) for output index g and starts at
+// layer n-1 -- the 2^(N-n) independent sub-FFTs of SimdBackend's
+// evaluate_into (SIMD/circle.rs:413-440), fused into one pass over the
+// output.
+//
+// MI355X mapping: a pass moves a 4096-element tile (16 KB) of one column
+// through LDS.  The tile owns up to 12 consecutive layers: the lowest pass
+// covers layers [0, 12) on a contiguous chunk; higher layers run in strided
+// passes of <= 6 layers whose tile rows are 64 consecutive elements (256 B
+// coalesced runs).  Columns up to 2^12 take one pass, up to 2^18 two, up to
+// 2^24 three.  blockIdx.y selects the column, so one launch covers a whole
+// group of same-size columns.
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+#include "field.hpp"
+
+namespace stwo_hip {
+
+constexpr uint32_t kTileLog = 12;
+constexpr uint32_t kTile = 1u << kTileLog;
+constexpr uint32_t kFftThreads = 256;
+constexpr uint32_t kStridedMaxLayers = 6;
+
+struct FftCol {
+  const uint32_t *in;
+  uint32_t *out;
+};
+
+struct FftPass {
+  const FftCol *cols;     // one entry per blockIdx.y
+  const uint32_t *tw_end; // one past the end of the (inverse) twiddle tree
+  uint32_t N;             // domain log size
+  uint32_t lo;            // lowest layer of the tile's butterfly bits
+  uint32_t m;             // tile butterfly bits: layers [lo, lo + m)
+  uint32_t w;             // tile inner bits (consecutive elements per row)
+  uint32_t s_begin, s_end;// layers processed, subset of [lo, lo + m)
+  uint32_t in_log;        // input index = g mod 2^in_log
+  uint32_t inverse;
+  uint32_t scale;         // multiply outputs by this when nonzero
+};
+
+__device__ __forceinline__ uint32_t line_twiddle(const uint32_t *tw_end, uint32_t N, uint32_t s, uint32_t h) {
+  return tw_end[(int64_t)h - ((int64_t)1 << (N - s))];
+}
+
+// Circle twiddle for pair h (elements 2h, 2h+1): [y, -y, -x, x] from the pair
+// (x, y) = line layer 0 entries 2(h/4), 2(h/4)+1 (CPUB/circle.rs:332-354).
+__device__ __forceinline__ uint32_t circle_twiddle(const uint32_t *tw_end, uint32_t N, uint32_t h) {
+  const uint32_t *l0 = tw_end - ((int64_t)1 << (N - 1));
+  uint32_t x = l0[2 * (h >> 2)], y = l0[2 * (h >> 2) + 1];
+  uint32_t v = (h & 2) ? x : y;
+  bool neg = ((h & 3) == 1) || ((h & 3) == 2);
+  return neg ? m31_neg(v) : v;
+}
+
+__device__ __forceinline__ uint32_t tile_global(const FftPass &p, uint32_t tile, uint32_t t) {
+  uint32_t b = t >> p.w;
+  uint32_t il = t & ((1u << p.w) - 1);
+  uint32_t ih_bits = p.lo - p.w;
+  uint32_t ih = tile & ((1u << ih_bits) - 1);
+  uint32_t o = tile >> ih_bits;
+  return (o << (p.lo + p.m)) | (b << p.lo) | (ih << p.w) | il;
+}
+
+__global__ void __launch_bounds__(kFftThreads) fft_tile_kernel(FftPass p) {
+  __shared__ uint32_t lds[kTile];
+  const FftCol col = p.cols[blockIdx.y];
+  const uint32_t tile = blockIdx.x;
+  const uint32_t T = p.m + p.w;
+  const uint32_t tsize = 1u << T;
+  const uint32_t in_mask = (uint32_t)((1ull << p.in_log) - 1);
+  for (uint32_t t = threadIdx.x; t < tsize; t += kFftThreads) lds[t] = col.in[tile_global(p, tile, t) & in_mask];
+  __syncthreads();
+  const uint32_t n_layers = p.s_end - p.s_begin;
+  for (uint32_t k = 0; k < n_layers; k++) {
+    const uint32_t s = p.inverse ? p.s_begin + k : p.s_end - 1 - k;
+    const uint32_t qb = s - p.lo + p.w;  // tile bit of this layer
+    for (uint32_t pr = threadIdx.x; pr < tsize / 2; pr += kFftThreads) {
+      uint32_t i0 = ((pr >> qb) << (qb + 1)) | (pr & ((1u << qb) - 1));
+      uint32_t i1 = i0 | (1u << qb);
+      uint32_t h = tile_global(p, tile, i0) >> (s + 1);
+      uint32_t t = s ? line_twiddle(p.tw_end, p.N, s, h) : circle_twiddle(p.tw_end, p.N, h);
+      uint32_t v0 = lds[i0], v1 = lds[i1];
+      if (p.inverse) {
+        lds[i0] = m31_add(v0, v1);
+        lds[i1] = m31_mul(m31_sub(v0, v1), t);
+      } else {
+        uint32_t tmp = m31_mul(v1, t);
+        lds[i0] = m31_add(v0, tmp);
+        lds[i1] = m31_sub(v0, tmp);
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t t = threadIdx.x; t < tsize; t += kFftThreads) {
+    uint32_t v = lds[t];
+    if (p.scale) v = m31_mul(v, p.scale);
+    col.out[tile_global(p, tile, t)] = v;
+  }
+}
+
+// Domains of 2 and 4 points: the closed forms of CPUB/circle.rs:35-59
+// (interpolate) and :215-235 (evaluate).  c[] holds host-computed constants.
+struct SmallFft {
+  const FftCol *cols;
+  uint32_t N, in_log, inverse;
+  uint32_t c0, c1, c2, c3;
+};
+
+__global__ void fft_small_kernel(SmallFft a) {
+  const FftCol col = a.cols[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  uint32_t mask = (1u << a.in_log) - 1;
+  uint32_t n = 1u << a.N;
+  uint32_t v[4];
+  for (uint32_t i = 0; i < n; i++) v[i] = (i <= mask) ? col.in[i] : 0;
+  auto bf = [](uint32_t &x0, uint32_t &x1, uint32_t t) {
+    uint32_t tmp = m31_mul(x1, t);
+    x1 = m31_sub(x0, tmp);
+    x0 = m31_add(x0, tmp);
+  };
+  auto ibf = [](uint32_t &x0, uint32_t &x1, uint32_t t) {
+    uint32_t tmp = x0;
+    x0 = m31_add(tmp, x1);
+    x1 = m31_mul(m31_sub(tmp, x1), t);
+  };
+  if (!a.inverse) {
+    if (a.N == 1) {
+      bf(v[0], v[1], a.c1);  // y
+    } else {
+      bf(v[0], v[2], a.c0);  // x
+      bf(v[1], v[3], a.c0);
+      bf(v[0], v[1], a.c1);  // y
+      bf(v[2], v[3], m31_neg(a.c1));
+    }
+  } else {
+    if (a.N == 1) {
+      ibf(v[0], v[1], a.c1);  // y_inv
+      v[0] = m31_mul(v[0], a.c2);
+      v[1] = m31_mul(v[1], a.c2);  // n_inv
+    } else {
+      ibf(v[0], v[1], a.c1);  // y_inv
+      ibf(v[2], v[3], m31_neg(a.c1));
+      ibf(v[0], v[2], a.c0);  // x_inv
+      ibf(v[1], v[3], a.c0);
+      for (int i = 0; i < 4; i++) v[i] = m31_mul(v[i], a.c2);
+    }
+  }
+  for (uint32_t i = 0; i < n; i++) col.out[i] = v[i];
+}
+
+// ---- bit reversal -----------------------------------------------------------
+// In-place v[i] <-> v[bitrev(i)] (STWO/core/utils.rs:91-101).  A tile of
+// 64 x 64 elements, rows a, columns b, is exchanged with the tile at
+// (rev(b-block), rev(a-block)) through LDS so both reads and writes are
+// 256-byte coalesced runs; tiles on the diagonal swap within themselves.
+// Index split for log n = 2k + r bits: i = [hi: 6 | mid: r' | lo: 6].
+constexpr uint32_t kBrLog = 6;
+
+__global__ void __launch_bounds__(256) bit_reverse_small_kernel(uint32_t *v, uint32_t log_n) {
+  uint32_t n = 1u << log_n;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t j = bitrev32(i, log_n);
+    if (i < j) {
+      uint32_t a = v[i], b = v[j];
+      v[i] = b;
+      v[j] = a;
+    }
+  }
+}
+
+// For log_n >= 2 * kBrLog: i = (hi << (log_n - 6)) | (mid << 6) | lo with
+// hi, lo 6 bits; bitrev(i) = (rev6(lo) << (log_n - 6)) | (rev(mid) << 6) | rev6(hi).
+// Block (mid) handles the 64x64 tile {hi, lo} at `mid` and its partner at rev(mid).
+__global__ void __launch_bounds__(256) bit_reverse_tiled_kernel(uint32_t *v, uint32_t log_n) {
+  __shared__ uint32_t ta[64][65];
+  __shared__ uint32_t tb[64][65];
+  const uint32_t mid_bits = log_n - 2 * kBrLog;
+  const uint32_t mid = blockIdx.x;
+  const uint32_t rmid = bitrev32(mid, mid_bits);
+  if (rmid < mid) return;  // the partner block does the swap
+  const uint32_t hs = log_n - kBrLog;
+  for (uint32_t e = threadIdx.x; e < 4096; e += 256) {
+    uint32_t hi = e >> 6, lo = e & 63;
+    ta[hi][lo] = v[(hi << hs) | (mid << kBrLog) | lo];
+    tb[hi][lo] = v[(hi << hs) | (rmid << kBrLog) | lo];
+  }
+  __syncthreads();
+  // new v[(hi, mid, lo)] = old v[bitrev] = old v[(rev(lo), rmid, rev(hi))] = tb[rev lo][rev hi]
+  for (uint32_t e = threadIdx.x; e < 4096; e += 256) {
+    uint32_t hi = e >> 6, lo = e & 63;
+    uint32_t rh = bitrev32(hi, kBrLog), rl = bitrev32(lo, kBrLog);
+    v[(hi << hs) | (mid << kBrLog) | lo] = tb[rl][rh];
+    if (rmid != mid) v[(hi << hs) | (rmid << kBrLog) | lo] = ta[rl][rh];
+  }
+}
+
+// ---- planning ---------------------------------------------------------------
+
+struct PassPlan {
+  uint32_t lo, m, w, s_begin, s_end;
+};
+
+// Passes covering layers [0, L) of a domain of log size N (N >= 3), in
+// forward (top layers first) order.
+static std::vector<PassPlan> plan_passes(uint32_t N, uint32_t L) {
+  std::vector<PassPlan> fwd;
+  uint32_t t0 = std::min(N, kTileLog);
+  if (L > t0) {
+    uint32_t rest = L - t0;
+    uint32_t npass = (rest + kStridedMaxLayers - 1) / kStridedMaxLayers;
+    uint32_t hi = L;
+    for (uint32_t k = 0; k < npass; k++) {
+      uint32_t left = hi - t0, passes_left = npass - k;
+      uint32_t m = (left + passes_left - 1) / passes_left;
+      uint32_t lo = hi - m;
+      fwd.push_back({lo, m, kTileLog - m, lo, hi});
+      hi = lo;
+    }
+  }
+  fwd.push_back({0, t0, 0, 0, std::min(L, t0)});
+  return fwd;
+}
+
+// Validates that the circle domain with half coset `dom` is a doubling of
+// the tree's root (CPUB/circle.rs:205 asserts is_doubling_of).
+static int check_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
+  STWO_HIP_CHECK_ARG(ctx, tw != nullptr && tw->device == ctx->device, "twiddles missing or from another device");
+  STWO_HIP_CHECK_ARG(ctx, dom.log_size <= tw->root.log_size, "domain half coset log %u exceeds twiddle root log %u",
+                     dom.log_size, tw->root.log_size);
+  uint32_t k = tw->root.log_size - dom.log_size;
+  uint32_t expect = (uint32_t)(((uint64_t)tw->root.initial_index << k) & IDX_MASK);
+  STWO_HIP_CHECK_ARG(ctx, (dom.initial_index & IDX_MASK) == expect,
+                     "domain (initial %u, log %u) is not a doubling of the twiddle root (initial %u, log %u)",
+                     dom.initial_index, dom.log_size, tw->root.initial_index, tw->root.log_size);
+  return STWO_HIP_OK;
+}
+
+// Runs the transform of a group of same-shape columns.  Forward: `in` holds
+// 2^n coefficients, `out` receives 2^N evaluations.  Inverse: in == out holds
+// 2^N evaluations replaced by coefficients (n == N).
+static int run_fft_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, uint32_t n, uint32_t N,
+                         const stwo_hip_twiddles *tw, bool inverse, stwo_hip_coset dom) {
+  if (cols.empty()) return STWO_HIP_OK;
+  STWO_HIP_CHECK_ARG(ctx, cols.size() <= 65535, "too many columns in one group (%zu)", cols.size());
+  void *dtab = nullptr;
+  int rc = upload_table(ctx, cols.data(), cols.size() * sizeof(FftCol), &dtab);
+  if (rc) return rc;
+  if (N <= 2) {
+    SmallFft a{};
+    a.cols = (const FftCol *)dtab;
+    a.N = N;
+    a.in_log = n;
+    a.inverse = inverse;
+    cpoint pt = host_index_to_point(dom.initial_index);
+    if (!inverse) {
+      a.c0 = pt.x;
+      a.c1 = pt.y;
+    } else if (N == 1) {
+      uint32_t yn_inv = m31_inv(m31_mul(pt.y, 2));
+      a.c1 = m31_mul(yn_inv, 2);      // y^-1
+      a.c2 = m31_mul(yn_inv, pt.y);   // 1/2
+    } else {
+      uint32_t xyn_inv = m31_inv(m31_mul(m31_mul(pt.x, pt.y), 4));
+      a.c0 = m31_mul(m31_mul(xyn_inv, pt.y), 4);  // x^-1
+      a.c1 = m31_mul(m31_mul(xyn_inv, pt.x), 4);  // y^-1
+      a.c2 = m31_mul(m31_mul(xyn_inv, pt.x), pt.y);  // 1/4
+    }
+    return launch(ctx, "fft_small", fft_small_kernel, dim3((uint32_t)cols.size()), dim3(64), 0, a);
+  }
+  const uint32_t *tree = inverse ? tw->d_itw : tw->d_tw;
+  const uint32_t *tw_end = tree + tw->len;
+  uint32_t L = inverse ? N : n;
+  std::vector<PassPlan> plan = plan_passes(N, L);
+  if (inverse) std::reverse(plan.begin(), plan.end());
+  // Output-side table (in place after the first forward pass).
+  void *dtab_out = dtab;
+  if (!inverse && plan.size() > 1) {
+    std::vector<FftCol> c2(cols);
+    for (auto &c : c2) c.in = c.out;
+    rc = upload_table(ctx, c2.data(), c2.size() * sizeof(FftCol), &dtab_out);
+    if (rc) return rc;
+  }
+  uint32_t n_inv = m31_inv((uint32_t)(((uint64_t)1 << N) % P));
+  for (size_t k = 0; k < plan.size(); k++) {
+    const PassPlan &pp = plan[k];
+    FftPass p{};
+    p.cols = (const FftCol *)((!inverse && k > 0) ? dtab_out : dtab);
+    p.tw_end = tw_end;
+    p.N = N;
+    p.lo = pp.lo;
+    p.m = pp.m;
+    p.w = pp.w;
+    p.s_begin = pp.s_begin;
+    p.s_end = pp.s_end;
+    p.in_log = (!inverse && k == 0) ? n : N;
+    p.inverse = inverse;
+    p.scale = (inverse && k + 1 == plan.size()) ? n_inv : 0;
+    uint32_t tiles = 1u << (N - pp.m - pp.w);
+    rc = launch(ctx, inverse ? "ifft_tile" : "rfft_tile", fft_tile_kernel, dim3(tiles, (uint32_t)cols.size()),
+                dim3(kFftThreads), 0, p);
+    if (rc) return rc;
+  }
+  return STWO_HIP_OK;
+}
+
+static int bit_reverse_one(stwo_hip_ctx *ctx, uint32_t *col, uint32_t log_size) {
+  if (log_size >= 2 * kBrLog) {
+    uint32_t mids = 1u << (log_size - 2 * kBrLog);
+    return launch(ctx, "bit_reverse_tiled", bit_reverse_tiled_kernel, dim3(mids), dim3(256), 0, col, log_size);
+  }
+  uint32_t n = 1u << log_size;
+  return launch(ctx, "bit_reverse_small", bit_reverse_small_kernel, dim3((n + 255) / 256), dim3(256), 0, col,
+                log_size);
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_bit_reverse_column(stwo_hip_ctx *ctx, uint32_t *col, uint32_t log_size) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, col != nullptr && log_size <= 31, "bit_reverse_column: bad column");
+  return bit_reverse_one(ctx, col, log_size);
+}
+
+int stwo_hip_bit_reverse_secure_column(stwo_hip_ctx *ctx, uint32_t *const coords[4], uint32_t log_size) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, coords != nullptr && log_size <= 31, "bit_reverse_secure_column: bad column");
+  for (int k = 0; k < 4; k++) {
+    STWO_HIP_CHECK_ARG(ctx, coords[k] != nullptr, "bit_reverse_secure_column: NULL coordinate");
+    int rc = bit_reverse_one(ctx, coords[k], log_size);
+    if (rc) return rc;
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_interpolate(stwo_hip_ctx *ctx, uint32_t *col, stwo_hip_coset domain, const stwo_hip_twiddles *tw) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, col != nullptr, "interpolate: NULL column");
+  int rc = check_domain(ctx, domain, tw);
+  if (rc) return rc;
+  uint32_t N = domain.log_size + 1;
+  STWO_HIP_CHECK_ARG(ctx, N <= 28, "interpolate: log size %u too large", N);
+  std::vector<FftCol> cols{{col, col}};
+  return run_fft_group(ctx, cols, N, N, tw, true, domain);
+}
+
+int stwo_hip_interpolate_columns(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                                 const stwo_hip_twiddles *tw) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, n_cols == 0 || (cols && log_sizes), "interpolate_columns: NULL arrays");
+  std::map<uint32_t, std::vector<FftCol>> groups;
+  for (size_t i = 0; i < n_cols; i++) {
+    STWO_HIP_CHECK_ARG(ctx, cols[i] != nullptr, "interpolate_columns: NULL column %zu", i);
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 1 && log_sizes[i] <= 28, "interpolate_columns: bad log size %u",
+                       log_sizes[i]);
+    groups[log_sizes[i]].push_back({cols[i], cols[i]});
+  }
+  for (auto &g : groups) {
+    uint32_t N = g.first;
+    stwo_hip_coset dom = {half_odds_initial(N - 1), N - 1};
+    int rc = check_domain(ctx, dom, tw);
+    if (rc) return rc;
+    for (size_t off = 0; off < g.second.size(); off += 65535) {
+      std::vector<FftCol> part(g.second.begin() + off,
+                               g.second.begin() + std::min(g.second.size(), off + (size_t)65535));
+      rc = run_fft_group(ctx, part, N, N, tw, true, dom);
+      if (rc) return rc;
+    }
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_evaluate(stwo_hip_ctx *ctx, const uint32_t *coeffs, uint32_t coeffs_log_size, stwo_hip_coset domain,
+                      const stwo_hip_twiddles *tw, uint32_t *out) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, coeffs != nullptr && out != nullptr, "evaluate: NULL buffer");
+  int rc = check_domain(ctx, domain, tw);
+  if (rc) return rc;
+  uint32_t N = domain.log_size + 1;
+  STWO_HIP_CHECK_ARG(ctx, coeffs_log_size <= N, "evaluate: can only evaluate on larger domains (%u > %u)",
+                     coeffs_log_size, N);
+  STWO_HIP_CHECK_ARG(ctx, N <= 28, "evaluate: log size %u too large", N);
+  std::vector<FftCol> cols{{coeffs, out}};
+  return run_fft_group(ctx, cols, coeffs_log_size, N, tw, false, domain);
+}
+
+int stwo_hip_evaluate_polynomials(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes,
+                                  size_t n_cols, uint32_t log_blowup, const stwo_hip_twiddles *tw,
+                                  uint32_t *const *outs) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, n_cols == 0 || (coeffs && log_sizes && outs), "evaluate_polynomials: NULL arrays");
+  std::map<uint32_t, std::vector<FftCol>> groups;
+  for (size_t i = 0; i < n_cols; i++) {
+    STWO_HIP_CHECK_ARG(ctx, coeffs[i] && outs[i], "evaluate_polynomials: NULL column %zu", i);
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] + log_blowup >= 1 && log_sizes[i] + log_blowup <= 28,
+                       "evaluate_polynomials: bad log size %u + %u", log_sizes[i], log_blowup);
+    groups[log_sizes[i]].push_back({coeffs[i], outs[i]});
+  }
+  for (auto &g : groups) {
+    uint32_t n = g.first, N = n + log_blowup;
+    stwo_hip_coset dom = {half_odds_initial(N - 1), N - 1};
+    int rc = check_domain(ctx, dom, tw);
+    if (rc) return rc;
+    for (size_t off = 0; off < g.second.size(); off += 65535) {
+      std::vector<FftCol> part(g.second.begin() + off,
+                               g.second.begin() + std::min(g.second.size(), off + (size_t)65535));
+      rc = run_fft_group(ctx, part, n, N, tw, false, dom);
+      if (rc) return rc;
+    }
+  }
+  return STWO_HIP_OK;
+}
+
+}  // extern "C"

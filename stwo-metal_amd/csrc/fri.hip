@@ -1,0 +1,384 @@
+// FRI folds (FriOps) and the FRI commitment driver (FriProver::commit).
+//
+// Fold factors come from the resident inverse twiddle tree, as SimdBackend
+// does (SIMD/fri.rs:32-172): for a circle domain of log size N the factor of
+// pair i is 1/p.y with p = domain.at(bitrev(2i, N)) -- the inverse circle
+// twiddle of pair i -- and for a line domain of log size n the factor of
+// pair i is 1/x with x = domain.at(bitrev(2i, n)) -- entry i of the inverse
+// tree level of that domain, itw[len - 2^n + i].  QM31 columns are 4
+// coordinate columns (SoA), so each lane's loads are coalesced.
+//
+// fold_line folds up to 4 steps per launch with the 2^step inputs of an
+// output held in VGPRs (alpha, alpha^2, ... per step, SIMD/fri.rs:52-111).
+#include <string.h>
+
+#include <vector>
+
+#include "blake2s.hpp"
+#include "common.hpp"
+#include "field.hpp"
+
+namespace stwo_hip {
+
+struct Qm31Cols {
+  uint32_t *c[4];
+};
+struct Qm31ColsConst {
+  const uint32_t *c[4];
+};
+
+__device__ __forceinline__ uint32_t circle_itwiddle(const uint32_t *itw_end, uint32_t N, uint32_t h) {
+  const uint32_t *l0 = itw_end - ((int64_t)1 << (N - 1));
+  uint32_t x = l0[2 * (h >> 2)], y = l0[2 * (h >> 2) + 1];
+  uint32_t v = (h & 2) ? x : y;
+  bool neg = ((h & 3) == 1) || ((h & 3) == 2);
+  return neg ? m31_neg(v) : v;
+}
+
+__global__ void __launch_bounds__(256) fold_circle_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
+                                                          uint32_t N, qm31 alpha, qm31 alpha_sq, uint32_t accumulate) {
+  const uint32_t n_out = 1u << (N - 1);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  const uint32_t iy = circle_itwiddle(itw_end, N, i);
+  qm31 f0, f1;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint2 v = reinterpret_cast<const uint2 *>(src.c[k])[i];
+    f0.v[k] = m31_add(v.x, v.y);
+    f1.v[k] = m31_mul(m31_sub(v.x, v.y), iy);
+  }
+  qm31 r = qm31_add(f0, qm31_mul(alpha, f1));
+  if (accumulate) {
+    qm31 d = qm31_make(dst.c[0][i], dst.c[1][i], dst.c[2][i], dst.c[3][i]);
+    r = qm31_add(qm31_mul(d, alpha_sq), r);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) dst.c[k][i] = r.v[k];
+}
+
+template <int STEP>
+__global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
+                                                        uint32_t n, qm31 alpha) {
+  constexpr int W = 1 << STEP;
+  const uint32_t n_out = 1u << (n - STEP);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  qm31 v[W];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t *s = src.c[k] + (size_t)i * W;
+#pragma unroll
+    for (int j = 0; j < W; j++) v[j].v[k] = s[j];
+  }
+  qm31 a = alpha;
+#pragma unroll
+  for (int layer = 0; layer < STEP; layer++) {
+    const int half = W >> (layer + 1);
+    const uint32_t *lvl = itw_end - ((int64_t)1 << (n - layer));
+#pragma unroll
+    for (int j = 0; j < half; j++) {
+      uint32_t xi = lvl[(size_t)i * half + j];
+      qm31 f0, f1;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        f0.v[k] = m31_add(v[2 * j].v[k], v[2 * j + 1].v[k]);
+        f1.v[k] = m31_mul(m31_sub(v[2 * j].v[k], v[2 * j + 1].v[k]), xi);
+      }
+      v[j] = qm31_add(f0, qm31_mul(a, f1));
+    }
+    a = qm31_mul(a, a);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) dst.c[k][i] = v[0].v[k];
+}
+
+static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
+  STWO_HIP_CHECK_ARG(ctx, tw != nullptr && tw->device == ctx->device, "twiddles missing or from another device");
+  STWO_HIP_CHECK_ARG(ctx, dom.log_size <= tw->root.log_size, "domain log %u exceeds twiddle root log %u",
+                     dom.log_size, tw->root.log_size);
+  uint32_t k = tw->root.log_size - dom.log_size;
+  uint32_t expect = (uint32_t)(((uint64_t)tw->root.initial_index << k) & IDX_MASK);
+  STWO_HIP_CHECK_ARG(ctx, (dom.initial_index & IDX_MASK) == expect,
+                     "domain (initial %u, log %u) is not a doubling of the twiddle root (initial %u, log %u)",
+                     dom.initial_index, dom.log_size, tw->root.initial_index, tw->root.log_size);
+  return STWO_HIP_OK;
+}
+
+static int fold_circle(stwo_hip_ctx *ctx, uint32_t *const dst[4], const uint32_t *const src[4], uint32_t N,
+                       const qm31 &alpha, const stwo_hip_twiddles *tw, bool accumulate) {
+  Qm31Cols d;
+  Qm31ColsConst s;
+  for (int k = 0; k < 4; k++) {
+    d.c[k] = dst[k];
+    s.c[k] = src[k];
+  }
+  uint32_t n_out = 1u << (N - 1);
+  return launch(ctx, accumulate ? "fold_circle_into_line" : "fold_circle_first_layer", fold_circle_kernel,
+                dim3((n_out + 255) / 256), dim3(256), 0, d, s, (const uint32_t *)(tw->d_itw + tw->len), N, alpha,
+                qm31_mul(alpha, alpha), (uint32_t)accumulate);
+}
+
+// fold_step folds of a line evaluation of log size n (domain checked by caller).
+static int fold_line_steps(stwo_hip_ctx *ctx, const uint32_t *const src[4], uint32_t n, qm31 alpha, uint32_t fold_step,
+                           const stwo_hip_twiddles *tw, uint32_t *const dst[4]) {
+  const uint32_t *itw_end = tw->d_itw + tw->len;
+  const uint32_t *cur[4] = {src[0], src[1], src[2], src[3]};
+  uint32_t iter = 0;
+  while (fold_step > 0) {
+    uint32_t step = std::min<uint32_t>(fold_step, 4);
+    uint32_t n_out_log = n - step;
+    uint32_t *out[4];
+    if (fold_step == step) {
+      for (int k = 0; k < 4; k++) out[k] = dst[k];
+    } else {
+      // Intermediate results alternate between two scratch buffers.
+      void *p = nullptr;
+      int rc = get_scratch(ctx, (iter & 1) ? "fold_line_b" : "fold_line_a", ((size_t)4 << n_out_log) * 4, &p);
+      if (rc) return rc;
+      for (int k = 0; k < 4; k++) out[k] = (uint32_t *)p + ((size_t)k << n_out_log);
+    }
+    Qm31Cols d;
+    Qm31ColsConst s;
+    for (int k = 0; k < 4; k++) {
+      d.c[k] = out[k];
+      s.c[k] = cur[k];
+    }
+    uint32_t blocks = ((1u << n_out_log) + 255) / 256;
+    int rc;
+    switch (step) {
+      case 1: rc = launch(ctx, "fold_line", fold_line_kernel<1>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
+      case 2: rc = launch(ctx, "fold_line", fold_line_kernel<2>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
+      case 3: rc = launch(ctx, "fold_line", fold_line_kernel<3>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
+      default: rc = launch(ctx, "fold_line", fold_line_kernel<4>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
+    }
+    if (rc) return rc;
+    for (uint32_t k = 0; k < step; k++) alpha = qm31_mul(alpha, alpha);
+    for (int k = 0; k < 4; k++) cur[k] = out[k];
+    n = n_out_log;
+    fold_step -= step;
+    iter++;
+  }
+  return STWO_HIP_OK;
+}
+
+// ---- host-side Blake2s channel (STWO/core/channel/blake2s.rs) -------------
+
+struct HostChannel {
+  uint8_t digest[32];
+  uint32_t n_draws;
+
+  void update(const uint8_t d[32]) {
+    memcpy(digest, d, 32);
+    n_draws = 0;
+  }
+  // MerkleChannel::mix_root: digest = H(digest || root), vcs_lifted/blake2_merkle.rs:49-55.
+  void mix_root(const uint32_t root[8]) {
+    b2_host h;
+    h.update(digest, 32);
+    h.update(root, 32);
+    uint8_t o[32];
+    h.finalize(o);
+    update(o);
+  }
+  // mix_felts, blake2s.rs:58-68.
+  void mix_felts(const uint32_t *felts4, size_t n) {
+    b2_host h;
+    h.update(digest, 32);
+    h.update(felts4, 16 * n);
+    uint8_t o[32];
+    h.finalize(o);
+    update(o);
+  }
+  // draw_u32s: H(digest || le32(n_draws) || 0), blake2s.rs:103-117.
+  void draw_u32s(uint32_t out[8]) {
+    uint8_t in[37];
+    memcpy(in, digest, 32);
+    memcpy(in + 32, &n_draws, 4);
+    in[36] = 0;
+    n_draws++;
+    b2_host h;
+    h.update(in, 37);
+    h.finalize((uint8_t *)out);
+  }
+  // draw_secure_felt via draw_base_felts (retry unless all words < 2P), blake2s.rs:36-51,85-88.
+  qm31 draw_secure_felt() {
+    uint32_t w[8];
+    for (;;) {
+      draw_u32s(w);
+      bool ok = true;
+      for (int i = 0; i < 8; i++) ok = ok && w[i] < 2 * P;
+      if (ok) break;
+    }
+    return qm31_make(b2_reduce_word(w[0]), b2_reduce_word(w[1]), b2_reduce_word(w[2]), b2_reduce_word(w[3]));
+  }
+};
+
+// LineEvaluation::interpolate on the host (STWO/prover/line.rs:54-99): the
+// last FRI layer has at most 2^(10 + 16) values and usually 2^1..2^4.
+static void host_line_interpolate(std::vector<qm31> &vals, uint32_t line_initial, uint32_t log_n) {
+  size_t n = (size_t)1 << log_n;
+  for (size_t i = 0; i < n; i++) {
+    size_t j = bitrev32((uint32_t)i, log_n);
+    if (j > i) std::swap(vals[i], vals[j]);
+  }
+  uint32_t init = line_initial, lg = log_n;
+  while (lg >= 1) {
+    size_t size = (size_t)1 << lg;
+    cpoint c = host_index_to_point(init), st = host_index_to_point(coset_step(lg));
+    for (size_t i = 0; i < size / 2; i++) {
+      uint32_t xinv = m31_inv(c.x);
+      for (size_t base = 0; base < n; base += size) {
+        qm31 &a = vals[base + i], &b = vals[base + i + size / 2];
+        qm31 s = qm31_add(a, b), d = qm31_mul_m31(qm31_sub(a, b), xinv);
+        a = s;
+        b = d;
+      }
+      c = cp_add(c, st);
+    }
+    init = (uint32_t)(((uint64_t)init * 2) & IDX_MASK);
+    lg--;
+  }
+  uint32_t inv = m31_inv((uint32_t)(n % P));
+  for (auto &v : vals) v = qm31_mul_m31(v, inv);
+}
+
+static inline size_t layer_off(uint32_t l) { return (((size_t)1 << l) - 1) * 8; }
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_fold_circle_into_line(stwo_hip_ctx *ctx, uint32_t *const dst[4], const uint32_t *const src[4],
+                                   stwo_hip_coset src_domain, const uint32_t alpha[4], const stwo_hip_twiddles *tw) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, dst && src && alpha, "fold_circle_into_line: NULL argument");
+  for (int k = 0; k < 4; k++) STWO_HIP_CHECK_ARG(ctx, dst[k] && src[k], "fold_circle_into_line: NULL coordinate");
+  int rc = check_line_domain(ctx, src_domain, tw);
+  if (rc) return rc;
+  uint32_t N = src_domain.log_size + 1;
+  STWO_HIP_CHECK_ARG(ctx, N >= 3, "fold_circle_into_line: domain of log size %u is below 3", N);
+  return fold_circle(ctx, dst, src, N, qm31_make(alpha[0], alpha[1], alpha[2], alpha[3]), tw, true);
+}
+
+int stwo_hip_fold_line(stwo_hip_ctx *ctx, const uint32_t *const src[4], stwo_hip_coset line_domain,
+                       const uint32_t alpha[4], uint32_t fold_step, const stwo_hip_twiddles *tw,
+                       uint32_t *const dst[4]) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, dst && src && alpha, "fold_line: NULL argument");
+  for (int k = 0; k < 4; k++) STWO_HIP_CHECK_ARG(ctx, dst[k] && src[k], "fold_line: NULL coordinate");
+  STWO_HIP_CHECK_ARG(ctx, fold_step >= 1 && fold_step <= line_domain.log_size,
+                     "fold_line: fold_step %u invalid for log size %u", fold_step, line_domain.log_size);
+  int rc = check_line_domain(ctx, line_domain, tw);
+  if (rc) return rc;
+  return fold_line_steps(ctx, src, line_domain.log_size, qm31_make(alpha[0], alpha[1], alpha[2], alpha[3]), fold_step,
+                         tw, dst);
+}
+
+int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                        stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
+                        const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws && column && roots && n_roots && last_layer_poly,
+                     "fri_commit: NULL argument");
+  const uint32_t last_log = cfg.log_last_layer_degree_bound + cfg.log_blowup_factor;
+  STWO_HIP_CHECK_ARG(ctx, cfg.line_fold_step >= 1, "Line fold step must be positive.");
+  STWO_HIP_CHECK_ARG(ctx, cfg.log_last_layer_degree_bound <= 10, "log_last_layer_degree_bound must be <= 10");
+  STWO_HIP_CHECK_ARG(ctx, cfg.log_blowup_factor >= 1 && cfg.log_blowup_factor <= 16, "log_blowup_factor not in 1..=16");
+  STWO_HIP_CHECK_ARG(ctx, log_size >= 3 && log_size - 1 >= last_log && log_size <= 28,
+                     "fri_commit: log size %u incompatible with last layer log %u", log_size, last_log);
+  stwo_hip_coset dom = {half_odds_initial(log_size - 1), log_size - 1};
+  int rc = check_line_domain(ctx, dom, tw);
+  if (rc) return rc;
+
+  HostChannel ch;
+  memcpy(ch.digest, channel_digest, 32);
+  ch.n_draws = *channel_n_draws;
+
+  // Scratch: one packed tree of the largest layer, and two layer buffers.
+  void *tree_p = nullptr, *lay_a = nullptr, *lay_b = nullptr;
+  rc = get_scratch(ctx, "fri_tree", layer_off(log_size + 1) * 4, &tree_p);
+  if (rc) return rc;
+  rc = get_scratch(ctx, "fri_layer_a", ((size_t)4 << (log_size - 1)) * 4, &lay_a);
+  if (rc) return rc;
+  rc = get_scratch(ctx, "fri_layer_b", ((size_t)4 << (log_size - 1)) * 4, &lay_b);
+  if (rc) return rc;
+  uint32_t *tree = (uint32_t *)tree_p;
+  uint32_t nr = 0;
+
+  auto commit_root = [&](const uint32_t *const cols[4], uint32_t lg) -> int {
+    uint32_t logs[4] = {lg, lg, lg, lg};
+    int r = stwo_hip_merkle_lifted_commit(ctx, cols, logs, 4, lg, tree, STWO_HIP_HASH_BLAKE2S);
+    if (r) return r;
+    r = stwo_hip_download(ctx, roots + 8 * nr, tree, 8);
+    if (r) return r;
+    ch.mix_root(roots + 8 * nr);
+    nr++;
+    return STWO_HIP_OK;
+  };
+
+  // First layer (fri.rs:176-183, 360-369).
+  rc = commit_root(column, log_size);
+  if (rc) return rc;
+  // Inner layers (fri.rs:188-262).
+  uint32_t lg = log_size - 1;
+  uint32_t line_init = half_odds_initial(lg);
+  uint32_t *bufs[2] = {(uint32_t *)lay_a, (uint32_t *)lay_b};
+  int cur_b = 0;
+  uint32_t *cur[4];
+  for (int k = 0; k < 4; k++) cur[k] = bufs[cur_b] + ((size_t)k << lg);
+  qm31 alpha = ch.draw_secure_felt();
+  rc = fold_circle(ctx, cur, column, log_size, alpha, tw, false);
+  if (rc) return rc;
+  if (lg != last_log) {
+    for (;;) {
+      bool last = !(lg > last_log + cfg.line_fold_step);
+      uint32_t step = last ? lg - last_log : cfg.line_fold_step;
+      rc = commit_root(cur, lg);
+      if (rc) return rc;
+      alpha = ch.draw_secure_felt();
+      uint32_t *nxt[4];
+      for (int k = 0; k < 4; k++) nxt[k] = bufs[cur_b ^ 1] + ((size_t)k << (lg - step));
+      rc = fold_line_steps(ctx, cur, lg, alpha, step, tw, nxt);
+      if (rc) return rc;
+      cur_b ^= 1;
+      for (int k = 0; k < 4; k++) cur[k] = nxt[k];
+      line_init = (uint32_t)(((uint64_t)line_init << step) & IDX_MASK);
+      lg -= step;
+      if (last) break;
+    }
+  }
+  // Last layer (fri.rs:274-291).
+  size_t n = (size_t)1 << lg;
+  std::vector<uint32_t> h(4 * n);
+  for (int k = 0; k < 4; k++) {
+    rc = stwo_hip_download(ctx, h.data() + k * n, cur[k], n);
+    if (rc) return rc;
+  }
+  std::vector<qm31> vals(n);
+  for (size_t i = 0; i < n; i++) vals[i] = qm31_make(h[i], h[n + i], h[2 * n + i], h[3 * n + i]);
+  host_line_interpolate(vals, line_init, lg);
+  // into_ordered_coefficients: bit reverse (line.rs:157-160)
+  for (size_t i = 0; i < n; i++) {
+    size_t j = bitrev32((uint32_t)i, lg);
+    if (j > i) std::swap(vals[i], vals[j]);
+  }
+  size_t bound = (size_t)1 << cfg.log_last_layer_degree_bound;
+  for (size_t i = bound; i < n; i++)
+    for (int k = 0; k < 4; k++)
+      if (vals[i].v[k]) return set_error(ctx, STWO_HIP_EDEGREE, "invalid degree");
+  // from_ordered_coefficients bit-reverses the kept coefficients (line.rs:167-170)
+  for (size_t i = 0; i < bound; i++) {
+    size_t j = bitrev32((uint32_t)i, cfg.log_last_layer_degree_bound);
+    for (int k = 0; k < 4; k++) last_layer_poly[4 * j + k] = vals[i].v[k];
+  }
+  ch.mix_felts(last_layer_poly, bound);
+  memcpy(channel_digest, ch.digest, 32);
+  *channel_n_draws = ch.n_draws;
+  *n_roots = nr;
+  return STWO_HIP_OK;
+}
+
+}  // extern "C"

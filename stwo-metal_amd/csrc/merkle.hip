@@ -1,0 +1,275 @@
+// Blake2s Merkle commitments: mixed-degree (MerkleOps::commit_on_layer,
+// MerkleProver::commit) and lifted (MerkleOpsLifted).
+//
+// One hash per lane.  A node's message is streamed in 64-byte blocks exactly
+// as the `blake2` crate buffers it (the last block, full or partial, is the
+// finalized one, `t` counts the bytes so far):
+//   mixed node i  = [prev[2i] || prev[2i+1]] || le32(c0[i]) || le32(c1[i]) || ...
+//                   (STWO/core/vcs/blake2_merkle.rs:17-38, SIMD/blake2s.rs:60-144)
+//   lifted leaf i = le32(c[lift(i, L - log c)]) for columns in ascending
+//                   length order (CPUB/merkle_lifted.rs:44-78)
+//   lifted node i = prev[2i] || prev[2i+1] (vcs_lifted/blake2_merkle.rs:18-27)
+// Column words of one block are 16 columns at the same row: lane i reads
+// column c at row i, so each load instruction is a coalesced run over the
+// wave.  Column pointers come from a per-call device table (uniform across
+// the wave -> scalar loads).
+//
+// Node layers without injected columns run in `merkle_nodes_kernel`, which
+// hashes up to 8 consecutive layers per launch: each 256-lane workgroup
+// hashes 256 nodes, keeps the digests in LDS and halves the active lanes per
+// layer, writing every layer to global memory.
+#include <algorithm>
+#include <vector>
+
+#include "blake2s.hpp"
+#include "common.hpp"
+#include "field.hpp"
+
+namespace stwo_hip {
+
+struct ColRef {
+  const uint32_t *p;
+  uint32_t shift;  // lifted: L - log_size(col); mixed: unused
+  uint32_t pad;
+};
+
+__device__ __forceinline__ void store_hash(uint32_t *dst, const b2_state &s, uint32_t m31) {
+  uint4 a, b;
+  if (m31) {
+    a = make_uint4(b2_reduce_word(s.h[0]), b2_reduce_word(s.h[1]), b2_reduce_word(s.h[2]), b2_reduce_word(s.h[3]));
+    b = make_uint4(b2_reduce_word(s.h[4]), b2_reduce_word(s.h[5]), b2_reduce_word(s.h[6]), b2_reduce_word(s.h[7]));
+  } else {
+    a = make_uint4(s.h[0], s.h[1], s.h[2], s.h[3]);
+    b = make_uint4(s.h[4], s.h[5], s.h[6], s.h[7]);
+  }
+  reinterpret_cast<uint4 *>(dst)[0] = a;
+  reinterpret_cast<uint4 *>(dst)[1] = b;
+}
+
+__device__ __forceinline__ void load_block64(uint32_t m[16], const uint32_t *src) {
+  const uint4 *s = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint4 v = s[k];
+    m[4 * k] = v.x;
+    m[4 * k + 1] = v.y;
+    m[4 * k + 2] = v.z;
+    m[4 * k + 3] = v.w;
+  }
+}
+
+// Mixed-degree layer: optional previous layer, then the layer's columns.
+// `lifted` selects the lifted leaf index for column words.
+__global__ void __launch_bounds__(256) merkle_layer_kernel(const uint32_t *prev, const ColRef *cols, uint32_t n_cols,
+                                                           uint32_t log_size, uint32_t *out, uint32_t m31,
+                                                           uint32_t lifted) {
+  const uint32_t n = 1u << log_size;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  b2_state s;
+  b2_init(s);
+  uint32_t m[16];
+  uint32_t t = 0;
+  if (prev) {
+    load_block64(m, prev + 16 * (size_t)i);
+    t = 64;
+    b2_compress(s, m, t, n_cols == 0);
+  } else if (n_cols == 0) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = 0;
+    b2_compress(s, m, 0, true);
+  }
+  for (uint32_t c0 = 0; c0 < n_cols; c0 += 16) {
+    const uint32_t cnt = min(16u, n_cols - c0);
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+      uint32_t v = 0;
+      if (k < cnt) {
+        const ColRef cr = cols[c0 + k];
+        uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
+        v = cr.p[idx];
+      }
+      m[k] = v;
+    }
+    t += 4 * cnt;
+    b2_compress(s, m, t, c0 + 16 >= n_cols);
+  }
+  store_hash(out + 8 * (size_t)i, s, m31);
+}
+
+// Hashes `n_layers` (<= 8) consecutive node layers below `prev` (of log size
+// out_log + 1).  Layer k (k = 0 first) has log size out_log - k and is
+// written at out_base + offset(out_log - k) in the packed layout.
+__global__ void __launch_bounds__(256) merkle_nodes_kernel(const uint32_t *prev, uint32_t *packed, uint32_t out_log,
+                                                           uint32_t n_layers, uint32_t m31) {
+  __shared__ uint32_t hs[256][9];
+  const uint32_t n = 1u << out_log;
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t m[16];
+  b2_state s;
+  if (i < n) {
+    load_block64(m, prev + 16 * (size_t)i);
+    b2_init(s);
+    b2_compress(s, m, 64, true);
+    if (m31)
+#pragma unroll
+      for (int k = 0; k < 8; k++) s.h[k] = b2_reduce_word(s.h[k]);
+    store_hash(packed + (((size_t)1 << out_log) - 1) * 8 + 8 * (size_t)i, s, 0);
+#pragma unroll
+    for (int k = 0; k < 8; k++) hs[threadIdx.x][k] = s.h[k];
+  }
+  uint32_t active = min(256u, n);
+  for (uint32_t layer = 1; layer < n_layers; layer++) {
+    __syncthreads();
+    active >>= 1;
+    uint32_t lg = out_log - layer;
+    uint32_t gi = blockIdx.x * (256u >> layer) + threadIdx.x;
+    b2_state s2;
+    bool work = threadIdx.x < active;
+    if (work) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        m[k] = hs[2 * threadIdx.x][k];
+        m[8 + k] = hs[2 * threadIdx.x + 1][k];
+      }
+      b2_init(s2);
+      b2_compress(s2, m, 64, true);
+      if (m31)
+#pragma unroll
+        for (int k = 0; k < 8; k++) s2.h[k] = b2_reduce_word(s2.h[k]);
+      store_hash(packed + (((size_t)1 << lg) - 1) * 8 + 8 * (size_t)gi, s2, 0);
+    }
+    __syncthreads();
+    if (work)
+#pragma unroll
+      for (int k = 0; k < 8; k++) hs[threadIdx.x][k] = s2.h[k];
+  }
+}
+
+static inline size_t layer_off(uint32_t l) { return (((size_t)1 << l) - 1) * 8; }
+
+static int launch_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const std::vector<ColRef> &cols, uint32_t log_size,
+                        uint32_t *out, int mode, bool lifted) {
+  void *dtab = nullptr;
+  if (!cols.empty()) {
+    int rc = upload_table(ctx, cols.data(), cols.size() * sizeof(ColRef), &dtab);
+    if (rc) return rc;
+  }
+  uint32_t n = 1u << log_size;
+  return launch(ctx, lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer", merkle_layer_kernel,
+                dim3((n + 255) / 256), dim3(256), 0, prev, (const ColRef *)dtab, (uint32_t)cols.size(), log_size, out,
+                (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31), (uint32_t)lifted);
+}
+
+// Hashes node layers top_log-1 .. 0 of a packed tree whose layer top_log is filled.
+static int hash_node_layers(stwo_hip_ctx *ctx, uint32_t *layers, uint32_t top_log, uint32_t stop_log, int mode) {
+  // layers [stop_log, top_log) are produced, from top_log - 1 down.
+  uint32_t l = top_log;
+  while (l > stop_log) {
+    uint32_t out_log = l - 1;
+    uint32_t n_layers = std::min<uint32_t>(8, out_log - stop_log + 1);
+    // within one workgroup of 256 outputs we can go down at most 8 layers
+    uint32_t n_out = 1u << out_log;
+    uint32_t blocks = (n_out + 255) / 256;
+    if (n_out < 256) n_layers = std::min<uint32_t>(n_layers, out_log + 1);
+    int rc = launch(ctx, "merkle_nodes", merkle_nodes_kernel, dim3(blocks), dim3(256), 0,
+                    (const uint32_t *)(layers + layer_off(l)), layers, out_log, n_layers,
+                    (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31));
+    if (rc) return rc;
+    l = out_log + 1 - n_layers;
+  }
+  return STWO_HIP_OK;
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_merkle_commit_on_layer(stwo_hip_ctx *ctx, uint32_t log_size, const uint32_t *prev,
+                                    const uint32_t *const *cols, size_t n_cols, uint32_t *out, int hash_mode) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, out != nullptr && log_size <= 30, "commit_on_layer: bad output");
+  STWO_HIP_CHECK_ARG(ctx, n_cols == 0 || cols != nullptr, "commit_on_layer: NULL column array");
+  std::vector<ColRef> refs(n_cols);
+  for (size_t c = 0; c < n_cols; c++) {
+    STWO_HIP_CHECK_ARG(ctx, cols[c] != nullptr, "commit_on_layer: NULL column %zu", c);
+    refs[c] = {cols[c], 0, 0};
+  }
+  return launch_layer(ctx, prev, refs, log_size, out, hash_mode, false);
+}
+
+int stwo_hip_merkle_commit(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                           uint32_t *layers, int hash_mode) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, layers != nullptr, "merkle_commit: NULL output");
+  if (n_cols == 0) return launch_layer(ctx, nullptr, {}, 0, layers, hash_mode, false);
+  STWO_HIP_CHECK_ARG(ctx, cols && log_sizes, "merkle_commit: NULL arrays");
+  uint32_t max_log = 0;
+  for (size_t c = 0; c < n_cols; c++) {
+    STWO_HIP_CHECK_ARG(ctx, cols[c] && log_sizes[c] <= 30, "merkle_commit: bad column %zu", c);
+    max_log = std::max(max_log, log_sizes[c]);
+  }
+  int l = (int)max_log;
+  while (l >= 0) {
+    std::vector<ColRef> refs;
+    for (size_t c = 0; c < n_cols; c++)  // input order == stable order among equal lengths
+      if (log_sizes[c] == (uint32_t)l) refs.push_back({cols[c], 0, 0});
+    const uint32_t *prev = (l == (int)max_log) ? nullptr : layers + layer_off(l + 1);
+    if (refs.empty() && prev) {
+      // Run of pure node layers down to the next injected layer (or the root).
+      int next = -1;
+      for (size_t c = 0; c < n_cols; c++)
+        if ((int)log_sizes[c] < l) next = std::max(next, (int)log_sizes[c]);
+      uint32_t stop = (uint32_t)(next + 1);
+      int rc = hash_node_layers(ctx, layers, (uint32_t)l + 1, stop, hash_mode);
+      if (rc) return rc;
+      l = (int)stop - 1;
+      continue;
+    }
+    int rc = launch_layer(ctx, prev, refs, (uint32_t)l, layers + layer_off((uint32_t)l), hash_mode, false);
+    if (rc) return rc;
+    l--;
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_merkle_lifted_build_leaves(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes,
+                                        size_t n_cols, uint32_t lifting_log_size, uint32_t *out, int hash_mode) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, out != nullptr, "lifted_build_leaves: NULL output");
+  if (n_cols == 0) return launch_layer(ctx, nullptr, {}, 0, out, hash_mode, true);
+  STWO_HIP_CHECK_ARG(ctx, cols && log_sizes && lifting_log_size <= 30, "lifted_build_leaves: bad arguments");
+  std::vector<size_t> order(n_cols);
+  for (size_t c = 0; c < n_cols; c++) {
+    order[c] = c;
+    STWO_HIP_CHECK_ARG(ctx, cols[c] != nullptr, "lifted_build_leaves: NULL column %zu", c);
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[c] >= 1, "A column must be of length >= 2.");
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[c] <= lifting_log_size, "column %zu longer than the lifting size", c);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return log_sizes[a] < log_sizes[b]; });
+  std::vector<ColRef> refs;
+  for (size_t c : order) refs.push_back({cols[c], lifting_log_size - log_sizes[c], 0});
+  return launch_layer(ctx, nullptr, refs, lifting_log_size, out, hash_mode, true);
+}
+
+int stwo_hip_merkle_lifted_build_next_layer(stwo_hip_ctx *ctx, const uint32_t *prev, uint32_t out_log_size,
+                                            uint32_t *out, int hash_mode) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, prev && out && out_log_size < 30, "lifted_build_next_layer: bad arguments");
+  return launch_layer(ctx, prev, {}, out_log_size, out, hash_mode, false);
+}
+
+int stwo_hip_merkle_lifted_commit(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes,
+                                  size_t n_cols, uint32_t lifting_log_size, uint32_t *layers, int hash_mode) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, layers != nullptr, "lifted_commit: NULL output");
+  if (n_cols == 0) return stwo_hip_merkle_lifted_build_leaves(ctx, cols, log_sizes, 0, 0, layers, hash_mode);
+  int rc = stwo_hip_merkle_lifted_build_leaves(ctx, cols, log_sizes, n_cols, lifting_log_size,
+                                               layers + layer_off(lifting_log_size), hash_mode);
+  if (rc) return rc;
+  return hash_node_layers(ctx, layers, lifting_log_size, 0, hash_mode);
+}
+
+}  // extern "C"

@@ -1,0 +1,322 @@
+// Context, device memory, table arena, profiling and seeded input fill.
+//
+// Replaces the reference's Obj-C runtime singleton and buffer entry points
+// (SH/runtime.m:476-916, SYS/src/metal.rs:4442-4448) with an explicit
+// per-device context that owns one HIP stream.
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "field.hpp"
+
+namespace stwo_hip {
+
+int set_error(stwo_hip_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->last_error = buf;
+  return code;
+}
+
+hipEvent_t take_event(stwo_hip_ctx *ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+static void give_event(stwo_hip_ctx *ctx, hipEvent_t e) {
+  if (e) ctx->event_pool.push_back(e);
+}
+
+int check_launch(stwo_hip_ctx *ctx, const char *name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(ctx, STWO_HIP_EDEVICE, "launch of %s failed: %s", name, hipGetErrorString(e));
+  return STWO_HIP_OK;
+}
+
+void prof_begin(stwo_hip_ctx *ctx, const char *name, hipEvent_t *start) {
+  (void)name;
+  *start = nullptr;
+  if (!ctx->prof) return;
+  *start = take_event(ctx);
+  if (*start) hipEventRecord(*start, ctx->stream);
+}
+
+void prof_end(stwo_hip_ctx *ctx, const char *name, hipEvent_t start) {
+  if (!ctx->prof || !start) return;
+  hipEvent_t stop = take_event(ctx);
+  if (!stop) return;
+  hipEventRecord(stop, ctx->stream);
+  ctx->prof_pending.push_back({name, start, stop});
+}
+
+static const size_t kArenaBytes = 8u << 20;
+
+static int arena_init(stwo_hip_ctx *ctx) {
+  STWO_HIP_TRY(ctx, hipHostMalloc((void **)&ctx->h_arena, kArenaBytes, hipHostMallocDefault));
+  STWO_HIP_TRY(ctx, hipMalloc((void **)&ctx->d_arena, kArenaBytes));
+  ctx->arena_cap = kArenaBytes;
+  ctx->arena_head = 0;
+  return STWO_HIP_OK;
+}
+
+// Makes host bytes [b, e) of the arena reusable.  Uploads complete in stream
+// (= FIFO) order, so waiting for the newest overlapping upload retires it and
+// every older one.  Completed uploads at the front are retired on the way.
+static int arena_wait_range(stwo_hip_ctx *ctx, size_t b, size_t e) {
+  size_t last = SIZE_MAX;
+  for (size_t i = 0; i < ctx->inflight.size(); i++) {
+    const auto &f = ctx->inflight[i];
+    if (f.begin < e && b < f.end) last = i;
+  }
+  if (last != SIZE_MAX) {
+    STWO_HIP_TRY(ctx, hipEventSynchronize(ctx->inflight[last].ev));
+    for (size_t i = 0; i <= last; i++) {
+      give_event(ctx, ctx->inflight.front().ev);
+      ctx->inflight.pop_front();
+    }
+  }
+  while (!ctx->inflight.empty() && hipEventQuery(ctx->inflight.front().ev) == hipSuccess) {
+    give_event(ctx, ctx->inflight.front().ev);
+    ctx->inflight.pop_front();
+  }
+  return STWO_HIP_OK;
+}
+
+int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr) {
+  size_t sz = (bytes + 255) & ~(size_t)255;
+  if (sz == 0) sz = 256;
+  if (sz > ctx->arena_cap) return set_error(ctx, STWO_HIP_EINVAL, "table of %zu bytes exceeds the arena", bytes);
+  size_t off = ctx->arena_head;
+  if (off + sz > ctx->arena_cap) off = 0;
+  int rc = arena_wait_range(ctx, off, off + sz);
+  if (rc) return rc;
+  memcpy(ctx->h_arena + off, src, bytes);
+  STWO_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_arena + off, ctx->h_arena + off, bytes, hipMemcpyHostToDevice, ctx->stream));
+  hipEvent_t ev = take_event(ctx);
+  if (!ev) return set_error(ctx, STWO_HIP_EDEVICE, "hipEventCreate failed");
+  STWO_HIP_TRY(ctx, hipEventRecord(ev, ctx->stream));
+  ctx->inflight.push_back({off, off + sz, ev});
+  ctx->arena_head = off + sz;
+  *dptr = ctx->d_arena + off;
+  return STWO_HIP_OK;
+}
+
+int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr) {
+  auto it = ctx->scratch.find(key);
+  if (it != ctx->scratch.end() && it->second.second >= bytes) {
+    *dptr = it->second.first;
+    return STWO_HIP_OK;
+  }
+  if (it != ctx->scratch.end()) {
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(it->second.first);
+    ctx->scratch.erase(it);
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 256) != hipSuccess)
+    return set_error(ctx, STWO_HIP_ENOMEM, "scratch '%s' of %zu bytes: out of device memory", key, bytes);
+  ctx->scratch[key] = {p, bytes};
+  *dptr = p;
+  return STWO_HIP_OK;
+}
+
+const stwo_hip_twiddles *check_twiddles(stwo_hip_ctx *ctx, const stwo_hip_twiddles *tw) {
+  if (!tw || tw->device != ctx->device) return nullptr;
+  return tw;
+}
+
+// ---- seeded synthetic inputs -------------------------------------------
+
+__device__ __forceinline__ uint64_t splitmix64_step(uint64_t s) {
+  uint64_t z = s + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) fill_random_kernel(uint32_t *out, uint64_t n, uint64_t seed, uint64_t stream) {
+  uint64_t base = (seed << 32) ^ (stream * 0xD1B54A32D192ED03ull);
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t z = splitmix64_step(base ^ (j * 0x9E3779B97F4A7C15ull));
+    out[j] = (uint32_t)(z % P);
+  }
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_ctx_create(int device, stwo_hip_ctx **out) {
+  if (!out) return STWO_HIP_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return STWO_HIP_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return STWO_HIP_EDEVICE;
+  stwo_hip_ctx *ctx = new (std::nothrow) stwo_hip_ctx();
+  if (!ctx) return STWO_HIP_ENOMEM;
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return STWO_HIP_EDEVICE;
+  }
+  if (arena_init(ctx) != STWO_HIP_OK) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return STWO_HIP_ENOMEM;
+  }
+  *out = ctx;
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_ctx_destroy(stwo_hip_ctx *ctx) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto &f : ctx->inflight) hipEventDestroy(f.ev);
+  for (auto &p : ctx->prof_pending) {
+    hipEventDestroy(p.start);
+    hipEventDestroy(p.stop);
+  }
+  for (auto e : ctx->event_pool) hipEventDestroy(e);
+  for (auto &kv : ctx->scratch) hipFree(kv.second.first);
+  if (ctx->h_arena) hipHostFree(ctx->h_arena);
+  if (ctx->d_arena) hipFree(ctx->d_arena);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_last_error(const stwo_hip_ctx *ctx, char *buf, size_t len) {
+  if (!ctx || !buf || !len) return STWO_HIP_EINVAL;
+  size_t n = std::min(len - 1, ctx->last_error.size());
+  memcpy(buf, ctx->last_error.data(), n);
+  buf[n] = 0;
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_sync(stwo_hip_ctx *ctx) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return STWO_HIP_OK;
+}
+
+void *stwo_hip_stream(stwo_hip_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int stwo_hip_alloc(stwo_hip_ctx *ctx, size_t n_words, uint32_t **out) {
+  if (!ctx || !out) return STWO_HIP_EINVAL;
+  *out = nullptr;
+  if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, STWO_HIP_EDEVICE, "hipSetDevice failed");
+  hipError_t e = hipMalloc((void **)out, (n_words ? n_words : 1) * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    hipGetLastError();
+    *out = nullptr;
+    return set_error(ctx, STWO_HIP_ENOMEM, "hipMalloc(%zu words): %s", n_words, hipGetErrorString(e));
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_free(stwo_hip_ctx *ctx, uint32_t *ptr) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  if (!ptr) return STWO_HIP_OK;
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  STWO_HIP_TRY(ctx, hipFree(ptr));
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_upload(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words) {
+  if (!ctx || (!dst && n_words) || (!src && n_words)) return STWO_HIP_EINVAL;
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  STWO_HIP_TRY(ctx, hipMemcpy(dst, src, n_words * 4, hipMemcpyHostToDevice));
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_download(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words) {
+  if (!ctx || (!dst && n_words) || (!src && n_words)) return STWO_HIP_EINVAL;
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  STWO_HIP_TRY(ctx, hipMemcpy(dst, src, n_words * 4, hipMemcpyDeviceToHost));
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_copy(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words) {
+  if (!ctx || (!dst && n_words) || (!src && n_words)) return STWO_HIP_EINVAL;
+  STWO_HIP_TRY(ctx, hipMemcpyAsync(dst, src, n_words * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_memset_zero(stwo_hip_ctx *ctx, uint32_t *dst, size_t n_words) {
+  if (!ctx || (!dst && n_words)) return STWO_HIP_EINVAL;
+  STWO_HIP_TRY(ctx, hipMemsetAsync(dst, 0, n_words * 4, ctx->stream));
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_fill_random(stwo_hip_ctx *ctx, uint32_t *dst, size_t n, uint64_t seed, uint64_t stream) {
+  if (!ctx || (!dst && n)) return STWO_HIP_EINVAL;
+  uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 65536);
+  return launch(ctx, "fill_random", fill_random_kernel, dim3((uint32_t)blocks), dim3(256), 0, dst, (uint64_t)n, seed,
+                stream);
+}
+
+int stwo_hip_prof_enable(stwo_hip_ctx *ctx, int enable) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  ctx->prof = enable != 0;
+  return STWO_HIP_OK;
+}
+
+static int prof_collect(stwo_hip_ctx *ctx) {
+  if (ctx->prof_pending.empty()) return STWO_HIP_OK;
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto &p : ctx->prof_pending) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, p.start, p.stop);
+    auto &t = ctx->prof_totals[p.name];
+    t.first += 1;
+    t.second += ms;
+    give_event(ctx, p.start);
+    give_event(ctx, p.stop);
+  }
+  ctx->prof_pending.clear();
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_prof_reset(stwo_hip_ctx *ctx) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  int rc = prof_collect(ctx);
+  ctx->prof_totals.clear();
+  return rc;
+}
+
+int stwo_hip_prof_read(stwo_hip_ctx *ctx, char *name_buf, size_t name_buf_len, uint64_t *counts, double *total_ms,
+                       size_t max_entries, size_t *n_entries) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  int rc = prof_collect(ctx);
+  if (rc) return rc;
+  size_t k = 0, pos = 0;
+  for (auto &kv : ctx->prof_totals) {
+    if (k >= max_entries) break;
+    size_t need = kv.first.size() + 1;
+    if (name_buf && pos + need <= name_buf_len) {
+      memcpy(name_buf + pos, kv.first.c_str(), need);
+      pos += need;
+    }
+    if (counts) counts[k] = kv.second.first;
+    if (total_ms) total_ms[k] = kv.second.second;
+    k++;
+  }
+  if (n_entries) *n_entries = k;
+  return STWO_HIP_OK;
+}
+
+}  // extern "C"

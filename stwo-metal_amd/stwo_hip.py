@@ -1,0 +1,329 @@
+"""Python binding of libstwo_hip (the C ABI in include/stwo_hip.h) via ctypes.
+
+The method names follow stwo's Backend traits (PolyOps::interpolate_columns,
+evaluate_polynomials, ColumnOps::bit_reverse_column,
+MerkleOps::commit_on_layer, MerkleOpsLifted::build_leaves,
+FriOps::fold_circle_into_line / fold_line, FriProver::commit) so tests read
+like the reference's own trait-level tests.  Device columns are `DeviceBuf`
+objects owning a device allocation; numpy uint32 arrays cross the host
+boundary.  There is no CPU fallback: if the library is missing or no GPU is
+visible, loading fails with an exception.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstwo_hip.so")
+P = (1 << 31) - 1
+
+HASH_BLAKE2S = 0
+HASH_BLAKE2S_M31 = 1
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+class StwoHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("stwo_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Coset(ctypes.Structure):
+    _fields_ = [("initial_index", ctypes.c_uint32), ("log_size", ctypes.c_uint32)]
+
+
+class FriConfig(ctypes.Structure):
+    _fields_ = [("log_blowup_factor", ctypes.c_uint32), ("log_last_layer_degree_bound", ctypes.c_uint32),
+                ("n_queries", ctypes.c_uint32), ("line_fold_step", ctypes.c_uint32)]
+
+
+def half_odds(n):
+    """Coset::half_odds(n) (STWO/core/circle.rs:328-330)."""
+    return Coset((1 << (29 - n)) & ((1 << 31) - 1), n)
+
+
+def canonic_domain(log_size):
+    """Half coset of CanonicCoset(log_size).circle_domain()."""
+    return half_odds(log_size - 1)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError("libstwo_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "stwo_hip_ctx_create": [ctypes.c_int, ctypes.POINTER(_vp)],
+            "stwo_hip_ctx_destroy": [_vp],
+            "stwo_hip_last_error": [_vp, ctypes.c_char_p, ctypes.c_size_t],
+            "stwo_hip_sync": [_vp],
+            "stwo_hip_alloc": [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)],
+            "stwo_hip_free": [_vp, _vp],
+            "stwo_hip_upload": [_vp, _vp, _vp, ctypes.c_size_t],
+            "stwo_hip_download": [_vp, _vp, _vp, ctypes.c_size_t],
+            "stwo_hip_copy": [_vp, _vp, _vp, ctypes.c_size_t],
+            "stwo_hip_memset_zero": [_vp, _vp, ctypes.c_size_t],
+            "stwo_hip_fill_random": [_vp, _vp, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64],
+            "stwo_hip_twiddles_create": [_vp, Coset, ctypes.POINTER(_vp)],
+            "stwo_hip_twiddles_destroy": [_vp, _vp],
+            "stwo_hip_twiddles_buffers": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_bit_reverse_column": [_vp, _vp, ctypes.c_uint32],
+            "stwo_hip_bit_reverse_secure_column": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32],
+            "stwo_hip_interpolate": [_vp, _vp, Coset, _vp],
+            "stwo_hip_interpolate_columns": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, _vp],
+            "stwo_hip_evaluate": [_vp, _vp, ctypes.c_uint32, Coset, _vp, _vp],
+            "stwo_hip_evaluate_polynomials": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
+                                              ctypes.POINTER(_vp)],
+            "stwo_hip_merkle_commit_on_layer": [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(_vp), ctypes.c_size_t, _vp,
+                                                ctypes.c_int],
+            "stwo_hip_merkle_commit": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, _vp, ctypes.c_int],
+            "stwo_hip_merkle_lifted_build_leaves": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32,
+                                                    _vp, ctypes.c_int],
+            "stwo_hip_merkle_lifted_build_next_layer": [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_int],
+            "stwo_hip_merkle_lifted_commit": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
+                                              ctypes.c_int],
+            "stwo_hip_fold_circle_into_line": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), Coset, _u32p, _vp],
+            "stwo_hip_fold_line": [_vp, ctypes.POINTER(_vp), Coset, _u32p, ctypes.c_uint32, _vp, ctypes.POINTER(_vp)],
+            "stwo_hip_fri_commit": [_vp, ctypes.c_char_p, _u32p, FriConfig, ctypes.POINTER(_vp), ctypes.c_uint32, _vp,
+                                    _u32p, _u32p, _u32p],
+            "stwo_hip_prof_enable": [_vp, ctypes.c_int],
+            "stwo_hip_prof_reset": [_vp],
+            "stwo_hip_prof_read": [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64),
+                                   ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.stwo_hip_stream.argtypes = [_vp]
+        L.stwo_hip_stream.restype = _vp
+        _lib = L
+    return _lib
+
+
+def _ptrs(bufs):
+    arr = (_vp * max(1, len(bufs)))()
+    for i, b in enumerate(bufs):
+        arr[i] = b.ptr if isinstance(b, DeviceBuf) else b
+    return arr
+
+
+class DeviceBuf:
+    """A device allocation of `n` u32 words owned by a Context."""
+
+    def __init__(self, ctx, n, ptr):
+        self.ctx, self.n, self.ptr = ctx, n, ptr
+
+    def download(self, count=None, offset=0):
+        count = self.n - offset if count is None else count
+        out = np.empty(count, dtype=np.uint32)
+        self.ctx._check(lib().stwo_hip_download(self.ctx.h, out.ctypes.data, self.ptr + 4 * offset, count))
+        return out
+
+    def upload(self, arr, offset=0):
+        a = np.ascontiguousarray(arr, dtype=np.uint32)
+        assert offset + len(a) <= self.n
+        self.ctx._check(lib().stwo_hip_upload(self.ctx.h, self.ptr + 4 * offset, a.ctypes.data, len(a)))
+
+    def at(self, offset_words):
+        """Raw device address of word `offset_words` (for sub-buffers)."""
+        return self.ptr + 4 * offset_words
+
+    def free(self):
+        if self.ptr:
+            self.ctx._check(lib().stwo_hip_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+
+class Twiddles:
+    def __init__(self, ctx, handle, root):
+        self.ctx, self.h, self.root = ctx, handle, root
+
+    def buffers(self):
+        t, it, n = _vp(), _vp(), ctypes.c_size_t()
+        lib().stwo_hip_twiddles_buffers(self.h, ctypes.byref(t), ctypes.byref(it), ctypes.byref(n))
+        return t.value, it.value, n.value
+
+    def download(self):
+        t, it, n = self.buffers()
+        tw = np.empty(n, dtype=np.uint32)
+        itw = np.empty(n, dtype=np.uint32)
+        self.ctx._check(lib().stwo_hip_download(self.ctx.h, tw.ctypes.data, t, n))
+        self.ctx._check(lib().stwo_hip_download(self.ctx.h, itw.ctypes.data, it, n))
+        return tw, itw
+
+    def destroy(self):
+        if self.h:
+            self.ctx._check(lib().stwo_hip_twiddles_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
+class Context:
+    """One device + one stream (stwo_hip_ctx)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        rc = lib().stwo_hip_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise StwoHipError(rc, "stwo_hip_ctx_create(%d) failed (no GPU?)" % device)
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            lib().stwo_hip_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def last_error(self):
+        buf = ctypes.create_string_buffer(1024)
+        lib().stwo_hip_last_error(self.h, buf, len(buf))
+        return buf.value.decode()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise StwoHipError(rc, self.last_error())
+
+    # -- memory ---------------------------------------------------------------
+    def alloc(self, n):
+        p = _vp()
+        self._check(lib().stwo_hip_alloc(self.h, n, ctypes.byref(p)))
+        return DeviceBuf(self, n, p.value)
+
+    def to_device(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.uint32)
+        b = self.alloc(len(a))
+        b.upload(a)
+        return b
+
+    def zeros(self, n):
+        b = self.alloc(n)
+        self._check(lib().stwo_hip_memset_zero(self.h, b.ptr, n))
+        return b
+
+    def fill_random(self, buf, n, seed, stream, offset=0):
+        self._check(lib().stwo_hip_fill_random(self.h, buf.ptr + 4 * offset, n, seed, stream))
+
+    def sync(self):
+        self._check(lib().stwo_hip_sync(self.h))
+
+    # -- twiddles -------------------------------------------------------------
+    def precompute_twiddles(self, root):
+        h = _vp()
+        self._check(lib().stwo_hip_twiddles_create(self.h, root, ctypes.byref(h)))
+        return Twiddles(self, h.value, root)
+
+    # -- ColumnOps --------------------------------------------------------------
+    def bit_reverse_column(self, col, log_size):
+        self._check(lib().stwo_hip_bit_reverse_column(self.h, col.ptr if isinstance(col, DeviceBuf) else col, log_size))
+
+    def bit_reverse_secure_column(self, coords, log_size):
+        self._check(lib().stwo_hip_bit_reverse_secure_column(self.h, _ptrs(coords), log_size))
+
+    # -- PolyOps ----------------------------------------------------------------
+    def interpolate(self, col, domain, tw):
+        self._check(lib().stwo_hip_interpolate(self.h, col.ptr if isinstance(col, DeviceBuf) else col, domain, tw.h))
+
+    def interpolate_columns(self, cols, log_sizes, tw):
+        ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
+        self._check(lib().stwo_hip_interpolate_columns(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols), tw.h))
+
+    def evaluate(self, coeffs, coeffs_log_size, domain, tw, out):
+        self._check(lib().stwo_hip_evaluate(self.h, coeffs.ptr if isinstance(coeffs, DeviceBuf) else coeffs,
+                                            coeffs_log_size, domain, tw.h,
+                                            out.ptr if isinstance(out, DeviceBuf) else out))
+
+    def evaluate_polynomials(self, coeffs, log_sizes, log_blowup, tw, outs):
+        ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
+        self._check(lib().stwo_hip_evaluate_polynomials(self.h, _ptrs(coeffs), ls.ctypes.data_as(_u32p), len(coeffs),
+                                                        log_blowup, tw.h, _ptrs(outs)))
+
+    # -- Merkle -----------------------------------------------------------------
+    def commit_on_layer(self, log_size, prev, cols, out, mode=HASH_BLAKE2S):
+        pp = (prev.ptr if isinstance(prev, DeviceBuf) else prev) if prev is not None else None
+        self._check(lib().stwo_hip_merkle_commit_on_layer(self.h, log_size, pp, _ptrs(cols), len(cols),
+                                                          out.ptr if isinstance(out, DeviceBuf) else out, mode))
+
+    def merkle_commit(self, cols, log_sizes, layers, mode=HASH_BLAKE2S):
+        ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
+        self._check(lib().stwo_hip_merkle_commit(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols),
+                                                 layers.ptr if isinstance(layers, DeviceBuf) else layers, mode))
+
+    def lifted_build_leaves(self, cols, log_sizes, lifting_log, out, mode=HASH_BLAKE2S):
+        ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
+        self._check(lib().stwo_hip_merkle_lifted_build_leaves(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p),
+                                                              len(cols), lifting_log,
+                                                              out.ptr if isinstance(out, DeviceBuf) else out, mode))
+
+    def lifted_build_next_layer(self, prev, out_log, out, mode=HASH_BLAKE2S):
+        self._check(lib().stwo_hip_merkle_lifted_build_next_layer(
+            self.h, prev.ptr if isinstance(prev, DeviceBuf) else prev, out_log,
+            out.ptr if isinstance(out, DeviceBuf) else out, mode))
+
+    def lifted_commit(self, cols, log_sizes, lifting_log, layers, mode=HASH_BLAKE2S):
+        ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
+        self._check(lib().stwo_hip_merkle_lifted_commit(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols),
+                                                        lifting_log,
+                                                        layers.ptr if isinstance(layers, DeviceBuf) else layers, mode))
+
+    # -- FRI --------------------------------------------------------------------
+    def fold_circle_into_line(self, dst4, src4, src_domain, alpha, tw):
+        a = np.ascontiguousarray(alpha, dtype=np.uint32)
+        self._check(lib().stwo_hip_fold_circle_into_line(self.h, _ptrs(dst4), _ptrs(src4), src_domain,
+                                                         a.ctypes.data_as(_u32p), tw.h))
+
+    def fold_line(self, src4, line_domain, alpha, fold_step, tw, dst4):
+        a = np.ascontiguousarray(alpha, dtype=np.uint32)
+        self._check(lib().stwo_hip_fold_line(self.h, _ptrs(src4), line_domain, a.ctypes.data_as(_u32p), fold_step,
+                                             tw.h, _ptrs(dst4)))
+
+    def fri_commit(self, digest: bytes, n_draws: int, config: FriConfig, column4, log_size, tw):
+        """FriProver::commit; returns (roots [n][8], last_layer_poly [k][4], digest, n_draws)."""
+        d = ctypes.create_string_buffer(bytes(digest), 32)
+        nd = np.array([n_draws], dtype=np.uint32)
+        roots = np.zeros(8 * (log_size + 1), dtype=np.uint32)
+        nr = np.zeros(1, dtype=np.uint32)
+        bound = 1 << config.log_last_layer_degree_bound
+        last = np.zeros(4 * bound, dtype=np.uint32)
+        self._check(lib().stwo_hip_fri_commit(self.h, d, nd.ctypes.data_as(_u32p), config, _ptrs(column4), log_size,
+                                              tw.h, roots.ctypes.data_as(_u32p), nr.ctypes.data_as(_u32p),
+                                              last.ctypes.data_as(_u32p)))
+        n = int(nr[0])
+        return roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), d.raw[:32], int(nd[0])
+
+    # -- profiling --------------------------------------------------------------
+    def prof_enable(self, on=True):
+        self._check(lib().stwo_hip_prof_enable(self.h, int(on)))
+
+    def prof_reset(self):
+        self._check(lib().stwo_hip_prof_reset(self.h))
+
+    def prof_read(self):
+        """{kernel name: (launches, total ms)} since the last reset."""
+        names = ctypes.create_string_buffer(8192)
+        counts = (ctypes.c_uint64 * 64)()
+        ms = (ctypes.c_double * 64)()
+        n = ctypes.c_size_t()
+        self._check(lib().stwo_hip_prof_read(self.h, names, len(names), counts, ms, 64, ctypes.byref(n)))
+        keys = names.raw.split(b"\0")[: n.value]
+        return {k.decode(): (int(counts[i]), float(ms[i])) for i, k in enumerate(keys)}
+
+
+def layer_offset(l):
+    """Word offset of layer l in the packed tree layout (root first)."""
+    return ((1 << l) - 1) * 8
+
+
+def tree_words(max_log):
+    return ((2 << max_log) - 1) * 8

@@ -1,0 +1,514 @@
+"""GPU parity: every hot-path entry point of libstwo_hip, called through the C
+ABI, against the pinned CPU oracle and the golden fixtures.  Bit-exact
+equality is the bar (all arithmetic is exact integer work).
+
+Shapes follow the reference's parity tests (crates/stwo-metal/tests/*, cited
+per test) plus the first benchmark workload (16 columns of 2^16 -> blowup 2),
+checked in full.  Inputs are seeded (SplitMix64, u64 % P); the reference's
+SmallRng vectors cannot be regenerated here.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
+import oracle  # noqa: E402
+import stwo_hip as sh  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+P = sh.P
+U = np.uint32
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sh.Context(0)
+    yield c
+    c.close()
+
+
+def arr(x):
+    return np.array(x, dtype=U)
+
+
+def rand(n, seed, stream):
+    return oracle.fill_random(n, seed, stream)
+
+
+# ---------------------------------------------------------------- twiddles --
+
+def test_twiddles_golden(ctx, golden):
+    """metal_twiddle_precompute_matches_cpu_on_{canonic,shifted}_cosets
+    (crates/stwo-metal/tests/metal_twiddle_precompute_native.rs:17-49)."""
+    for case in golden("twiddles"):
+        tw = ctx.precompute_twiddles(sh.Coset(case["initial"], case["log_size"]))
+        t, it = tw.download()
+        assert t.tolist() == case["twiddles"], case
+        assert it.tolist() == case["itwiddles"], case
+        tw.destroy()
+
+
+@pytest.mark.parametrize("log_size", [13, 21])
+def test_twiddles_large_vs_oracle(ctx, log_size):
+    root = sh.half_odds(log_size)
+    tw = ctx.precompute_twiddles(root)
+    t, it = tw.download()
+    et, eit = oracle.precompute_twiddles(root.initial_index, log_size)
+    assert np.array_equal(t, et) and np.array_equal(it, eit)
+    tw.destroy()
+
+
+def test_fill_random_matches_oracle(ctx):
+    b = ctx.alloc(1 << 12)
+    ctx.fill_random(b, 1 << 12, 7, 3)
+    assert np.array_equal(b.download(), rand(1 << 12, 7, 3))
+    b.free()
+
+
+# ------------------------------------------------------------- bit reverse --
+
+@pytest.mark.parametrize("log_size", [0, 1, 4, 10, 11, 12, 13, 16, 19])
+def test_bit_reverse_column(ctx, log_size):
+    """metal_base_field_bit_reverse_matches_cpu_backend (tests/metal_base_field_bit_reverse.rs:8): 0..N."""
+    n = 1 << log_size
+    v = np.arange(n, dtype=U)
+    d = ctx.to_device(v)
+    ctx.bit_reverse_column(d, log_size)
+    assert np.array_equal(d.download(), oracle.bit_reverse(v))
+    d.free()
+
+
+def test_bit_reverse_secure_column(ctx):
+    """metal_secure_field_bit_reverse_matches_cpu_backend (tests/metal_secure_field_bit_reverse.rs:8)."""
+    log_size = 10
+    cols = [rand(1 << log_size, 11, k) for k in range(4)]
+    ds = [ctx.to_device(c) for c in cols]
+    ctx.bit_reverse_secure_column(ds, log_size)
+    for c, d in zip(cols, ds):
+        assert np.array_equal(d.download(), oracle.bit_reverse(c))
+        d.free()
+
+
+# --------------------------------------------------------------------- FFT --
+
+def test_fft_golden(ctx, golden):
+    """Brute-force evaluations (log 1..10, blowups, non-canonic domains)."""
+    for case in golden("fft"):
+        N, n = case["log_size"], case["coeff_log"]
+        dom = sh.Coset(case["domain_initial"], N - 1)
+        tw = ctx.precompute_twiddles(dom)
+        co = ctx.to_device(arr(case["coeffs"]))
+        out = ctx.alloc(1 << N)
+        ctx.evaluate(co, n, dom, tw, out)
+        assert out.download().tolist() == case["evals"], (n, N)
+        if n == N:
+            ctx.interpolate(out, dom, tw)
+            assert out.download().tolist() == case["coeffs"], N
+        co.free()
+        out.free()
+        tw.destroy()
+
+
+@pytest.mark.parametrize("log_size", [3, 8, 11, 12, 13, 15, 17, 18, 19, 20])
+def test_interpolate_vs_oracle(ctx, log_size):
+    """Includes metal_ifft_large_column_regression's log 17 (tests/metal_rfft_ifft_native.rs:60)."""
+    root = sh.half_odds(log_size + 1)
+    tw = ctx.precompute_twiddles(root)
+    dom = sh.canonic_domain(log_size)
+    ev = rand(1 << log_size, 42, log_size)
+    d = ctx.to_device(ev)
+    ctx.interpolate(d, dom, tw)
+    _, itw = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    exp = oracle.interpolate(ev, dom.initial_index, log_size, itw)
+    assert np.array_equal(d.download(), exp)
+    d.free()
+    tw.destroy()
+
+
+@pytest.mark.parametrize("n,b", [(3, 0), (4, 1), (7, 1), (11, 1), (12, 1), (12, 2), (13, 1), (16, 2), (17, 1),
+                                 (18, 2), (20, 2)])
+def test_evaluate_vs_oracle(ctx, n, b):
+    N = n + b
+    root = sh.half_odds(N - 1)
+    tw = ctx.precompute_twiddles(root)
+    dom = sh.canonic_domain(N)
+    coeffs = rand(1 << n, 5, n * 10 + b)
+    co = ctx.to_device(coeffs)
+    out = ctx.alloc(1 << N)
+    ctx.evaluate(co, n, dom, tw, out)
+    et, _ = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    exp = oracle.evaluate(coeffs, dom.initial_index, N, et)
+    assert np.array_equal(out.download(), exp)
+    co.free()
+    out.free()
+    tw.destroy()
+
+
+def test_rfft_ifft_shifted_domain_round_trip(ctx):
+    """metal_rfft_and_ifft_match_cpu_on_shifted_domain (tests/metal_rfft_ifft_native.rs:21):
+    CircleDomain(half_odds(7)), twiddles on its half coset."""
+    dom = sh.half_odds(7)
+    tw = ctx.precompute_twiddles(dom)
+    coeffs = rand(1 << 8, 23, 0)
+    co = ctx.to_device(coeffs)
+    out = ctx.alloc(1 << 8)
+    ctx.evaluate(co, 8, dom, tw, out)
+    et, eit = oracle.precompute_twiddles(dom.initial_index, 7)
+    ev = oracle.evaluate(coeffs, dom.initial_index, 8, et)
+    assert np.array_equal(out.download(), ev)
+    ctx.interpolate(out, dom, tw)
+    assert np.array_equal(out.download(), coeffs)
+    co.free()
+    out.free()
+    tw.destroy()
+
+
+def test_edge_values_canonical(ctx):
+    """All-zero, all-(P-1) and alternating columns stay canonical and exact."""
+    n = 13
+    root = sh.half_odds(n)
+    tw = ctx.precompute_twiddles(root)
+    et, _ = oracle.precompute_twiddles(root.initial_index, n)
+    for col in [np.zeros(1 << n, U), np.full(1 << n, P - 1, U), np.tile(arr([0, P - 1]), 1 << (n - 1))]:
+        co = ctx.to_device(col)
+        out = ctx.alloc(1 << (n + 1))
+        ctx.evaluate(co, n, sh.canonic_domain(n + 1), tw, out)
+        got = out.download()
+        assert int(got.max()) < P
+        assert np.array_equal(got, oracle.evaluate(col, root.initial_index, n + 1, et))
+        co.free()
+        out.free()
+    tw.destroy()
+
+
+def test_batched_columns_mixed_sizes(ctx):
+    """interpolate_columns + evaluate_polynomials over columns of several sizes."""
+    logs = [3, 5, 9, 12, 12, 14, 16, 5, 13]
+    b = 2
+    root = sh.half_odds(max(logs) + b - 1)
+    tw = ctx.precompute_twiddles(root)
+    et, eit = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    evs = [rand(1 << l, 9, i) for i, l in enumerate(logs)]
+    ds = [ctx.to_device(e) for e in evs]
+    ctx.interpolate_columns(ds, logs, tw)
+    coeffs = [e.copy() for e in evs]
+    oracle.interpolate_columns(coeffs, eit)
+    for d, c in zip(ds, coeffs):
+        assert np.array_equal(d.download(), c)
+    outs = [ctx.alloc(1 << (l + b)) for l in logs]
+    ctx.evaluate_polynomials(ds, logs, b, tw, outs)
+    exp = oracle.evaluate_polynomials(coeffs, b, et)
+    for o, e in zip(outs, exp):
+        assert np.array_equal(o.download(), e)
+    for x in ds + outs:
+        x.free()
+    tw.destroy()
+
+
+def test_domain_not_in_tree_is_rejected(ctx):
+    """A domain that is not a doubling of the twiddle root is an EINVAL, not a launch."""
+    tw = ctx.precompute_twiddles(sh.half_odds(10))
+    d = ctx.alloc(1 << 8)
+    with pytest.raises(sh.StwoHipError) as e:
+        ctx.interpolate(d, sh.Coset(12345, 7), tw)
+    assert e.value.code == -1
+    with pytest.raises(sh.StwoHipError):
+        ctx.interpolate(d, sh.canonic_domain(13), tw)  # larger than the tree
+    d.free()
+    tw.destroy()
+
+
+# ------------------------------------------------------------------ Merkle --
+
+def test_merkle_golden(ctx, golden):
+    for case in golden("merkle"):
+        cols = [ctx.to_device(arr(c)) for c in case["cols"]]
+        shape = case["shape"]
+        L = max(shape)
+        layers = ctx.alloc(sh.tree_words(L))
+        ctx.merkle_commit(cols, shape, layers)
+        got = layers.download()
+        assert got[:8].tolist() == case["mixed_root"], shape
+        assert got[sh.layer_offset(L): sh.layer_offset(L + 1)].reshape(-1, 8).tolist() == case["mixed_leaf_layer"]
+        if case["lifted_root"] is not None:
+            ctx.lifted_commit(cols, shape, L, layers)
+            assert layers.download(8).tolist() == case["lifted_root"], shape
+        layers.free()
+        for c in cols:
+            c.free()
+
+
+@pytest.mark.parametrize("mode", [sh.HASH_BLAKE2S, sh.HASH_BLAKE2S_M31])
+def test_commit_on_layer_vs_oracle(ctx, mode):
+    for log_size, n_cols, with_prev in [(10, 0, True), (10, 1, False), (11, 16, False), (9, 17, True),
+                                        (12, 33, True), (3, 5, True), (0, 2, True)]:
+        cols = [rand(1 << log_size, 13, 100 * log_size + c) for c in range(n_cols)]
+        prev = rand(16 << log_size, 17, log_size) if with_prev else None
+        dcols = [ctx.to_device(c) for c in cols]
+        dprev = ctx.to_device(prev) if with_prev else None
+        out = ctx.alloc(8 << log_size)
+        ctx.commit_on_layer(log_size, dprev, dcols, out, mode)
+        exp = oracle.commit_on_layer(log_size, prev, cols, m31_output=bool(mode))
+        assert np.array_equal(out.download(), exp), (log_size, n_cols, with_prev)
+        for x in dcols + [out] + ([dprev] if dprev else []):
+            x.free()
+
+
+@pytest.mark.parametrize("mode", [sh.HASH_BLAKE2S, sh.HASH_BLAKE2S_M31])
+def test_mixed_degree_commit_vs_oracle(ctx, mode):
+    """MerkleProver::commit over columns injected at many layers (config-3 shape, small)."""
+    logs = [14, 3, 9, 14, 11, 11, 2, 7, 13, 9, 1, 14]
+    cols = [rand(1 << l, 19, i) for i, l in enumerate(logs)]
+    dcols = [ctx.to_device(c) for c in cols]
+    layers = ctx.alloc(sh.tree_words(max(logs)))
+    ctx.merkle_commit(dcols, logs, layers, mode)
+    exp = oracle.merkle_commit(cols, m31_output=bool(mode))
+    assert np.array_equal(layers.download(), exp)
+    layers.free()
+    for d in dcols:
+        d.free()
+
+
+def test_empty_commitments(ctx):
+    import hashlib
+    e = np.frombuffer(hashlib.blake2s(b"").digest(), dtype="<u4")
+    out = ctx.alloc(8)
+    ctx.merkle_commit([], [], out)
+    assert np.array_equal(out.download(), e)
+    ctx.lifted_build_leaves([], [], 0, out)
+    assert np.array_equal(out.download(), e)
+    out.free()
+
+
+@pytest.mark.parametrize("mode", [sh.HASH_BLAKE2S, sh.HASH_BLAKE2S_M31])
+def test_lifted_vs_oracle(ctx, mode):
+    """metal_blake2s_merkle_* (tests/metal_blake2s_channel_cpu_bridge.rs:56-110): lifted leaves
+    of uneven columns, 32 equal columns, and a large next layer."""
+    for logs, L in [([3, 4, 4], 4), ([8] * 32, 8), ([12, 5, 9, 12, 1, 7] * 3, 13)]:
+        cols = [rand(1 << l, 29, 50 * i + l) for i, l in enumerate(logs)]
+        dcols = [ctx.to_device(c) for c in cols]
+        layers = ctx.alloc(sh.tree_words(L))
+        ctx.lifted_commit(dcols, logs, L, layers, mode)
+        exp = oracle.lifted_commit(cols, L, m31_output=bool(mode))
+        assert np.array_equal(layers.download(), exp), logs
+        layers.free()
+        for d in dcols:
+            d.free()
+    prev = rand(8 << 12, 31, 0)
+    dp = ctx.to_device(prev)
+    out = ctx.alloc(8 << 11)
+    ctx.lifted_build_next_layer(dp, 11, out, mode)
+    exp = np.zeros(8 << 11, dtype=U)
+    oracle.lib().or_lifted_build_next_layer(oracle._p(prev), 11, oracle._p(exp), int(mode))
+    assert np.array_equal(out.download(), exp)
+    dp.free()
+    out.free()
+
+
+def test_lifted_reference_kat(ctx):
+    """test_lifted_merkle_leaves (STWO/prover/vcs_lifted/prover.rs:247-275)."""
+    import hashlib
+    cols = [np.arange(1 << i, dtype=U) for i in range(2, 5)]
+    d = [ctx.to_device(c) for c in cols]
+    out = ctx.alloc(8 << 4)
+    ctx.lifted_build_leaves(d, [2, 3, 4], 4, out)
+    leaves = out.download().reshape(-1, 8)
+
+    def hw(vals):
+        return np.frombuffer(hashlib.blake2s(b"".join(int(v).to_bytes(4, "little") for v in vals)).digest(), "<u4")
+
+    assert np.array_equal(leaves[0], hw([0, 0, 0]))
+    assert np.array_equal(leaves[4], hw([0, 2, 4]))
+    assert np.array_equal(leaves[15], hw([3, 7, 15]))
+    out.free()
+    for x in d:
+        x.free()
+
+
+# -------------------------------------------------------------------- folds --
+
+def test_folds_golden(ctx, golden):
+    g = golden("folds")
+    for case in g["circle"]:
+        N = case["log_size"]
+        if N < 3:
+            continue
+        tw = ctx.precompute_twiddles(sh.half_odds(N - 1))
+        src = [ctx.to_device(arr([v[k] for v in case["src"]])) for k in range(4)]
+        dst = [ctx.to_device(arr([v[k] for v in case["prev"]])) for k in range(4)]
+        ctx.fold_circle_into_line(dst, src, sh.half_odds(N - 1), case["alpha"], tw)
+        got = np.stack([d.download() for d in dst], axis=1).tolist()
+        assert got == case["dst"], N
+        for x in src + dst:
+            x.free()
+        tw.destroy()
+    for case in g["line"]:
+        n, step = case["log_size"], case["fold_step"]
+        tw = ctx.precompute_twiddles(sh.Coset(case["line_initial"], n))
+        src = [ctx.to_device(arr([v[k] for v in case["src"]])) for k in range(4)]
+        dst = [ctx.alloc(1 << (n - step)) for _ in range(4)]
+        ctx.fold_line(src, sh.Coset(case["line_initial"], n), case["alpha"], step, tw, dst)
+        got = np.stack([d.download() for d in dst], axis=1).tolist()
+        assert got == case["dst"], (n, step)
+        for x in src + dst:
+            x.free()
+        tw.destroy()
+
+
+def test_fri_first_layer_fold_reference_shape(ctx):
+    """metal_fri_first_layer_fold_matches_cpu_backend (tests/metal_fri_first_layer_fold.rs:17):
+    log 10 canonic, alpha = (1, 3, 5, 7), values 4i + k."""
+    N = 10
+    tw = ctx.precompute_twiddles(sh.half_odds(N - 1))
+    src = [np.array([4 * i + k for i in range(1 << N)], dtype=U) for k in range(4)]
+    dsrc = [ctx.to_device(s) for s in src]
+    dst = [ctx.zeros(1 << (N - 1)) for _ in range(4)]
+    ctx.fold_circle_into_line(dst, dsrc, sh.half_odds(N - 1), [1, 3, 5, 7], tw)
+    exp = [np.zeros(1 << (N - 1), U) for _ in range(4)]
+    oracle.fold_circle_into_line(exp, src, oracle.half_odds_initial(N - 1), N, [1, 3, 5, 7])
+    for d, e in zip(dst, exp):
+        assert np.array_equal(d.download(), e)
+    for x in dsrc + dst:
+        x.free()
+    tw.destroy()
+
+
+@pytest.mark.parametrize("n,step", [(9, 2), (12, 1), (16, 3), (14, 4), (13, 6), (20, 1)])
+def test_fold_line_vs_oracle(ctx, n, step):
+    """Includes metal_line_fold_matches_cpu_backend's shape (tests/metal_line_fold.rs:12): log 9,
+    fold_step 2, alpha (2, 4, 6, 8), values 8i + k."""
+    dom = sh.half_odds(n)
+    tw = ctx.precompute_twiddles(dom)
+    if n == 9:
+        src = [np.array([8 * i + k for i in range(1 << n)], dtype=U) for k in range(4)]
+        alpha = [2, 4, 6, 8]
+    else:
+        src = [rand(1 << n, 37, 10 * n + k) for k in range(4)]
+        alpha = [int(x) for x in rand(4, 41, n)]
+    dsrc = [ctx.to_device(s) for s in src]
+    dst = [ctx.alloc(1 << (n - step)) for _ in range(4)]
+    ctx.fold_line(dsrc, dom, alpha, step, tw, dst)
+    exp = oracle.fold_line(src, dom.initial_index, n, alpha, step)
+    for d, e in zip(dst, exp):
+        assert np.array_equal(d.download(), e)
+    for x in dsrc + dst:
+        x.free()
+    tw.destroy()
+
+
+@pytest.mark.parametrize("N", [3, 11, 18])
+def test_fold_circle_accumulate_vs_oracle(ctx, N):
+    tw = ctx.precompute_twiddles(sh.half_odds(N - 1))
+    src = [rand(1 << N, 43, 10 * N + k) for k in range(4)]
+    prev = [rand(1 << (N - 1), 47, 10 * N + k) for k in range(4)]
+    alpha = [int(x) for x in rand(4, 53, N)]
+    dsrc = [ctx.to_device(s) for s in src]
+    dst = [ctx.to_device(p) for p in prev]
+    ctx.fold_circle_into_line(dst, dsrc, sh.half_odds(N - 1), alpha, tw)
+    exp = [p.copy() for p in prev]
+    oracle.fold_circle_into_line(exp, src, oracle.half_odds_initial(N - 1), N, alpha)
+    for d, e in zip(dst, exp):
+        assert np.array_equal(d.download(), e)
+    for x in dsrc + dst:
+        x.free()
+    tw.destroy()
+
+
+# ---------------------------------------------------------------------- FRI --
+
+def test_fri_commit_golden(ctx, golden):
+    """FriProver::commit transcripts; includes metal_bounded_fri_prover's config
+    (tests/metal_fri_prover.rs:24-26: log 10, FriConfig::new(3, 2, 3, 2))."""
+    for case in golden("fri"):
+        lb, lld, nq, step = case["config"]
+        n = case["log_size"]
+        tw = ctx.precompute_twiddles(sh.half_odds(n - 1))
+        col = [ctx.to_device(arr([v[k] for v in case["evals"]])) for k in range(4)]
+        ch = oracle.Channel()
+        ch.mix_root(arr([int.from_bytes(bytes(case["prior_root"][i:i + 4]), "little") for i in range(0, 32, 4)]))
+        roots, last, digest, nd = ctx.fri_commit(ch.digest_bytes(), ch.n_draws, sh.FriConfig(lb, lld, nq, step), col,
+                                                  n, tw)
+        assert roots.tolist() == case["roots"], n
+        assert last.tolist() == case["last_layer_poly"], n
+        assert list(digest) == case["final_digest"], n
+        for x in col:
+            x.free()
+        tw.destroy()
+
+
+def _low_degree_qm31_eval(ctx, log_size, log_blowup, seed, tw):
+    """4 coordinate columns: random polys of degree < 2^(log_size - log_blowup) evaluated on CanonicCoset(log_size)."""
+    n = log_size - log_blowup
+    cols = []
+    for k in range(4):
+        co = ctx.alloc(1 << n)
+        ctx.fill_random(co, 1 << n, seed, k)
+        out = ctx.alloc(1 << log_size)
+        ctx.evaluate(co, n, sh.canonic_domain(log_size), tw, out)
+        co.free()
+        cols.append(out)
+    return cols
+
+
+@pytest.mark.parametrize("log_size,cfg", [(14, (1, 3, 3, 1)), (16, (2, 0, 3, 3)), (12, (1, 2, 3, 2))])
+def test_fri_commit_vs_oracle(ctx, log_size, cfg):
+    lb, lld, nq, step = cfg
+    tw = ctx.precompute_twiddles(sh.half_odds(log_size - 1))
+    col = _low_degree_qm31_eval(ctx, log_size, lb, 61, tw)
+    host = [c.download() for c in col]
+    ch = oracle.Channel()
+    ch.mix_u64(0xDEADBEEF)
+    roots, last, digest, nd = ctx.fri_commit(ch.digest_bytes(), ch.n_draws, sh.FriConfig(lb, lld, nq, step), col,
+                                              log_size, tw)
+    eroots, elast, _ = oracle.fri_commit(ch, oracle.FriConfig(lb, lld, nq, step), host, log_size)
+    assert np.array_equal(roots, eroots)
+    assert np.array_equal(last, elast)
+    assert digest == ch.digest_bytes() and nd == ch.n_draws
+    for c in col:
+        c.free()
+    tw.destroy()
+
+
+def test_fri_rejects_high_degree(ctx):
+    n = 10
+    tw = ctx.precompute_twiddles(sh.half_odds(n - 1))
+    col = [ctx.to_device(rand(1 << n, 67, k)) for k in range(4)]
+    with pytest.raises(sh.StwoHipError) as e:
+        ctx.fri_commit(bytes(32), 0, sh.FriConfig(1, 1, 3, 1), col, n, tw)
+    assert e.value.code == -4
+    for c in col:
+        c.free()
+    tw.destroy()
+
+
+# ------------------------------------------------- first benchmark workload --
+
+def test_config1_lde_merkle_full(ctx):
+    """16 columns of 2^16 (seed 1): interpolate, evaluate at blowup 2 on
+    CanonicCoset(17), mixed-degree Blake2s commit; every evaluation and every
+    tree layer compared with the oracle."""
+    n_cols, n, b = 16, 16, 1
+    root = sh.half_odds(n + b - 1)
+    tw = ctx.precompute_twiddles(root)
+    et, eit = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    evs = [rand(1 << n, 1, c) for c in range(n_cols)]
+    d = [ctx.to_device(e) for e in evs]
+    ctx.interpolate_columns(d, [n] * n_cols, tw)
+    outs = [ctx.alloc(1 << (n + b)) for _ in range(n_cols)]
+    ctx.evaluate_polynomials(d, [n] * n_cols, b, tw, outs)
+    layers = ctx.alloc(sh.tree_words(n + b))
+    ctx.merkle_commit(outs, [n + b] * n_cols, layers)
+    coeffs = [e.copy() for e in evs]
+    oracle.interpolate_columns(coeffs, eit)
+    exp_evals = oracle.evaluate_polynomials(coeffs, b, et)
+    for dd, c in zip(d, coeffs):
+        assert np.array_equal(dd.download(), c)
+    for o, e in zip(outs, exp_evals):
+        assert np.array_equal(o.download(), e)
+    exp_layers = oracle.merkle_commit(exp_evals)
+    assert np.array_equal(layers.download(), exp_layers)
+    for x in d + outs + [layers]:
+        x.free()
+    tw.destroy()
