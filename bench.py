@@ -72,25 +72,108 @@ def fri_chain(ctx, tw):
     return best * 1e3, int(roots.shape[0])
 
 
-def cpu_baseline(sample_cols=16):
-    """The oracle (C, OpenMP over columns/nodes) on a sample of the same workload."""
+def uneven_logs(n_cols=1024, lo=10, hi=22, seed=3):
+    """log_size = 10 + j, P(j) proportional to 0.7^j, j in 0..12 (seeded)."""
+    rng = np.random.default_rng(seed)
+    w = np.array([0.7 ** j for j in range(hi - lo + 1)])
+    return [int(lo + j) for j in rng.choice(len(w), size=n_cols, p=w / w.sum())]
+
+
+def batched_vs_plain(ctx, steps=3):
+    """1,024 uneven columns (log 10..22), LDE at blowup 2, one mixed-degree tree:
+    one batched plan replayed as a hipGraph vs per-column dispatch."""
+    logs = uneven_logs()
+    b = 1
+    max_log = max(logs) + b
+    tw = ctx.precompute_twiddles(sh.half_odds(max_log - 1))
+    offs, acc = [], 0
+    for l in logs:
+        offs.append(acc)
+        acc += 1 << l
+    oouts, oacc = [], 0
+    for l in logs:
+        oouts.append(oacc)
+        oacc += 1 << (l + b)
+    pristine = ctx.alloc(acc)
+    ctx.fill_random(pristine, acc, 3, 0)
+    work = ctx.alloc(acc)
+    outs = ctx.alloc(oacc)
+    layers = ctx.alloc(sh.tree_words(max_log))
+    cols = [work.at(o) for o in offs]
+    out_cols = [outs.at(o) for o in oouts]
+    elems = acc
+
+    def restore():
+        ctx._check(sh.lib().stwo_hip_copy(ctx.h, work.ptr, pristine.ptr, acc))
+        ctx.sync()
+
+    # plain: per-column interpolate + evaluate, then the mixed-degree commit
+    plain = []
+    for it in range(steps + 1):
+        restore()
+        t0 = time.perf_counter()
+        for c, o, l in zip(cols, out_cols, logs):
+            ctx.interpolate(c, sh.canonic_domain(l), tw)
+            ctx.evaluate(c, l, sh.canonic_domain(l + b), tw, o)
+        ctx.merkle_commit(out_cols, [l + b for l in logs], layers)
+        ctx.sync()
+        if it:
+            plain.append(time.perf_counter() - t0)
+    root_plain = layers.download(8)
+    plan = ctx.batch_create(cols, logs, b, tw, out_cols, layers, sh.TREE_MIXED, sh.HASH_BLAKE2S, True)
+    batched = []
+    for it in range(steps + 1):
+        restore()
+        t0 = time.perf_counter()
+        plan.run()
+        ctx.sync()
+        if it:
+            batched.append(time.perf_counter() - t0)
+    root_batched = layers.download(8)
+    plan.destroy()
+    for x in (pristine, work, outs, layers):
+        x.free()
+    tw.destroy()
+    hist = {str(l): logs.count(l) for l in sorted(set(logs))}
+    tp, tb = min(plain), min(batched)
+    return {
+        "workload": "1,024 M31 columns, log_size 10..22 (P(10+j) ~ 0.7^j, seed 3), LDE blowup 2, one "
+                    "mixed-degree Blake2s tree",
+        "input_elems": elems,
+        "log_size_histogram": hist,
+        "plain_ms": round(tp * 1e3, 3),
+        "batched_ms": round(tb * 1e3, 3),
+        "plain_melem_s": round(elems / tp / 1e6, 1),
+        "batched_melem_s": round(elems / tb / 1e6, 1),
+        "batched_over_plain": round(tp / tb, 2),
+        "roots_equal": bool(np.array_equal(root_plain, root_batched)),
+    }
+
+
+def cpu_baseline(sample_cols=N_COLS, runs=3):
+    """The oracle (C, OpenMP over columns and nodes) on the same workload, on the host cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
     init = oracle.half_odds_initial(LOG_N + LOG_BLOWUP - 1)
     tw, itw = oracle.precompute_twiddles(init, LOG_N + LOG_BLOWUP - 1)
-    evs = [oracle.fill_random(1 << LOG_N, 1, c) for c in range(sample_cols)]
-    t0 = time.perf_counter()
-    oracle.interpolate_columns(evs, itw)
-    outs = oracle.evaluate_polynomials(evs, LOG_BLOWUP, tw)
-    oracle.merkle_commit(outs)
-    dt = time.perf_counter() - t0
+    pristine = [oracle.fill_random(1 << LOG_N, 2, c) for c in range(sample_cols)]
+    times = []
+    for _ in range(runs):
+        evs = [p.copy() for p in pristine]
+        t0 = time.perf_counter()
+        oracle.interpolate_columns(evs, itw)
+        outs = oracle.evaluate_polynomials(evs, LOG_BLOWUP, tw)
+        oracle.merkle_commit(outs)
+        times.append(time.perf_counter() - t0)
+        del outs
+    dt = sorted(times)[len(times) // 2]
     return {
         "value": round(sample_cols * (1 << LOG_N) / dt / 1e6, 3),
         "unit": "Melem/s",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": "%d of the 256 columns (2^20 -> blowup 4 -> 2^22) + Blake2s Merkle commit of those columns; "
-                  "%.2f s on the host" % (sample_cols, dt),
+        "sample": "%d columns x 2^20 -> blowup 4 -> Blake2s Merkle commit (the full workload), median of %d "
+                  "runs, %.2f s each" % (sample_cols, runs, dt),
     }
 
 
@@ -101,6 +184,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fri", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
@@ -157,6 +241,11 @@ def main():
             kernels[k]["algorithmic_GBps"] = round(alg[k] / (kernels[k]["ms_total"] / args.steps * 1e-3) / 1e9, 1)
     log("kernels: %s" % json.dumps(kernels))
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_vs_plain(ctx)
+        log("batched vs plain: %s" % json.dumps(batched))
+
     fri_ms, fri_layers = (None, None)
     if not args.no_fri:
         fri_ms, fri_layers = fri_chain(ctx, tw)
@@ -203,6 +292,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "batched_uneven": batched,
         "fri_chain_ms": round(fri_ms, 3) if fri_ms is not None else None,
         "fri_committed_layers": fri_layers,
     }

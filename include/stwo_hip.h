@@ -212,6 +212,32 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
                         stwo_hip_fri_config config, const uint32_t *const column[4], uint32_t log_size,
                         const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly);
 
+/* ---- batched commitment of uneven columns -------------------------------
+ * One plan for a whole set of columns of different log sizes, replacing the
+ * per-column dispatch of the reference's TreeBuilder path
+ * (TreeBuilder::extend_evals -> interpolate_columns, STWO/prover/pcs/mod.rs:707-716;
+ * CommitmentTreeProver::new -> evaluate_polynomials + commit, :766-803;
+ * Metal MB/poly.rs:658-793 and MB/prove_runtime_v1.rs:1181-1240):
+ *   cols[i]  (2^log_sizes[i] bit-reversed evaluations on CanonicCoset(log))
+ *            -> interpolated in place to coefficients;
+ *   outs[i]  <- evaluations on CanonicCoset(log + log_blowup);
+ *   layers   <- packed tree (max layer = max log + log_blowup) of the outs:
+ *               STWO_HIP_TREE_MIXED  = MerkleProver::commit (each size at its layer),
+ *               STWO_HIP_TREE_LIFTED = MerkleProverLifted::commit at the max size.
+ * Columns must have log size >= 4.  Per-call tables are uploaded once; the
+ * plan's launches (one per FFT pass level for all sizes, one per tree layer)
+ * are captured into a hipGraph when use_graph != 0.  stwo_hip_batch_run
+ * enqueues the whole sequence on the context stream; the buffers named at
+ * creation must stay valid for the plan's lifetime. */
+#define STWO_HIP_TREE_MIXED 0
+#define STWO_HIP_TREE_LIFTED 1
+typedef struct stwo_hip_batch stwo_hip_batch;
+int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                          uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
+                          int tree_kind, int hash_mode, int use_graph, stwo_hip_batch **out);
+int stwo_hip_batch_run(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
+int stwo_hip_batch_destroy(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
+
 /* ---- profiling ------------------------------------------------------------
  * When enabled, every kernel launch of the context is bracketed by HIP
  * events on the context's stream; stwo_hip_prof_read synchronizes and

@@ -1,0 +1,298 @@
+// Batched LDE + Merkle commitment of columns of uneven sizes.
+//
+// The plain path dispatches per column (the reference's evaluate_polynomials
+// submits one RFFT per column, MB/poly.rs:715-779, and its IFFT one per
+// column, MB/poly.rs:364-391).  Here a plan covers a whole column set:
+//   - columns are grouped by log size; each group's FFT is a list of passes
+//     (radix-16, see fft.hip); passes of the same level from every group run
+//     in ONE launch (fft_pass_multi_kernel), so the IFFT and the RFFT take
+//     2-3 launches each whatever the number of sizes;
+//   - the Merkle tree is built layer by layer: mixed degree injects the
+//     evaluations of each size at their own layer (MerkleProver::commit,
+//     STWO/prover/vcs/prover.rs:39-65), lifted hashes all columns into the
+//     leaves (MerkleProverLifted::commit, STWO/prover/vcs_lifted/prover.rs:33-44);
+//   - every table (column pointers, pass descriptors) is uploaded once at
+//     plan time into plan-owned device memory, so the launch sequence has no
+//     host-to-device copies and can be captured into a hipGraph and replayed.
+#include <string.h>
+
+#include <algorithm>
+#include <functional>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+#include "field.hpp"
+#include "internal.hpp"
+
+struct stwo_hip_batch {
+  int device = 0;
+  void *d_mem = nullptr;
+  std::function<int(stwo_hip_ctx *)> issue;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t n_launches = 0;
+};
+
+namespace stwo_hip {
+
+namespace {
+
+// Bump allocator over one host staging vector mirrored by one device buffer.
+struct Staging {
+  std::vector<uint8_t> host;
+  size_t put(const void *src, size_t bytes) {
+    size_t off = (host.size() + 255) & ~(size_t)255;
+    host.resize(off + bytes);
+    memcpy(host.data() + off, src, bytes);
+    return off;
+  }
+};
+
+struct LevelLaunch {
+  bool inverse;
+  size_t groups_off, wgs_off;  // offsets into the device buffer
+  uint32_t n_groups, total_wgs, wg_log;
+};
+
+}  // namespace
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                          uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
+                          int tree_kind, int hash_mode, int use_graph, stwo_hip_batch **out) {
+  if (!ctx || !out) return STWO_HIP_EINVAL;
+  *out = nullptr;
+  STWO_HIP_CHECK_ARG(ctx, n_cols > 0 && cols && log_sizes && outs && layers, "batch_create: NULL or empty arguments");
+  STWO_HIP_CHECK_ARG(ctx, tw && tw->device == ctx->device, "batch_create: twiddles missing or from another device");
+  STWO_HIP_CHECK_ARG(ctx, tree_kind == STWO_HIP_TREE_MIXED || tree_kind == STWO_HIP_TREE_LIFTED,
+                     "batch_create: unknown tree kind %d", tree_kind);
+  uint32_t max_log = 0;
+  for (size_t i = 0; i < n_cols; i++) {
+    STWO_HIP_CHECK_ARG(ctx, cols[i] && outs[i], "batch_create: NULL column %zu", i);
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 4 && log_sizes[i] + log_blowup <= 28,
+                       "batch_create: column %zu log size %u outside [4, 28 - blowup]", i, log_sizes[i]);
+    max_log = std::max(max_log, log_sizes[i] + log_blowup);
+  }
+  // Every CanonicCoset(k).circle_domain(), k <= max_log, must be a doubling of the root.
+  uint32_t rl = tw->root.log_size;
+  STWO_HIP_CHECK_ARG(ctx, max_log - 1 <= rl && tw->root.initial_index == half_odds_initial(rl),
+                     "batch_create: twiddle root must be half_odds(r) with r >= %u", max_log - 1);
+  const uint32_t *itw_end = tw->d_itw + tw->len, *tw_end = tw->d_tw + tw->len;
+
+  // Groups by log size, largest first (their workgroups start first).
+  std::map<uint32_t, std::vector<size_t>, std::greater<uint32_t>> groups;
+  for (size_t i = 0; i < n_cols; i++) groups[log_sizes[i]].push_back(i);
+
+  Staging st;
+  struct GroupTabs {
+    uint32_t n;
+    size_t ifft_off, rfft_off, inplace_off;
+    uint32_t count;
+  };
+  std::vector<GroupTabs> gt;
+  for (auto &g : groups) {
+    std::vector<FftCol> a, b, c;
+    for (size_t i : g.second) {
+      a.push_back({cols[i], cols[i]});
+      b.push_back({cols[i], outs[i]});
+      c.push_back({outs[i], outs[i]});
+    }
+    GroupTabs t;
+    t.n = g.first;
+    t.count = (uint32_t)g.second.size();
+    t.ifft_off = st.put(a.data(), a.size() * sizeof(FftCol));
+    t.rfft_off = st.put(b.data(), b.size() * sizeof(FftCol));
+    t.inplace_off = st.put(c.data(), c.size() * sizeof(FftCol));
+    gt.push_back(t);
+  }
+  // Merkle column tables.
+  struct LayerTab {
+    uint32_t log_size;
+    size_t off;
+    uint32_t count;
+  };
+  std::vector<LayerTab> lt;
+  if (tree_kind == STWO_HIP_TREE_MIXED) {
+    for (int l = (int)max_log; l >= 0; l--) {
+      std::vector<ColRef> refs;
+      for (size_t i = 0; i < n_cols; i++)
+        if (log_sizes[i] + log_blowup == (uint32_t)l) refs.push_back({outs[i], 0, 0});
+      if (!refs.empty()) lt.push_back({(uint32_t)l, st.put(refs.data(), refs.size() * sizeof(ColRef)),
+                                       (uint32_t)refs.size()});
+    }
+  } else {
+    std::vector<size_t> order(n_cols);
+    for (size_t i = 0; i < n_cols; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return log_sizes[a] < log_sizes[b]; });
+    std::vector<ColRef> refs;
+    for (size_t i : order) refs.push_back({outs[i], max_log - (log_sizes[i] + log_blowup), 0});
+    lt.push_back({max_log, st.put(refs.data(), refs.size() * sizeof(ColRef)), (uint32_t)refs.size()});
+  }
+  // Reserve the pass descriptor area after the tables: the descriptors hold
+  // device addresses of the tables, known once the buffer exists.
+  size_t tables_bytes = (st.host.size() + 255) & ~(size_t)255;
+  void *dmem = nullptr;
+  // Upper bound on descriptors: 2 directions x 3 levels x 2 workgroup sizes, plus alignment.
+  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 512 + 4096;
+  if (hipMalloc(&dmem, tables_bytes + desc_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(ctx, STWO_HIP_ENOMEM, "batch_create: out of device memory for tables");
+  }
+  uint8_t *dbase = (uint8_t *)dmem;
+  std::vector<LevelLaunch> levels;
+  for (int dir = 0; dir < 2; dir++) {
+    bool inverse = dir == 0;
+    // (level, workgroup size) -> passes of every group
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<PassLaunch>> by_level;
+    for (const GroupTabs &t : gt) {
+      std::vector<PassLaunch> passes;
+      uint32_t n = t.n, N = t.n + (inverse ? 0 : log_blowup);
+      const FftCol *first = (const FftCol *)(dbase + (inverse ? t.ifft_off : t.rfft_off));
+      const FftCol *inpl = (const FftCol *)(dbase + t.inplace_off);
+      int rc = make_group_passes(ctx, inverse ? N : n, N, inverse, first, inpl, t.count, inverse ? itw_end : tw_end,
+                                 passes);
+      if (rc) {
+        (void)hipFree(dmem);
+        return rc;
+      }
+      for (size_t k = 0; k < passes.size(); k++) {
+        uint32_t level = inverse ? (uint32_t)k : (uint32_t)(passes.size() - 1 - k);
+        by_level[{level, passes[k].wg_log}].push_back(passes[k]);
+      }
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    for (auto &kv : by_level) order.push_back(kv.first);
+    if (!inverse) std::reverse(order.begin(), order.end());  // forward: highest level first
+    for (const auto &key : order) {
+      auto &v = by_level[key];
+      std::vector<FftPass2> descs;
+      std::vector<uint32_t> starts;
+      uint64_t acc = 0;
+      for (auto &pl : v) {
+        descs.push_back(pl.p);
+        starts.push_back((uint32_t)acc);
+        acc += pl.wgs;
+      }
+      if (acc >= (1ull << 31)) {
+        (void)hipFree(dmem);
+        return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
+      }
+      LevelLaunch ll;
+      ll.inverse = inverse;
+      ll.groups_off = st.put(descs.data(), descs.size() * sizeof(FftPass2));
+      ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
+      ll.n_groups = (uint32_t)descs.size();
+      ll.total_wgs = (uint32_t)acc;
+      ll.wg_log = key.second;
+      levels.push_back(ll);
+    }
+  }
+  if (st.host.size() > tables_bytes + desc_bytes) {
+    (void)hipFree(dmem);
+    return set_error(ctx, STWO_HIP_EINVAL, "internal: batch descriptor area overflow");
+  }
+  if (hipMemcpy(dmem, st.host.data(), st.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dmem);
+    return set_error(ctx, STWO_HIP_EDEVICE, "batch_create: table upload failed");
+  }
+
+  stwo_hip_batch *b = new (std::nothrow) stwo_hip_batch();
+  if (!b) {
+    (void)hipFree(dmem);
+    return set_error(ctx, STWO_HIP_ENOMEM, "host allocation failed");
+  }
+  b->device = ctx->device;
+  b->d_mem = dmem;
+  const int mode = hash_mode;
+  const int kind = tree_kind;
+  b->issue = [levels, lt, dbase, layers, max_log, mode, kind](stwo_hip_ctx *c) -> int {
+    for (const LevelLaunch &ll : levels) {
+      int rc = launch_fft_multi(c, ll.inverse, (const FftPass2 *)(dbase + ll.groups_off),
+                                (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs, ll.wg_log);
+      if (rc) return rc;
+    }
+    if (kind == STWO_HIP_TREE_LIFTED) {
+      const LayerTab &t = lt[0];
+      int rc = launch_merkle_layer(c, nullptr, (const ColRef *)(dbase + t.off), t.count, max_log,
+                                   layers + tree_layer_off(max_log), mode, true);
+      if (rc) return rc;
+      return hash_node_layers(c, layers, max_log, 0, mode);
+    }
+    // mixed degree: layers max_log .. 0; runs of pure node layers are fused.
+    int l = (int)max_log;
+    size_t ti = 0;
+    while (l >= 0) {
+      const uint32_t *prev = (l == (int)max_log) ? nullptr : layers + tree_layer_off((uint32_t)l + 1);
+      if (ti < lt.size() && lt[ti].log_size == (uint32_t)l) {
+        const LayerTab &t = lt[ti++];
+        int rc = launch_merkle_layer(c, prev, (const ColRef *)(dbase + t.off), t.count, (uint32_t)l,
+                                     layers + tree_layer_off((uint32_t)l), mode, false);
+        if (rc) return rc;
+        l--;
+        continue;
+      }
+      int next = ti < lt.size() ? (int)lt[ti].log_size : -1;
+      int rc = hash_node_layers(c, layers, (uint32_t)l + 1, (uint32_t)(next + 1), mode);
+      if (rc) return rc;
+      l = next;
+    }
+    return STWO_HIP_OK;
+  };
+  b->n_launches = levels.size();
+  if (use_graph) {
+    bool prof = ctx->prof;
+    ctx->prof = false;  // no event records inside the captured sequence
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    int rc = e == hipSuccess ? b->issue(ctx) : STWO_HIP_EDEVICE;
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    ctx->prof = prof;
+    if (rc == STWO_HIP_OK && e2 == hipSuccess && g) {
+      if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) rc = STWO_HIP_EDEVICE;
+      b->graph = g;
+    } else if (rc == STWO_HIP_OK) {
+      rc = STWO_HIP_EDEVICE;
+    }
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipFree(dmem);
+      delete b;
+      (void)hipGetLastError();
+      return set_error(ctx, rc, "batch_create: graph capture failed (%s)",
+                       ctx->last_error.empty() ? "hip" : ctx->last_error.c_str());
+    }
+  }
+  *out = b;
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_batch_run(stwo_hip_ctx *ctx, stwo_hip_batch *b) {
+  if (!ctx || !b) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, b->device == ctx->device, "batch_run: plan belongs to another device");
+  if (b->exec) {
+    hipEvent_t start = nullptr;
+    prof_begin(ctx, "batch_graph", &start);
+    STWO_HIP_TRY(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    prof_end(ctx, "batch_graph", start);
+    return STWO_HIP_OK;
+  }
+  return b->issue(ctx);
+}
+
+int stwo_hip_batch_destroy(stwo_hip_ctx *ctx, stwo_hip_batch *b) {
+  if (!ctx || !b) return STWO_HIP_EINVAL;
+  (void)hipStreamSynchronize(ctx->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_mem) (void)hipFree(b->d_mem);
+  delete b;
+  return STWO_HIP_OK;
+}
+
+}  // extern "C"

@@ -31,6 +31,7 @@
 
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
@@ -39,10 +40,6 @@ constexpr uint32_t kTile = 1u << kTileLog;
 constexpr uint32_t kFftThreads = 256;
 constexpr uint32_t kStridedMaxLayers = 6;
 
-struct FftCol {
-  const uint32_t *in;
-  uint32_t *out;
-};
 
 struct FftPass {
   const FftCol *cols;     // one entry per blockIdx.y
@@ -213,11 +210,293 @@ __global__ void __launch_bounds__(256) bit_reverse_tiled_kernel(uint32_t *v, uin
   }
 }
 
+// ---- register-resident radix-16 passes -------------------------------------
+//
+// A pass owns a tile of 2^T elements (T <= 12) of one column: tile bits
+// [w, w + m) are butterfly layers [lo, lo + m) (global bit = tile bit - w + lo)
+// and tile bits [0, w) are consecutive elements ("inner").  2^(T-4) lanes
+// share a tile, each holding 16 elements in VGPRs.  A stage works on a
+// 4-bit register group at tile bits [b0, b0 + 4): lane `t` holds tile
+// indices base(t) | (j << b0), j < 16, and applies the layers at group bits
+// [r_lo, r_hi) with unrolled register butterflies.  Between stages the tile
+// is exchanged through LDS (one padding word per 16 avoids most bank
+// conflicts).  The first stage reads HBM directly and the last writes HBM
+// directly.
+//
+// Twiddles: for the group of lane t the pair (j0, j0 | 1 << r) of layer s
+// uses h = (g(base) >> (s + 1)) | (j0 >> (r + 1)), so a stage needs 2^(3-r)
+// consecutive words of tree level s per layer -- 15 words per lane per
+// stage, loaded once.  The circle layer (s = 0) reads 4 words (two (x, y)
+// pairs of line layer 0) for its 8 pairs.
+
+__device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
+
+template <bool INV>
+__device__ __forceinline__ void bfly(uint32_t &a, uint32_t &b, uint32_t t) {
+  if (INV) {
+    uint32_t s = m31_add(a, b);
+    b = m31_mul(m31_sub(a, b), t);
+    a = s;
+  } else {
+    uint32_t tmp = m31_mul(b, t);
+    b = m31_sub(a, tmp);
+    a = m31_add(a, tmp);
+  }
+}
+
+// Twiddles of one stage: for register bit r, 8 >> r consecutive words of
+// tree level s = sh + r starting at gb >> (s + 1); the circle layer (s = 0)
+// is expanded from two (x, y) pairs of line layer 0 into [y, -y, -x, x].
+struct StageTw {
+  uint32_t t0[8], t1[4], t2[2], t3[1];
+};
+
+__device__ __forceinline__ void load_level(uint32_t *dst, int cnt, const uint32_t *tw_end, uint32_t N, uint32_t s,
+                                           uint32_t gb) {
+  const uint32_t hs = gb >> (s + 1);
+  if (s == 0) {
+    const uint32_t *l0 = tw_end - ((size_t)1 << (N - 1)) + (hs >> 1);
+    uint32_t w4[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) w4[k] = l0[k];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      uint32_t xx = w4[2 * (u >> 2)], yy = w4[2 * (u >> 2) + 1];
+      uint32_t v = (u & 2) ? xx : yy;
+      dst[u] = ((u & 3) == 1 || (u & 3) == 2) ? m31_neg(v) : v;
+    }
+    return;
+  }
+  const uint32_t *lv = tw_end - ((size_t)1 << (N - s)) + hs;
+#pragma unroll
+  for (int u = 0; u < 8; u++)
+    if (u < cnt) dst[u] = lv[u];
+}
+
+// Loads only the levels of the stage's active register bits [r_lo, r_hi).
+__device__ __forceinline__ void load_stage_twiddles(StageTw &t, const uint32_t *tw_end, uint32_t N, uint32_t sh,
+                                                    uint32_t gb, uint32_t r_lo, uint32_t r_hi) {
+  if (r_lo <= 0 && 0 < r_hi) load_level(t.t0, 8, tw_end, N, sh + 0, gb);
+  if (r_lo <= 1 && 1 < r_hi) load_level(t.t1, 4, tw_end, N, sh + 1, gb);
+  if (r_lo <= 2 && 2 < r_hi) load_level(t.t2, 2, tw_end, N, sh + 2, gb);
+  if (r_lo <= 3 && 3 < r_hi) load_level(t.t3, 1, tw_end, N, sh + 3, gb);
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t stage_tw(const StageTw &t, int u) {
+  return R == 0 ? t.t0[u] : R == 1 ? t.t1[u] : R == 2 ? t.t2[u] : t.t3[u];
+}
+
+// Layers at register bits [RLO, RHI) of the group (forward: high to low).
+template <bool INV, int RLO, int RHI>
+__device__ __forceinline__ void stage_layers(uint32_t (&x)[16], const StageTw &t) {
+#pragma unroll
+  for (int rr = 0; rr < RHI - RLO; rr++) {
+    const int r = INV ? RLO + rr : RHI - 1 - rr;
+#pragma unroll
+    for (int j0 = 0; j0 < 16; j0++)
+      if (!((j0 >> r) & 1)) {
+        const int u = j0 >> (r + 1);
+        const uint32_t tw = r == 0 ? stage_tw<0>(t, u) : r == 1 ? stage_tw<1>(t, u) : r == 2 ? stage_tw<2>(t, u)
+                                                                                           : stage_tw<3>(t, u);
+        bfly<INV>(x[j0], x[j0 | (1 << r)], tw);
+      }
+  }
+}
+
+template <bool INV>
+__device__ __forceinline__ void stage_dispatch(uint32_t (&x)[16], const StageTw &t, uint32_t r_lo, uint32_t r_hi) {
+  switch (r_lo * 8 + r_hi) {
+    case 4: stage_layers<INV, 0, 4>(x, t); break;
+    case 3: stage_layers<INV, 0, 3>(x, t); break;
+    case 2: stage_layers<INV, 0, 2>(x, t); break;
+    case 1: stage_layers<INV, 0, 1>(x, t); break;
+    case 10: stage_layers<INV, 1, 2>(x, t); break;
+    case 11: stage_layers<INV, 1, 3>(x, t); break;
+    case 12: stage_layers<INV, 1, 4>(x, t); break;
+    case 19: stage_layers<INV, 2, 3>(x, t); break;
+    case 20: stage_layers<INV, 2, 4>(x, t); break;
+    case 28: stage_layers<INV, 3, 4>(x, t); break;
+    default: break;  // (0, 0): copy-only stage; every other shape is rejected on the host
+  }
+}
+
+// One workgroup's share of a pass: workgroup `wg` of the pass's grid, with
+// 2^WG_LOG lanes (a tile of 2^T elements takes 2^(T-4) of them).
+//
+// Data movement: global loads and stores use a lane-consecutive layout --
+// the top register group (b0 = T - 4, lane `local` holds tile indices
+// local + j * 2^(T-4)) or any group with b0 >= 5 (runs of >= 32 lanes).
+// Between stages whose groups differ, the tile is exchanged through LDS.
+// Each stage's twiddles are loaded before the exchange that precedes it.
+//
+// With rep_log > 0 (the first RFFT pass of a blowup), the tile grid covers
+// the coefficient domain 2^(N - rep_log) and each workgroup writes the tile
+// of every one of the 2^rep_log output blocks: the coefficients are read
+// once from HBM (repeats hit L2) and the blocks differ only in twiddles.
+template <bool INV, int WG_LOG>
+__device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, uint32_t *lds) {
+  const uint32_t tpt_log = p.T - 4;  // log2 lanes per tile
+  const uint32_t local = threadIdx.x & ((1u << tpt_log) - 1);
+  const uint32_t slot = threadIdx.x >> tpt_log;
+  const uint64_t gtile = ((uint64_t)wg << (WG_LOG - tpt_log)) | slot;
+  const uint32_t dom_log = p.N - p.rep_log;
+  const uint32_t tpc_log = dom_log - p.T;  // log2 tiles per column
+  const uint64_t col_idx = gtile >> tpc_log;
+  const bool active = col_idx < p.n_cols;
+  const uint32_t tile = (uint32_t)(gtile & ((1ull << tpc_log) - 1));
+  const uint32_t *in = nullptr;
+  uint32_t *out = nullptr;
+  if (active) {
+    in = p.cols[col_idx].in;
+    out = p.cols[col_idx].out;
+  }
+  if (tpt_log >= 6) {  // a tile spans whole waves: the column is wave-uniform
+    in = (const uint32_t *)readfirstlane_u64((uint64_t)in);
+    out = (uint32_t *)readfirstlane_u64((uint64_t)out);
+  }
+  gconst_u32 *gin = as_global(in);
+  g_u32 *gout = as_global(out);
+  uint32_t *L = lds + slot * ((1u << p.T) + (1u << (p.T - 4)));
+  const uint32_t ih_bits = p.lo - p.w;
+  const uint32_t ih = tile & ((1u << ih_bits) - 1);
+  const uint32_t o = tile >> ih_bits;
+  const uint32_t in_mask = (uint32_t)((1ull << p.in_log) - 1);
+  const uint32_t wmask = (1u << p.w) - 1;
+  const uint32_t top_b0 = p.T - 4;
+  const uint32_t n_rep = 1u << p.rep_log;
+  uint32_t x[16];
+  for (uint32_t rep = 0; rep < n_rep; rep++) {
+    const uint32_t tile_gb = (rep << dom_log) | (o << (p.lo + p.m)) | (ih << p.w);
+    // global index of tile index `b` (group bits zero) and the shift of group bit 0
+    auto gidx = [&](uint32_t b) { return tile_gb | ((b >> p.w) << p.lo) | (b & wmask); };
+    auto group_base = [&](uint32_t b0) {
+      return ((local >> b0) << (b0 + 4)) | (local & ((1u << b0) - 1));
+    };
+    // --- load in the top layout
+    uint32_t cur = top_b0;
+    {
+      const uint32_t gb = gidx(local), sh = top_b0 - p.w + p.lo;
+      if (rep > 0) __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) x[j] = gin[(gb | ((uint32_t)j << sh)) & in_mask];
+      }
+    }
+    for (uint32_t si = 0; si < p.n_stages; si++) {
+      const FftStage S = p.st[si];
+      const uint32_t base = group_base(S.b0);
+      const uint32_t gb = gidx(base);
+      const uint32_t sh = S.b0 - p.w + p.lo;
+      StageTw t;
+      load_stage_twiddles(t, p.tw_end, p.N, sh, gb, S.r_lo, S.r_hi);
+      if (S.b0 != cur) {
+        const uint32_t cb = group_base(cur);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j++) L[lds_pad(cb | ((uint32_t)j << cur))] = x[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j++) x[j] = L[lds_pad(base | ((uint32_t)j << S.b0))];
+        cur = S.b0;
+      }
+      stage_dispatch<INV>(x, t, S.r_lo, S.r_hi);
+    }
+    if (p.scale) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) x[j] = m31_mul(x[j], p.scale);
+    }
+    // --- store in a lane-consecutive layout
+    if (cur != top_b0 && cur < 5) {
+      const uint32_t cb = group_base(cur);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; j++) L[lds_pad(cb | ((uint32_t)j << cur))] = x[j];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 16; j++) x[j] = L[lds_pad(local | ((uint32_t)j << top_b0))];
+      cur = top_b0;
+    }
+    if (active) {
+      const uint32_t gb = gidx(group_base(cur)), sh = cur - p.w + p.lo;
+#pragma unroll
+      for (int j = 0; j < 16; j++) gout[gb | ((uint32_t)j << sh)] = x[j];
+    }
+  }
+}
+
+template <bool INV, int WG_LOG>
+__global__ void __launch_bounds__(1 << WG_LOG) fft_pass_kernel(FftPass2 p) {
+  __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
+  fft_pass_body<INV, WG_LOG>(p, blockIdx.x, lds);
+}
+
+// Several passes of different column groups (sizes) in one launch: group g
+// owns workgroups [wg_start[g], wg_start[g + 1]).  Used by the batched path,
+// where every column size contributes its pass of the same level.
+template <bool INV, int WG_LOG>
+__global__ void __launch_bounds__(1 << WG_LOG) fft_pass_multi_kernel(const FftPass2 *groups,
+                                                                    const uint32_t *wg_start, uint32_t n_groups) {
+  __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
+  uint32_t g = 0;
+  while (g + 1 < n_groups && wg_start[g + 1] <= blockIdx.x) g++;
+  const FftPass2 p = groups[g];
+  fft_pass_body<INV, WG_LOG>(p, blockIdx.x - wg_start[g], lds);
+}
+
+// Stages covering layer tile bits [A, B) of a tile of T bits, forward order.
+static std::vector<FftStage> plan_stages(uint32_t A, uint32_t B, uint32_t T) {
+  std::vector<FftStage> st;
+  uint32_t hi = B;
+  while (hi > A) {
+    uint32_t lo_s = (hi >= A + 4) ? hi - 4 : A;
+    uint32_t b0 = std::min(lo_s, T - 4);
+    st.push_back({b0, lo_s - b0, hi - b0});
+    hi = lo_s;
+  }
+  return st;
+}
+
 // ---- planning ---------------------------------------------------------------
 
 struct PassPlan {
   uint32_t lo, m, w, s_begin, s_end;
 };
+
+constexpr uint32_t kStridedMaxLayers2 = 8;
+constexpr uint32_t kMaxTileLog = 13;  // 8192-element tiles: 34 KB of LDS, 512 lanes
+
+// Passes of the radix-16 kernel covering layers [0, L) of a domain of log
+// size N (N >= 4), forward order: strided passes of <= 8 layers, then the
+// contiguous low pass of min(N, 12) tile bits.
+static std::vector<PassPlan> plan_passes2(uint32_t N, uint32_t L) {
+  std::vector<PassPlan> fwd;
+  uint32_t t0 = std::min(N, kTileLog);
+  if (L > t0) {
+    uint32_t rest = L - t0;
+    uint32_t npass = (rest + kStridedMaxLayers2 - 1) / kStridedMaxLayers2;
+    uint32_t hi = L;
+    for (uint32_t k = 0; k < npass; k++) {
+      uint32_t left = hi - t0, passes_left = npass - k;
+      uint32_t m = (left + passes_left - 1) / passes_left;
+      uint32_t lo = hi - m;
+      // The tile's row bits must hold a whole 4-bit register group: a pass of
+      // fewer than 4 layers takes 4 rows, the extra ones (batch bits) being
+      // the global bits just above the layers, or below them at the top.
+      uint32_t rows = std::max<uint32_t>(m, 4);
+      uint32_t row_lo = std::min(lo, L - rows);
+      // 2^w consecutive elements per tile row: 256 B runs, 128 B for 8-row tiles.
+      uint32_t w = std::min<uint32_t>(6, kMaxTileLog - rows);
+      fwd.push_back({row_lo, rows, w, lo, hi});
+      hi = lo;
+    }
+  }
+  // With a blowup (L < N) and no strided pass, the low pass tiles the
+  // coefficient domain so that its output blocks can repeat the tile.
+  uint32_t tl = (L < N && L <= kTileLog && L >= 4) ? L : t0;
+  fwd.push_back({0, tl, 0, 0, std::min(L, tl)});
+  return fwd;
+}
 
 // Passes covering layers [0, L) of a domain of log size N (N >= 3), in
 // forward (top layers first) order.
@@ -251,6 +530,97 @@ static int check_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_tw
   STWO_HIP_CHECK_ARG(ctx, (dom.initial_index & IDX_MASK) == expect,
                      "domain (initial %u, log %u) is not a doubling of the twiddle root (initial %u, log %u)",
                      dom.initial_index, dom.log_size, tw->root.initial_index, tw->root.log_size);
+  return STWO_HIP_OK;
+}
+
+// Pass parameters (in execution order) of one group of same-shape columns on
+// the radix-16 path (N >= 4).  `d_tab` is the group's FftCol table for the
+// first pass; later forward passes run in place on the outputs and use
+// `d_tab_inplace` (entries {out, out}).
+int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, const FftCol *d_tab,
+                      const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end,
+                      std::vector<PassLaunch> &out) {
+  out.clear();
+  STWO_HIP_CHECK_ARG(ctx, N >= 4 && N <= 28 && n <= N, "internal: radix-16 path needs 4 <= N <= 28 (N = %u)", N);
+  uint32_t L = inverse ? N : n;
+  std::vector<PassPlan> plan = plan_passes2(N, L);
+  if (inverse) std::reverse(plan.begin(), plan.end());
+  uint32_t n_inv = m31_inv((uint32_t)(((uint64_t)1 << N) % P));
+  for (size_t k = 0; k < plan.size(); k++) {
+    const PassPlan &pp = plan[k];
+    FftPass2 p{};
+    p.cols = (!inverse && k > 0) ? d_tab_inplace : d_tab;
+    p.n_cols = n_cols;
+    p.tw_end = tw_end;
+    p.N = N;
+    p.T = pp.m + pp.w;
+    p.lo = pp.lo;
+    p.w = pp.w;
+    p.m = pp.m;
+    // layer tile bits [s_begin - lo + w, s_end - lo + w)
+    std::vector<FftStage> st = plan_stages(pp.s_begin - pp.lo + pp.w, pp.s_end - pp.lo + pp.w, p.T);
+    if (inverse) std::reverse(st.begin(), st.end());
+    if (st.empty()) st.push_back({0, 0, 0});  // pure copy (e.g. evaluating one coefficient)
+    STWO_HIP_CHECK_ARG(ctx, st.size() <= kMaxStages, "internal: %zu stages", st.size());
+    for (const FftStage &sg : st)
+      STWO_HIP_CHECK_ARG(ctx, (sg.r_lo < sg.r_hi && sg.r_hi <= 4) || (sg.r_lo == 0 && sg.r_hi == 0),
+                         "internal: unsupported register stage [%u, %u)", sg.r_lo, sg.r_hi);
+    p.n_stages = (uint32_t)st.size();
+    for (size_t i = 0; i < st.size(); i++) p.st[i] = st[i];
+    p.in_log = (!inverse && k == 0) ? n : N;
+    p.scale = (inverse && k + 1 == plan.size()) ? n_inv : 0;
+    // First RFFT pass of a blowup whose tile rows stay inside the coefficient
+    // domain: one workgroup writes all 2^(N-n) output blocks of its tile.
+    p.rep_log = (!inverse && k == 0 && N > n && pp.lo + pp.m <= n) ? N - n : 0;
+    uint32_t wg_log = std::max<uint32_t>(8, p.T - 4);
+    uint64_t total_tiles = (uint64_t)n_cols << (N - p.rep_log - p.T);
+    uint32_t tiles_per_wg = 1u << (wg_log + 4 - p.T);
+    uint64_t wgs = (total_tiles + tiles_per_wg - 1) / tiles_per_wg;
+    STWO_HIP_CHECK_ARG(ctx, wgs < (1ull << 31), "internal: grid too large");
+    out.push_back({p, (uint32_t)wgs, wg_log});
+  }
+  return STWO_HIP_OK;
+}
+
+int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log) {
+  if (wg_log == 9)
+    return inverse ? launch(ctx, "ifft_pass", fft_pass_kernel<true, 9>, dim3(wgs), dim3(512), 0, p)
+                   : launch(ctx, "rfft_pass", fft_pass_kernel<false, 9>, dim3(wgs), dim3(512), 0, p);
+  return inverse ? launch(ctx, "ifft_pass", fft_pass_kernel<true, 8>, dim3(wgs), dim3(256), 0, p)
+                 : launch(ctx, "rfft_pass", fft_pass_kernel<false, 8>, dim3(wgs), dim3(256), 0, p);
+}
+
+int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log) {
+  if (wg_log == 9)
+    return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 9>, dim3(total_wgs), dim3(512), 0,
+                            d_groups, d_wg_start, n_groups)
+                   : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 9>, dim3(total_wgs), dim3(512), 0,
+                            d_groups, d_wg_start, n_groups);
+  return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 8>, dim3(total_wgs), dim3(256), 0,
+                          d_groups, d_wg_start, n_groups)
+                 : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 8>, dim3(total_wgs), dim3(256), 0,
+                          d_groups, d_wg_start, n_groups);
+}
+
+// Radix-16 path for N >= 4: one launch per pass over all columns of the group.
+static int run_fft_group2(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, void *dtab, uint32_t n, uint32_t N,
+                          const uint32_t *tw_end, bool inverse) {
+  void *dtab_out = dtab;
+  if (!inverse) {
+    std::vector<FftCol> c2(cols);
+    for (auto &c : c2) c.in = c.out;
+    int rc = upload_table(ctx, c2.data(), c2.size() * sizeof(FftCol), &dtab_out);
+    if (rc) return rc;
+  }
+  std::vector<PassLaunch> passes;
+  int rc = make_group_passes(ctx, n, N, inverse, (const FftCol *)dtab, (const FftCol *)dtab_out,
+                             (uint32_t)cols.size(), tw_end, passes);
+  if (rc) return rc;
+  for (const PassLaunch &pl : passes) {
+    rc = launch_fft_pass(ctx, inverse, pl.p, pl.wgs, pl.wg_log);
+    if (rc) return rc;
+  }
   return STWO_HIP_OK;
 }
 
@@ -289,6 +659,7 @@ static int run_fft_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, uin
   const uint32_t *tree = inverse ? tw->d_itw : tw->d_tw;
   const uint32_t *tw_end = tree + tw->len;
   uint32_t L = inverse ? N : n;
+  if (N >= 4) return run_fft_group2(ctx, cols, dtab, n, N, tw_end, inverse);
   std::vector<PassPlan> plan = plan_passes(N, L);
   if (inverse) std::reverse(plan.begin(), plan.end());
   // Output-side table (in place after the first forward pass).

@@ -138,3 +138,25 @@ __host__ __device__ __forceinline__ uint32_t bitrev32(uint32_t i, uint32_t log_s
 inline uint32_t half_odds_initial(uint32_t n) { return 1u << (29 - n); }
 
 }  // namespace stwo_hip
+
+namespace stwo_hip {
+// Device pointers read from tables are generic (flat) pointers to the
+// compiler; these casts state that they point to global memory so loads and
+// stores use global_* instructions (SGPR base + 32-bit VGPR offset when the
+// base is wave-uniform) instead of flat ones with 64-bit per-lane addresses.
+typedef __attribute__((address_space(1))) const uint32_t gconst_u32;
+typedef __attribute__((address_space(1))) uint32_t g_u32;
+__device__ __forceinline__ gconst_u32 *as_global(const uint32_t *p) { return (gconst_u32 *)p; }
+__device__ __forceinline__ g_u32 *as_global(uint32_t *p) { return (g_u32 *)p; }
+}  // namespace stwo_hip
+
+namespace stwo_hip {
+// Wave-uniform copy of a 64-bit value (e.g. a pointer) into SGPRs.
+// __builtin_amdgcn_readfirstlane is a 32-bit operation: each half is read
+// separately so the upper half of a device address is preserved.
+__device__ __forceinline__ uint64_t readfirstlane_u64(uint64_t v) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+}  // namespace stwo_hip

@@ -24,14 +24,10 @@
 #include "blake2s.hpp"
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
-struct ColRef {
-  const uint32_t *p;
-  uint32_t shift;  // lifted: L - log_size(col); mixed: unused
-  uint32_t pad;
-};
 
 __device__ __forceinline__ void store_hash(uint32_t *dst, const b2_state &s, uint32_t m31) {
   uint4 a, b;
@@ -87,7 +83,7 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(const uint32_t *prev,
       if (k < cnt) {
         const ColRef cr = cols[c0 + k];
         uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
-        v = cr.p[idx];
+        v = as_global(cr.p)[idx];
       }
       m[k] = v;
     }
@@ -146,7 +142,15 @@ __global__ void __launch_bounds__(256) merkle_nodes_kernel(const uint32_t *prev,
   }
 }
 
-static inline size_t layer_off(uint32_t l) { return (((size_t)1 << l) - 1) * 8; }
+static inline size_t layer_off(uint32_t l) { return tree_layer_off(l); }
+
+int launch_merkle_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const ColRef *d_cols, uint32_t n_cols,
+                        uint32_t log_size, uint32_t *out, int mode, bool lifted) {
+  uint32_t n = 1u << log_size;
+  return launch(ctx, lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer", merkle_layer_kernel,
+                dim3((n + 255) / 256), dim3(256), 0, prev, d_cols, n_cols, log_size, out,
+                (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31), (uint32_t)lifted);
+}
 
 static int launch_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const std::vector<ColRef> &cols, uint32_t log_size,
                         uint32_t *out, int mode, bool lifted) {
@@ -155,14 +159,11 @@ static int launch_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const std::vect
     int rc = upload_table(ctx, cols.data(), cols.size() * sizeof(ColRef), &dtab);
     if (rc) return rc;
   }
-  uint32_t n = 1u << log_size;
-  return launch(ctx, lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer", merkle_layer_kernel,
-                dim3((n + 255) / 256), dim3(256), 0, prev, (const ColRef *)dtab, (uint32_t)cols.size(), log_size, out,
-                (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31), (uint32_t)lifted);
+  return launch_merkle_layer(ctx, prev, (const ColRef *)dtab, (uint32_t)cols.size(), log_size, out, mode, lifted);
 }
 
-// Hashes node layers top_log-1 .. 0 of a packed tree whose layer top_log is filled.
-static int hash_node_layers(stwo_hip_ctx *ctx, uint32_t *layers, uint32_t top_log, uint32_t stop_log, int mode) {
+// Hashes node layers top_log-1 .. stop_log of a packed tree whose layer top_log is filled.
+int hash_node_layers(stwo_hip_ctx *ctx, uint32_t *layers, uint32_t top_log, uint32_t stop_log, int mode) {
   // layers [stop_log, top_log) are produced, from top_log - 1 down.
   uint32_t l = top_log;
   while (l > stop_log) {

@@ -20,6 +20,8 @@ P = (1 << 31) - 1
 
 HASH_BLAKE2S = 0
 HASH_BLAKE2S_M31 = 1
+TREE_MIXED = 0
+TREE_LIFTED = 1
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
@@ -93,6 +95,11 @@ def lib():
             "stwo_hip_fold_line": [_vp, ctypes.POINTER(_vp), Coset, _u32p, ctypes.c_uint32, _vp, ctypes.POINTER(_vp)],
             "stwo_hip_fri_commit": [_vp, ctypes.c_char_p, _u32p, FriConfig, ctypes.POINTER(_vp), ctypes.c_uint32, _vp,
                                     _u32p, _u32p, _u32p],
+            "stwo_hip_batch_create": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
+                                      ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(_vp)],
+            "stwo_hip_batch_run": [_vp, _vp],
+            "stwo_hip_batch_destroy": [_vp, _vp],
             "stwo_hip_prof_enable": [_vp, ctypes.c_int],
             "stwo_hip_prof_reset": [_vp],
             "stwo_hip_prof_read": [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64),
@@ -162,6 +169,21 @@ class Twiddles:
     def destroy(self):
         if self.h:
             self.ctx._check(lib().stwo_hip_twiddles_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
+class Batch:
+    """A batched LDE + Merkle plan (stwo_hip_batch)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+
+    def run(self):
+        self.ctx._check(lib().stwo_hip_batch_run(self.ctx.h, self.h))
+
+    def destroy(self):
+        if self.h:
+            self.ctx._check(lib().stwo_hip_batch_destroy(self.ctx.h, self.h))
             self.h = None
 
 
@@ -301,6 +323,17 @@ class Context:
                                               last.ctypes.data_as(_u32p)))
         n = int(nr[0])
         return roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), d.raw[:32], int(nd[0])
+
+    # -- batched commitment -------------------------------------------------------
+    def batch_create(self, cols, log_sizes, log_blowup, tw, outs, layers, tree_kind=TREE_MIXED,
+                     mode=HASH_BLAKE2S, use_graph=True):
+        ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
+        h = _vp()
+        self._check(lib().stwo_hip_batch_create(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols), log_blowup,
+                                                tw.h, _ptrs(outs),
+                                                layers.ptr if isinstance(layers, DeviceBuf) else layers,
+                                                tree_kind, mode, int(use_graph), ctypes.byref(h)))
+        return Batch(self, h.value)
 
     # -- profiling --------------------------------------------------------------
     def prof_enable(self, on=True):

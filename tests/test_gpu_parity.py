@@ -146,6 +146,37 @@ def test_evaluate_vs_oracle(ctx, n, b):
     tw.destroy()
 
 
+def test_fft_all_pass_shapes(ctx):
+    """Every pass/stage layout the planner produces: interpolate at log 4..22 and
+    evaluate at log 4..21 with blowups 1..3, against the oracle."""
+    top = 23
+    root = sh.half_odds(top - 1)
+    tw = ctx.precompute_twiddles(root)
+    et, eit = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    for log_size in range(4, 23):
+        dom = sh.canonic_domain(log_size)
+        ev = rand(1 << log_size, 71, log_size)
+        d = ctx.to_device(ev)
+        ctx.interpolate(d, dom, tw)
+        exp = oracle.interpolate(ev, dom.initial_index, log_size, eit)
+        assert np.array_equal(d.download(), exp), ("interpolate", log_size)
+        d.free()
+    for n in range(4, 22):
+        for b in (1, 2, 3):
+            N = n + b
+            if N > top:
+                continue
+            coeffs = rand(1 << n, 73, 100 * n + b)
+            co = ctx.to_device(coeffs)
+            out = ctx.alloc(1 << N)
+            ctx.evaluate(co, n, sh.canonic_domain(N), tw, out)
+            exp = oracle.evaluate(coeffs, sh.canonic_domain(N).initial_index, N, et)
+            assert np.array_equal(out.download(), exp), ("evaluate", n, b)
+            co.free()
+            out.free()
+    tw.destroy()
+
+
 def test_rfft_ifft_shifted_domain_round_trip(ctx):
     """metal_rfft_and_ifft_match_cpu_on_shifted_domain (tests/metal_rfft_ifft_native.rs:21):
     CircleDomain(half_odds(7)), twiddles on its half coset."""
@@ -510,5 +541,68 @@ def test_config1_lde_merkle_full(ctx):
     exp_layers = oracle.merkle_commit(exp_evals)
     assert np.array_equal(layers.download(), exp_layers)
     for x in d + outs + [layers]:
+        x.free()
+    tw.destroy()
+
+
+# ---------------------------------------------------------- batched commitment --
+
+def _geometric_logs(n_cols, lo, hi, seed):
+    """log sizes lo + j with P(j) proportional to 0.7^j (the uneven-column workload's law)."""
+    r = np.random.default_rng(seed)
+    w = np.array([0.7 ** j for j in range(hi - lo + 1)])
+    return [int(lo + j) for j in r.choice(len(w), size=n_cols, p=w / w.sum())]
+
+
+@pytest.mark.parametrize("tree_kind,use_graph,mode", [(sh.TREE_MIXED, True, sh.HASH_BLAKE2S),
+                                                      (sh.TREE_MIXED, False, sh.HASH_BLAKE2S_M31),
+                                                      (sh.TREE_LIFTED, True, sh.HASH_BLAKE2S)])
+def test_batch_lde_commit_vs_oracle(ctx, tree_kind, use_graph, mode):
+    """Uneven columns (log 4..17) -> interpolate + evaluate (blowup 2) + one tree, three
+    replays of the same plan (graph replays with eager work and syncs between them),
+    each checked bit-exactly against the oracle."""
+    logs = _geometric_logs(60, 4, 17, 5) + [17, 4, 13]
+    b = 1
+    max_log = max(logs) + b
+    root = sh.half_odds(max_log - 1)
+    tw = ctx.precompute_twiddles(root)
+    et, eit = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    evs = [rand(1 << l, 83, i) for i, l in enumerate(logs)]
+    cols = [ctx.to_device(e) for e in evs]
+    outs = [ctx.alloc(1 << (l + b)) for l in logs]
+    layers = ctx.alloc(sh.tree_words(max_log))
+    plan = ctx.batch_create(cols, logs, b, tw, outs, layers, tree_kind, mode, use_graph)
+    coeffs = [e.copy() for e in evs]
+    oracle.interpolate_columns(coeffs, eit)
+    exp_evals = oracle.evaluate_polynomials(coeffs, b, et)
+    if tree_kind == sh.TREE_MIXED:
+        exp_tree = oracle.merkle_commit(exp_evals, m31_output=bool(mode))
+    else:
+        exp_tree = oracle.lifted_commit(exp_evals, max_log, m31_output=bool(mode))
+    for rep in range(3):
+        for c, e in zip(cols, evs):   # eager work between replays: restore the inputs
+            c.upload(e)
+        layers.upload(np.zeros(sh.tree_words(max_log), dtype=U))
+        plan.run()
+        ctx.sync()
+        for c, e in zip(cols, coeffs):
+            assert np.array_equal(c.download(), e), ("coeffs", rep)
+        for o, e in zip(outs, exp_evals):
+            assert np.array_equal(o.download(), e), ("evals", rep)
+        assert np.array_equal(layers.download(), exp_tree), ("tree", rep)
+    plan.destroy()
+    for x in cols + outs + [layers]:
+        x.free()
+    tw.destroy()
+
+
+def test_batch_rejects_small_columns(ctx):
+    tw = ctx.precompute_twiddles(sh.half_odds(8))
+    c = ctx.alloc(8)
+    o = ctx.alloc(16)
+    layers = ctx.alloc(sh.tree_words(4))
+    with pytest.raises(sh.StwoHipError):
+        ctx.batch_create([c], [3], 1, tw, [o], layers)
+    for x in (c, o, layers):
         x.free()
     tw.destroy()
