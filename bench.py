@@ -150,6 +150,83 @@ def batched_vs_plain(ctx, steps=3):
     }
 
 
+def serving(rates=(10.0, 30.0, 60.0), n_jobs=30, budget=96 << 30, n_slots=4):
+    """Poisson streams of commit + FRI jobs (8-512 columns of log 12-22, blowup 2 or 4,
+    seeded values) through the continuous-batching server; p50/p99 latency and jobs/s
+    per arrival rate.  Three jobs are re-run alone on a 1-slot server and must give
+    identical results; the two smallest are checked against the oracle."""
+    import threading
+    rng = np.random.default_rng(5)
+    out = {"workload": "Poisson arrivals; per job 8-512 M31 columns of log_size 12-22, blowup 2 or 4, "
+                       "lifted Blake2s commit + FRI (seeded values, seed 5)", "rates": []}
+    all_jobs = []
+    srv = sh.Server(0, budget, 24, n_slots)
+    for rate in rates:
+        jobs = []
+        for k in range(n_jobs):
+            n_cols = int(rng.integers(8, 513))
+            lg = int(rng.integers(12, 23))
+            b = int(rng.integers(1, 3))
+            jobs.append((n_cols, lg, b, int(rng.integers(1, 1 << 40)), (b, 2, 3, 1)))
+        gaps = rng.exponential(1.0 / rate, size=n_jobs)
+        ids = [None] * n_jobs
+
+        def producer():
+            t_next = time.perf_counter()
+            for k, job in enumerate(jobs):
+                t_next += gaps[k]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                n, lg_, b_, seed, f = job
+                ids[k] = srv.submit(n, lg_, b_, seed, sh.FriConfig(*f))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        res = [srv.wait(i) for i in ids]
+        assert all(r.status == 0 for r in res), [r.status for r in res]
+        lat = np.array([r.t_done_ms - r.t_submit_ms for r in res])
+        span_ms = max(r.t_done_ms for r in res) - min(r.t_submit_ms for r in res)
+        elems = sum(n << lg for (n, lg, b, _, _) in jobs)
+        out["rates"].append({
+            "arrival_rate_jobs_s": rate, "jobs": n_jobs, "jobs_s": round(n_jobs / (span_ms / 1e3), 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "mean_queue_ms": round(float(np.mean([r.t_admit_ms - r.t_submit_ms for r in res])), 2),
+            "input_melem_s": round(elems / (span_ms / 1e3) / 1e6, 1)})
+        all_jobs += list(zip(jobs, res))
+    srv.close()
+    # alone vs mix, and the oracle on the two smallest jobs
+    picks = sorted(all_jobs, key=lambda jr: jr[0][0] << jr[0][1])[:3]
+    alone_ok = True
+    with sh.Server(0, budget, 24, 1) as s1:
+        for (job, r) in picks:
+            n, lg_, b_, seed, f = job
+            a = s1.wait(s1.submit(n, lg_, b_, seed, sh.FriConfig(*f)))
+            alone_ok = alone_ok and a.key() == r.key()
+    out["alone_equals_mix_checked"] = len(picks)
+    out["alone_equals_mix"] = bool(alone_ok)
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    ok = True
+    for (job, r) in picks[:2]:
+        n, lg_, b_, seed, f = job
+        L = lg_ + b_
+        tw_, itw_ = oracle.precompute_twiddles(oracle.half_odds_initial(L - 1), L - 1)
+        evs = [oracle.fill_random(1 << lg_, seed, c) for c in range(n)]
+        oracle.interpolate_columns(evs, itw_)
+        outs = oracle.evaluate_polynomials(evs, b_, tw_)
+        root = oracle.lifted_commit(outs, L)[:8]
+        ch = oracle.Channel()
+        ch.mix_root(root)
+        roots, last, _ = oracle.fri_commit(ch, oracle.FriConfig(*f), outs[:4], L)
+        ok = ok and list(r.trace_root) == root.tolist() and bytes(r.final_digest) == ch.digest_bytes() \
+            and list(r.fri_roots[: 8 * len(roots)]) == roots.reshape(-1).tolist()
+    out["oracle_checked"] = 2
+    out["oracle_equal"] = bool(ok)
+    return out
+
+
 def cpu_baseline(sample_cols=N_COLS, runs=3):
     """The oracle (C, OpenMP over columns and nodes) on the same workload, on the host cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -185,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fri", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
@@ -228,8 +306,8 @@ def main():
     #   Merkle leaves read 4 B per element per column + 32 B per leaf; nodes 96 B per node.
     n_out = 1 << (LOG_N + LOG_BLOWUP)
     alg = {
-        "ifft_tile": 8.0 * elems,
-        "rfft_tile": 4.0 * elems + 4.0 * N_COLS * n_out,
+        "ifft_pass": 8.0 * elems,
+        "rfft_pass": 4.0 * elems + 4.0 * N_COLS * n_out,
         "merkle_commit_on_layer": 4.0 * N_COLS * n_out + 32.0 * n_out,
         "merkle_nodes": 96.0 * (n_out - 1),
     }
@@ -241,6 +319,7 @@ def main():
             kernels[k]["algorithmic_GBps"] = round(alg[k] / (kernels[k]["ms_total"] / args.steps * 1e-3) / 1e9, 1)
     log("kernels: %s" % json.dumps(kernels))
 
+    serve = None
     batched = None
     if not args.no_batched:
         batched = batched_vs_plain(ctx)
@@ -251,14 +330,19 @@ def main():
         fri_ms, fri_layers = fri_chain(ctx, tw)
         log("fri chain: %.2f ms, %d committed layers" % (fri_ms, fri_layers))
 
+    for x in pristine + work + outs + [layers]:
+        x.free()
+    pristine, work, outs, layers = [], [], [], None
+    if not args.no_serve:
+        serve = serving()
+        log("serving: %s" % json.dumps(serve))
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline()
         log("cpu: %s" % json.dumps(cpu))
 
-    for x in pristine + work + outs + [layers]:
-        x.free()
     tw.destroy()
     ctx.close()
 
@@ -293,6 +377,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
         "batched_uneven": batched,
+        "serving": serve,
         "fri_chain_ms": round(fri_ms, 3) if fri_ms is not None else None,
         "fri_committed_layers": fri_layers,
     }

@@ -238,6 +238,56 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
 int stwo_hip_batch_run(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
 int stwo_hip_batch_destroy(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
 
+/* ---- serving: continuous batching of commit + FRI jobs --------------------
+ * No counterpart in the reference (its prover handles one proof at a time,
+ * MB/prove_runtime_v1.rs); this is the north star's serving mode.  A server
+ * owns `n_slots` contexts (streams), a resident twiddle tree for domains up
+ * to 2^max_log_size, and an HBM budget allocated once and sub-allocated per
+ * job.  Submitted jobs are admitted in arrival order when a slot is idle and
+ * the job fits; admitted jobs run concurrently.  A job: n_cols synthetic
+ * columns of 2^log_size (column c = stwo_hip_fill_random(seed, c)), LDE at
+ * log_blowup, lifted Blake2s commit (trace_root), then FriProver::commit on
+ * the QM31 column formed by the first four evaluations, with a channel that
+ * starts as H(0^32 || trace_root).  Times are milliseconds since the server
+ * was created. */
+typedef struct stwo_hip_server stwo_hip_server;
+typedef struct {
+  uint32_t n_cols;     /* >= 4 */
+  uint32_t log_size;   /* trace log size, >= 4 */
+  uint32_t log_blowup; /* >= 1, log_size + log_blowup <= max_log_size */
+  uint32_t pad;
+  uint64_t seed;
+  stwo_hip_fri_config fri; /* fri.log_blowup_factor == log_blowup, log_last_layer_degree_bound <= 4 */
+} stwo_hip_job_desc;
+typedef struct {
+  int status; /* STWO_HIP_OK, or the error of the job */
+  uint32_t trace_root[8];
+  uint32_t n_fri_roots;
+  uint32_t fri_roots[8 * 32];
+  uint32_t last_layer_poly[4 * 16];
+  uint8_t final_digest[32];
+  double t_submit_ms, t_admit_ms, t_start_ms, t_done_ms;
+} stwo_hip_job_result;
+int stwo_hip_server_create(int device, size_t hbm_budget_bytes, uint32_t max_log_size, uint32_t n_slots,
+                           stwo_hip_server **out);
+/* Non-blocking.  STWO_HIP_EBUSY if the queue is full, STWO_HIP_ENOMEM if the
+ * job can never fit the budget, STWO_HIP_EINVAL for a bad description. */
+int stwo_hip_server_submit(stwo_hip_server *server, const stwo_hip_job_desc *desc, uint64_t *job_id);
+/* STWO_HIP_OK and the result if done, STWO_HIP_EBUSY if not yet. */
+int stwo_hip_server_poll(stwo_hip_server *server, uint64_t job_id, stwo_hip_job_result *out);
+/* Blocks until the job is done, returns its result and forgets the job. */
+int stwo_hip_server_wait(stwo_hip_server *server, uint64_t job_id, stwo_hip_job_result *out);
+int stwo_hip_server_stats(stwo_hip_server *server, size_t *pool_used, size_t *pool_capacity, uint32_t *n_pending,
+                          uint32_t *n_running);
+/* Drains the queue (every submitted job runs), then releases everything. */
+int stwo_hip_server_destroy(stwo_hip_server *server);
+
+/* ---- introspection -----------------------------------------------------------
+ * No counterpart in the reference: describes the FFT passes this library
+ * runs for a transform (16 words per pass, see fft.hip).  Launches nothing. */
+int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, uint32_t max_passes,
+                            uint32_t *n_passes);
+
 /* ---- profiling ------------------------------------------------------------
  * When enabled, every kernel launch of the context is bracketed by HIP
  * events on the context's stream; stwo_hip_prof_read synchronizes and

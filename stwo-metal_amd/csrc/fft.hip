@@ -804,4 +804,31 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx *ctx, const uint32_t *const *coef
   return STWO_HIP_OK;
 }
 
+// Describes the passes the radix-16 path would run for a transform of log
+// size N (coefficient log size n), in execution order, 16 words per pass:
+// T, lo, w, m, rep_log, wg_log, in_log, scaled, n_stages, 4 packed stages
+// (b0 << 16 | r_lo << 8 | r_hi), workgroups for one column, 0, 0.
+// Host-only (launches nothing); lets tests check the planner without a device.
+int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, uint32_t max_passes,
+                            uint32_t *n_passes) {
+  if (!out || !n_passes || N < 4 || N > 28 || n > N) return STWO_HIP_EINVAL;
+  std::vector<PassLaunch> passes;
+  stwo_hip_ctx dummy;
+  int rc = make_group_passes(&dummy, inverse ? N : n, N, inverse != 0, nullptr, nullptr, 1, nullptr, passes);
+  if (rc) return rc;
+  if (passes.size() > max_passes) return STWO_HIP_EINVAL;
+  for (size_t k = 0; k < passes.size(); k++) {
+    const FftPass2 &p = passes[k].p;
+    uint32_t *o = out + 16 * k;
+    o[0] = p.T; o[1] = p.lo; o[2] = p.w; o[3] = p.m; o[4] = p.rep_log; o[5] = passes[k].wg_log;
+    o[6] = p.in_log; o[7] = p.scale ? 1 : 0; o[8] = p.n_stages;
+    for (uint32_t i = 0; i < kMaxStages; i++)
+      o[9 + i] = i < p.n_stages ? (p.st[i].b0 << 16) | (p.st[i].r_lo << 8) | p.st[i].r_hi : 0;
+    o[13] = passes[k].wgs;
+    o[14] = o[15] = 0;
+  }
+  *n_passes = (uint32_t)passes.size();
+  return STWO_HIP_OK;
+}
+
 }  // extern "C"

@@ -42,6 +42,24 @@ class FriConfig(ctypes.Structure):
                 ("n_queries", ctypes.c_uint32), ("line_fold_step", ctypes.c_uint32)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("n_cols", ctypes.c_uint32), ("log_size", ctypes.c_uint32), ("log_blowup", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("fri", FriConfig)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("trace_root", ctypes.c_uint32 * 8), ("n_fri_roots", ctypes.c_uint32),
+                ("fri_roots", ctypes.c_uint32 * 256), ("last_layer_poly", ctypes.c_uint32 * 64),
+                ("final_digest", ctypes.c_uint8 * 32), ("t_submit_ms", ctypes.c_double),
+                ("t_admit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double)]
+
+    def key(self):
+        """Everything the job computed (for alone-vs-mix comparisons)."""
+        n = self.n_fri_roots
+        return (self.status, tuple(self.trace_root), tuple(self.fri_roots[: 8 * n]), tuple(self.last_layer_poly),
+                bytes(self.final_digest))
+
+
 def half_odds(n):
     """Coset::half_odds(n) (STWO/core/circle.rs:328-330)."""
     return Coset((1 << (29 - n)) & ((1 << 31) - 1), n)
@@ -100,6 +118,14 @@ def lib():
                                       ctypes.POINTER(_vp)],
             "stwo_hip_batch_run": [_vp, _vp],
             "stwo_hip_batch_destroy": [_vp, _vp],
+            "stwo_hip_server_create": [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.POINTER(_vp)],
+            "stwo_hip_server_submit": [_vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)],
+            "stwo_hip_server_poll": [_vp, ctypes.c_uint64, ctypes.POINTER(JobResult)],
+            "stwo_hip_server_wait": [_vp, ctypes.c_uint64, ctypes.POINTER(JobResult)],
+            "stwo_hip_server_stats": [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                      ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)],
+            "stwo_hip_server_destroy": [_vp],
             "stwo_hip_prof_enable": [_vp, ctypes.c_int],
             "stwo_hip_prof_reset": [_vp],
             "stwo_hip_prof_read": [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64),
@@ -360,3 +386,51 @@ def layer_offset(l):
 
 def tree_words(max_log):
     return ((2 << max_log) - 1) * 8
+
+
+class Server:
+    """Continuous-batching commit + FRI server (stwo_hip_server)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=8 << 30, max_log_size=24, n_slots=4):
+        h = _vp()
+        rc = lib().stwo_hip_server_create(device, hbm_budget_bytes, max_log_size, n_slots, ctypes.byref(h))
+        if rc != 0:
+            raise StwoHipError(rc, "stwo_hip_server_create failed")
+        self.h = h.value
+
+    def submit(self, n_cols, log_size, log_blowup, seed, fri):
+        d = JobDesc(n_cols, log_size, log_blowup, 0, seed, fri)
+        jid = ctypes.c_uint64()
+        rc = lib().stwo_hip_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid))
+        if rc != 0:
+            raise StwoHipError(rc, "submit rejected")
+        return jid.value
+
+    def poll(self, jid):
+        r = JobResult()
+        rc = lib().stwo_hip_server_poll(self.h, jid, ctypes.byref(r))
+        return r if rc == 0 else None
+
+    def wait(self, jid):
+        r = JobResult()
+        rc = lib().stwo_hip_server_wait(self.h, jid, ctypes.byref(r))
+        if rc != 0:
+            raise StwoHipError(rc, "wait failed")
+        return r
+
+    def stats(self):
+        u, c = ctypes.c_size_t(), ctypes.c_size_t()
+        p, r = ctypes.c_uint32(), ctypes.c_uint32()
+        lib().stwo_hip_server_stats(self.h, ctypes.byref(u), ctypes.byref(c), ctypes.byref(p), ctypes.byref(r))
+        return {"pool_used": u.value, "pool_capacity": c.value, "pending": p.value, "running": r.value}
+
+    def close(self):
+        if self.h:
+            lib().stwo_hip_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
