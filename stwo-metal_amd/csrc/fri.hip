@@ -35,11 +35,20 @@ __device__ __forceinline__ uint32_t circle_itwiddle(const uint32_t *itw_end, uin
   return neg ? m31_neg(v) : v;
 }
 
+__device__ __forceinline__ qm31 load_qm31(const uint32_t *p) { return qm31_make(p[0], p[1], p[2], p[3]); }
+
+// `alpha_dev`, when not NULL, holds alpha in device memory (written by the
+// device channel) and replaces the by-value alpha.
 __global__ void __launch_bounds__(256) fold_circle_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
-                                                          uint32_t N, qm31 alpha, qm31 alpha_sq, uint32_t accumulate) {
+                                                          uint32_t N, qm31 alpha, qm31 alpha_sq, uint32_t accumulate,
+                                                          const uint32_t *alpha_dev) {
   const uint32_t n_out = 1u << (N - 1);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
+  if (alpha_dev) {
+    alpha = load_qm31(alpha_dev);
+    alpha_sq = qm31_mul(alpha, alpha);
+  }
   const uint32_t iy = circle_itwiddle(itw_end, N, i);
   qm31 f0, f1;
 #pragma unroll
@@ -59,11 +68,12 @@ __global__ void __launch_bounds__(256) fold_circle_kernel(Qm31Cols dst, Qm31Cols
 
 template <int STEP>
 __global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
-                                                        uint32_t n, qm31 alpha) {
+                                                        uint32_t n, qm31 alpha, const uint32_t *alpha_dev) {
   constexpr int W = 1 << STEP;
   const uint32_t n_out = 1u << (n - STEP);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
+  if (alpha_dev) alpha = load_qm31(alpha_dev);
   qm31 v[W];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -93,6 +103,54 @@ __global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsCo
   for (int k = 0; k < 4; k++) dst.c[k][i] = v[0].v[k];
 }
 
+// out = in^(2^k) (alpha for the folds after the first 4 of a longer fold step).
+__global__ void alpha_pow2k_kernel(const uint32_t *in, uint32_t *out, uint32_t k) {
+  if (threadIdx.x != 0) return;
+  qm31 a = load_qm31(in);
+  for (uint32_t i = 0; i < k; i++) a = qm31_mul(a, a);
+  for (int j = 0; j < 4; j++) out[j] = a.v[j];
+}
+
+// One Fiat-Shamir step of the FRI layer loop on the device, for the
+// Blake2sChannel (STWO/core/channel/blake2s.rs): digest = H(digest || root)
+// (MerkleChannel::mix_root, STWO/core/vcs_lifted/blake2_merkle.rs:49-55),
+// n_draws = 0, then alpha = draw_secure_felt(): draw H(digest || le32(n_draws)
+// || 0x00) until all 8 words are < 2P (blake2s.rs:36-51,103-117) and reduce
+// the first 4.  chan = [digest 8 words, n_draws, error flag].  The root is
+// also copied to root_out.  The retry loop is bounded (each try fails with
+// probability about 2^-28); running out sets the error flag.
+__global__ void fri_channel_step_kernel(uint32_t *chan, const uint32_t *root, uint32_t *root_out,
+                                        uint32_t *alpha_out) {
+  if (threadIdx.x != 0) return;
+  uint32_t m[16];
+  for (int k = 0; k < 8; k++) {
+    m[k] = chan[k];
+    m[8 + k] = root[k];
+    root_out[k] = root[k];
+  }
+  b2_state s;
+  b2_init(s);
+  b2_compress(s, m, 64, true);
+  uint32_t digest[8];
+  for (int k = 0; k < 8; k++) digest[k] = s.h[k];
+  uint32_t n_draws = 0;
+  bool ok = false;
+  for (int tries = 0; tries < 64 && !ok; tries++) {
+    for (int k = 0; k < 8; k++) m[k] = digest[k];
+    m[8] = n_draws;
+    for (int k = 9; k < 16; k++) m[k] = 0;  // the 0x00 separator byte and zero padding
+    b2_init(s);
+    b2_compress(s, m, 37, true);
+    n_draws++;
+    ok = true;
+    for (int k = 0; k < 8; k++) ok = ok && s.h[k] < 2u * P;
+  }
+  for (int k = 0; k < 8; k++) chan[k] = digest[k];
+  chan[8] = n_draws;
+  if (!ok) chan[9] = 1;
+  for (int k = 0; k < 4; k++) alpha_out[k] = b2_reduce_word(s.h[k]);
+}
+
 static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
   STWO_HIP_CHECK_ARG(ctx, tw != nullptr && tw->device == ctx->device, "twiddles missing or from another device");
   STWO_HIP_CHECK_ARG(ctx, dom.log_size <= tw->root.log_size, "domain log %u exceeds twiddle root log %u",
@@ -106,7 +164,8 @@ static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_h
 }
 
 static int fold_circle(stwo_hip_ctx *ctx, uint32_t *const dst[4], const uint32_t *const src[4], uint32_t N,
-                       const qm31 &alpha, const stwo_hip_twiddles *tw, bool accumulate) {
+                       const qm31 &alpha, const stwo_hip_twiddles *tw, bool accumulate,
+                       const uint32_t *alpha_dev = nullptr) {
   Qm31Cols d;
   Qm31ColsConst s;
   for (int k = 0; k < 4; k++) {
@@ -116,12 +175,15 @@ static int fold_circle(stwo_hip_ctx *ctx, uint32_t *const dst[4], const uint32_t
   uint32_t n_out = 1u << (N - 1);
   return launch(ctx, accumulate ? "fold_circle_into_line" : "fold_circle_first_layer", fold_circle_kernel,
                 dim3((n_out + 255) / 256), dim3(256), 0, d, s, (const uint32_t *)(tw->d_itw + tw->len), N, alpha,
-                qm31_mul(alpha, alpha), (uint32_t)accumulate);
+                qm31_mul(alpha, alpha), (uint32_t)accumulate, alpha_dev);
 }
 
 // fold_step folds of a line evaluation of log size n (domain checked by caller).
+// With `alpha_dev` (alpha in device memory), `alpha_tmp` is a 4-word device
+// slot that receives alpha^(2^4k) for fold steps beyond the first 4.
 static int fold_line_steps(stwo_hip_ctx *ctx, const uint32_t *const src[4], uint32_t n, qm31 alpha, uint32_t fold_step,
-                           const stwo_hip_twiddles *tw, uint32_t *const dst[4]) {
+                           const stwo_hip_twiddles *tw, uint32_t *const dst[4], const uint32_t *alpha_dev = nullptr,
+                           uint32_t *alpha_tmp = nullptr) {
   const uint32_t *itw_end = tw->d_itw + tw->len;
   const uint32_t *cur[4] = {src[0], src[1], src[2], src[3]};
   uint32_t iter = 0;
@@ -147,13 +209,18 @@ static int fold_line_steps(stwo_hip_ctx *ctx, const uint32_t *const src[4], uint
     uint32_t blocks = ((1u << n_out_log) + 255) / 256;
     int rc;
     switch (step) {
-      case 1: rc = launch(ctx, "fold_line", fold_line_kernel<1>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
-      case 2: rc = launch(ctx, "fold_line", fold_line_kernel<2>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
-      case 3: rc = launch(ctx, "fold_line", fold_line_kernel<3>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
-      default: rc = launch(ctx, "fold_line", fold_line_kernel<4>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha); break;
+      case 1: rc = launch(ctx, "fold_line", fold_line_kernel<1>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha, alpha_dev); break;
+      case 2: rc = launch(ctx, "fold_line", fold_line_kernel<2>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha, alpha_dev); break;
+      case 3: rc = launch(ctx, "fold_line", fold_line_kernel<3>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha, alpha_dev); break;
+      default: rc = launch(ctx, "fold_line", fold_line_kernel<4>, dim3(blocks), dim3(256), 0, d, s, itw_end, n, alpha, alpha_dev); break;
     }
     if (rc) return rc;
     for (uint32_t k = 0; k < step; k++) alpha = qm31_mul(alpha, alpha);
+    if (alpha_dev && fold_step > step) {
+      rc = launch(ctx, "alpha_pow2k", alpha_pow2k_kernel, dim3(1), dim3(64), 0, alpha_dev, alpha_tmp, step);
+      if (rc) return rc;
+      alpha_dev = alpha_tmp;
+    }
     for (int k = 0; k < 4; k++) cur[k] = out[k];
     n = n_out_log;
     fold_step -= step;
@@ -293,28 +360,45 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   int rc = check_line_domain(ctx, dom, tw);
   if (rc) return rc;
 
-  HostChannel ch;
-  memcpy(ch.digest, channel_digest, 32);
-  ch.n_draws = *channel_n_draws;
-
-  // Scratch: one packed tree of the largest layer, and two layer buffers.
-  void *tree_p = nullptr, *lay_a = nullptr, *lay_b = nullptr;
+  // The layer loop runs on the stream without host round trips: after each
+  // layer's Merkle commit, fri_channel_step_kernel mixes the root into the
+  // device copy of the channel and draws the next alpha, which the fold
+  // kernels read from device memory.  The host reads the roots, the channel
+  // and the last layer once, at the end.
+  constexpr uint32_t kMaxRoots = 32;
+  // chan[0..10) | alpha[10..14) | alpha_tmp[14..18) | pad | roots[32 ..)
+  constexpr uint32_t kRootsOff = 32;
+  // One root per layer and at most log_size layers (log_size <= 28).
+  static_assert(kMaxRoots >= 28, "root slots");
+  void *tree_p = nullptr, *lay_a = nullptr, *lay_b = nullptr, *state_p = nullptr;
   rc = get_scratch(ctx, "fri_tree", layer_off(log_size + 1) * 4, &tree_p);
   if (rc) return rc;
   rc = get_scratch(ctx, "fri_layer_a", ((size_t)4 << (log_size - 1)) * 4, &lay_a);
   if (rc) return rc;
   rc = get_scratch(ctx, "fri_layer_b", ((size_t)4 << (log_size - 1)) * 4, &lay_b);
   if (rc) return rc;
+  rc = get_scratch(ctx, "fri_state", (kRootsOff + 8 * kMaxRoots) * 4, &state_p);
+  if (rc) return rc;
   uint32_t *tree = (uint32_t *)tree_p;
+  uint32_t *state = (uint32_t *)state_p;
+  uint32_t *d_chan = state, *d_alpha = state + 10, *d_alpha_tmp = state + 14, *d_roots = state + kRootsOff;
+  {
+    uint32_t init[10];
+    memcpy(init, channel_digest, 32);
+    init[8] = *channel_n_draws;
+    init[9] = 0;
+    rc = stwo_hip_upload(ctx, d_chan, init, 10);
+    if (rc) return rc;
+  }
   uint32_t nr = 0;
 
   auto commit_root = [&](const uint32_t *const cols[4], uint32_t lg) -> int {
     uint32_t logs[4] = {lg, lg, lg, lg};
     int r = stwo_hip_merkle_lifted_commit(ctx, cols, logs, 4, lg, tree, STWO_HIP_HASH_BLAKE2S);
     if (r) return r;
-    r = stwo_hip_download(ctx, roots + 8 * nr, tree, 8);
+    r = launch(ctx, "fri_channel_step", fri_channel_step_kernel, dim3(1), dim3(64), 0, d_chan, (const uint32_t *)tree,
+               d_roots + 8 * nr, d_alpha);
     if (r) return r;
-    ch.mix_root(roots + 8 * nr);
     nr++;
     return STWO_HIP_OK;
   };
@@ -329,8 +413,8 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   int cur_b = 0;
   uint32_t *cur[4];
   for (int k = 0; k < 4; k++) cur[k] = bufs[cur_b] + ((size_t)k << lg);
-  qm31 alpha = ch.draw_secure_felt();
-  rc = fold_circle(ctx, cur, column, log_size, alpha, tw, false);
+  const qm31 unused = qm31_make(0, 0, 0, 0);
+  rc = fold_circle(ctx, cur, column, log_size, unused, tw, false, d_alpha);
   if (rc) return rc;
   if (lg != last_log) {
     for (;;) {
@@ -338,10 +422,9 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
       uint32_t step = last ? lg - last_log : cfg.line_fold_step;
       rc = commit_root(cur, lg);
       if (rc) return rc;
-      alpha = ch.draw_secure_felt();
       uint32_t *nxt[4];
       for (int k = 0; k < 4; k++) nxt[k] = bufs[cur_b ^ 1] + ((size_t)k << (lg - step));
-      rc = fold_line_steps(ctx, cur, lg, alpha, step, tw, nxt);
+      rc = fold_line_steps(ctx, cur, lg, unused, step, tw, nxt, d_alpha, d_alpha_tmp);
       if (rc) return rc;
       cur_b ^= 1;
       for (int k = 0; k < 4; k++) cur[k] = nxt[k];
@@ -350,6 +433,14 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
       if (last) break;
     }
   }
+  std::vector<uint32_t> st(kRootsOff + 8 * nr);
+  rc = stwo_hip_download(ctx, st.data(), state, st.size());
+  if (rc) return rc;
+  if (st[9]) return set_error(ctx, STWO_HIP_EDEVICE, "fri_commit: channel draw did not terminate");
+  memcpy(roots, st.data() + kRootsOff, 32 * nr);
+  HostChannel ch;
+  memcpy(ch.digest, st.data(), 32);
+  ch.n_draws = st[8];
   // Last layer (fri.rs:274-291).
   size_t n = (size_t)1 << lg;
   std::vector<uint32_t> h(4 * n);
