@@ -229,8 +229,6 @@ __global__ void __launch_bounds__(256) bit_reverse_tiled_kernel(uint32_t *v, uin
 // stage, loaded once.  The circle layer (s = 0) reads 4 words (two (x, y)
 // pairs of line layer 0) for its 8 pairs.
 
-__device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
-
 template <bool INV>
 __device__ __forceinline__ void bfly(uint32_t &a, uint32_t &b, uint32_t t) {
   if (INV) {
@@ -251,14 +249,24 @@ struct StageTw {
   uint32_t t0[8], t1[4], t2[2], t3[1];
 };
 
+// Word `byte_off / 4` of a table whose base is wave-uniform: the base stays
+// in SGPRs and the lane part is a 32-bit offset (global_load ... saddr).
+__device__ __forceinline__ uint32_t gld(const uint32_t *base, uint32_t byte_off) {
+  return *(gconst_u32 *)((gconst_u8 *)as_global(base) + byte_off);
+}
+__device__ __forceinline__ void gst(uint32_t *base, uint32_t byte_off, uint32_t v) {
+  *(g_u32 *)((g_u8 *)as_global(base) + byte_off) = v;
+}
+
 __device__ __forceinline__ void load_level(uint32_t *dst, int cnt, const uint32_t *tw_end, uint32_t N, uint32_t s,
                                            uint32_t gb) {
   const uint32_t hs = gb >> (s + 1);
   if (s == 0) {
-    const uint32_t *l0 = tw_end - ((size_t)1 << (N - 1)) + (hs >> 1);
+    const uint32_t *l0 = tw_end - ((size_t)1 << (N - 1));
+    const uint32_t o = (hs >> 1) << 2;
     uint32_t w4[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) w4[k] = l0[k];
+    for (int k = 0; k < 4; k++) w4[k] = gld(l0, o + 4 * k);
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       uint32_t xx = w4[2 * (u >> 2)], yy = w4[2 * (u >> 2) + 1];
@@ -267,10 +275,11 @@ __device__ __forceinline__ void load_level(uint32_t *dst, int cnt, const uint32_
     }
     return;
   }
-  const uint32_t *lv = tw_end - ((size_t)1 << (N - s)) + hs;
+  const uint32_t *lv = tw_end - ((size_t)1 << (N - s));
+  const uint32_t o = hs << 2;
 #pragma unroll
   for (int u = 0; u < 8; u++)
-    if (u < cnt) dst[u] = lv[u];
+    if (u < cnt) dst[u] = gld(lv, o + 4 * u);
 }
 
 // Loads only the levels of the stage's active register bits [r_lo, r_hi).
@@ -321,6 +330,58 @@ __device__ __forceinline__ void stage_dispatch(uint32_t (&x)[16], const StageTw 
   }
 }
 
+// LDS exchange of a lane's 16 group elements.  Element j of the group at
+// tile bit B0 with base index `base` (group bits zero) lives at word
+// pad(base | j << B0) = (base + (base >> 4)) + (j << B0) + ((j << B0) >> 4)
+// (pad(i) = i + (i >> 4): one padding word per 16),
+// since the two parts have disjoint bits: the per-element part is a
+// compile-time offset, selected by a wave-uniform switch on B0.
+template <int B0>
+__device__ __forceinline__ constexpr uint32_t lds_off(int j) {
+  return ((uint32_t)j << B0) + (((uint32_t)j << B0) >> 4);
+}
+template <int B0>
+__device__ __forceinline__ void lds_put16(uint32_t *Lv, const uint32_t (&x)[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; j++) Lv[lds_off<B0>(j)] = x[j];
+}
+template <int B0>
+__device__ __forceinline__ void lds_get16(const uint32_t *Lv, uint32_t (&x)[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; j++) x[j] = Lv[lds_off<B0>(j)];
+}
+// Group bit b0 is at most kMaxTileLog - 4 = 9.
+__device__ __forceinline__ void lds_put(uint32_t *L, uint32_t b0, uint32_t base, const uint32_t (&x)[16]) {
+  uint32_t *Lv = L + base + (base >> 4);
+  switch (b0) {
+    case 0: lds_put16<0>(Lv, x); break;
+    case 1: lds_put16<1>(Lv, x); break;
+    case 2: lds_put16<2>(Lv, x); break;
+    case 3: lds_put16<3>(Lv, x); break;
+    case 4: lds_put16<4>(Lv, x); break;
+    case 5: lds_put16<5>(Lv, x); break;
+    case 6: lds_put16<6>(Lv, x); break;
+    case 7: lds_put16<7>(Lv, x); break;
+    case 8: lds_put16<8>(Lv, x); break;
+    default: lds_put16<9>(Lv, x); break;
+  }
+}
+__device__ __forceinline__ void lds_get(const uint32_t *L, uint32_t b0, uint32_t base, uint32_t (&x)[16]) {
+  const uint32_t *Lv = L + base + (base >> 4);
+  switch (b0) {
+    case 0: lds_get16<0>(Lv, x); break;
+    case 1: lds_get16<1>(Lv, x); break;
+    case 2: lds_get16<2>(Lv, x); break;
+    case 3: lds_get16<3>(Lv, x); break;
+    case 4: lds_get16<4>(Lv, x); break;
+    case 5: lds_get16<5>(Lv, x); break;
+    case 6: lds_get16<6>(Lv, x); break;
+    case 7: lds_get16<7>(Lv, x); break;
+    case 8: lds_get16<8>(Lv, x); break;
+    default: lds_get16<9>(Lv, x); break;
+  }
+}
+
 // One workgroup's share of a pass: workgroup `wg` of the pass's grid, with
 // 2^WG_LOG lanes (a tile of 2^T elements takes 2^(T-4) of them).
 //
@@ -355,8 +416,6 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     in = (const uint32_t *)readfirstlane_u64((uint64_t)in);
     out = (uint32_t *)readfirstlane_u64((uint64_t)out);
   }
-  gconst_u32 *gin = as_global(in);
-  g_u32 *gout = as_global(out);
   uint32_t *L = lds + slot * ((1u << p.T) + (1u << (p.T - 4)));
   const uint32_t ih_bits = p.lo - p.w;
   const uint32_t ih = tile & ((1u << ih_bits) - 1);
@@ -376,11 +435,14 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     // --- load in the top layout
     uint32_t cur = top_b0;
     {
+      // (gb | j << sh) & in_mask = (gb & in_mask) | ((j << sh) & in_mask):
+      // a lane offset plus a wave-uniform one.
       const uint32_t gb = gidx(local), sh = top_b0 - p.w + p.lo;
       if (rep > 0) __syncthreads();
       if (active) {
+        const uint32_t vo = (gb & in_mask) << 2;
 #pragma unroll
-        for (int j = 0; j < 16; j++) x[j] = gin[(gb | ((uint32_t)j << sh)) & in_mask];
+        for (int j = 0; j < 16; j++) x[j] = gld(in + (((uint32_t)j << sh) & in_mask), vo);
       }
     }
     for (uint32_t si = 0; si < p.n_stages; si++) {
@@ -391,13 +453,10 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
       StageTw t;
       load_stage_twiddles(t, p.tw_end, p.N, sh, gb, S.r_lo, S.r_hi);
       if (S.b0 != cur) {
-        const uint32_t cb = group_base(cur);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 16; j++) L[lds_pad(cb | ((uint32_t)j << cur))] = x[j];
+        lds_put(L, cur, group_base(cur), x);
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < 16; j++) x[j] = L[lds_pad(base | ((uint32_t)j << S.b0))];
+        lds_get(L, S.b0, base, x);
         cur = S.b0;
       }
       stage_dispatch<INV>(x, t, S.r_lo, S.r_hi);
@@ -408,19 +467,17 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     }
     // --- store in a lane-consecutive layout
     if (cur != top_b0 && cur < 5) {
-      const uint32_t cb = group_base(cur);
       __syncthreads();
-#pragma unroll
-      for (int j = 0; j < 16; j++) L[lds_pad(cb | ((uint32_t)j << cur))] = x[j];
+      lds_put(L, cur, group_base(cur), x);
       __syncthreads();
-#pragma unroll
-      for (int j = 0; j < 16; j++) x[j] = L[lds_pad(local | ((uint32_t)j << top_b0))];
+      lds_get(L, top_b0, local, x);
       cur = top_b0;
     }
     if (active) {
       const uint32_t gb = gidx(group_base(cur)), sh = cur - p.w + p.lo;
+      const uint32_t vo = gb << 2;
 #pragma unroll
-      for (int j = 0; j < 16; j++) gout[gb | ((uint32_t)j << sh)] = x[j];
+      for (int j = 0; j < 16; j++) gst(out + ((uint32_t)j << sh), vo, x[j]);
     }
   }
 }

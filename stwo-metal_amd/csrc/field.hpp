@@ -33,7 +33,12 @@ __host__ __device__ __forceinline__ uint32_t m31_neg(uint32_t a) { return m31_su
 __host__ __device__ __forceinline__ uint32_t m31_mul(uint32_t a, uint32_t b) {
   uint64_t p = (uint64_t)a * b;
   uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
+#ifdef __HIP_DEVICE_COMPILE__
+  // (hi:lo) >> 31 in one v_alignbit_b32 (the shift-or form compiles to three).
+  uint32_t s = (lo & P) + __builtin_amdgcn_alignbit(hi, lo, 31);
+#else
   uint32_t s = (lo & P) + ((hi << 1) | (lo >> 31));
+#endif
   return umin32(s, s - P);
 }
 __host__ __device__ inline uint32_t m31_pow(uint32_t a, uint64_t e) {
@@ -146,6 +151,8 @@ namespace stwo_hip {
 // base is wave-uniform) instead of flat ones with 64-bit per-lane addresses.
 typedef __attribute__((address_space(1))) const uint32_t gconst_u32;
 typedef __attribute__((address_space(1))) uint32_t g_u32;
+typedef __attribute__((address_space(1))) const uint8_t gconst_u8;
+typedef __attribute__((address_space(1))) uint8_t g_u8;
 __device__ __forceinline__ gconst_u32 *as_global(const uint32_t *p) { return (gconst_u32 *)p; }
 __device__ __forceinline__ g_u32 *as_global(uint32_t *p) { return (g_u32 *)p; }
 }  // namespace stwo_hip
