@@ -65,6 +65,8 @@ def lib():
                                               _u32p, ctypes.c_size_t, ctypes.POINTER(_u32p)]
         L.or_blake2s.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p]
         L.or_num_threads.restype = ctypes.c_int
+        L.or_pow_prefixed_digest.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p]
+        L.or_grind.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -298,6 +300,23 @@ def fri_commit(channel, cfg, col4, log_size, keep_layers=False):
     if n < 0:
         raise ValueError("fri_commit failed: %d" % n)
     return roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), layer_bufs
+
+
+# --- proof of work -------------------------------------------------------------
+
+def pow_prefixed_digest(digest: bytes, pow_bits):
+    out = ctypes.create_string_buffer(32)
+    lib().or_pow_prefixed_digest(bytes(digest), pow_bits, out)
+    return out.raw
+
+
+def grind(digest: bytes, pow_bits, low_bits=20):
+    """SimdBackend's grind for the Blake2s channel: smallest (hi << 32) | lo, lo < 2^low_bits."""
+    n = ctypes.c_uint64()
+    rc = lib().or_grind(bytes(digest), pow_bits, low_bits, ctypes.byref(n))
+    if rc != 0:
+        raise ValueError("grind failed")
+    return n.value
 
 
 def num_threads():

@@ -744,6 +744,51 @@ int or_fri_commit(or_channel *ch, const or_fri_config *cfg, const uint32_t *cons
 }
 
 /* ------------------------------------------------------------------------ */
+/* Proof of work (GrindOps for the Blake2s channel)                          */
+/* ------------------------------------------------------------------------ */
+#define OR_POW_PREFIX 0x12345678u
+/* H(POW_PREFIX || 0^12 || digest || le32(pow_bits)), STWO/core/channel/blake2s.rs:124-131. */
+void or_pow_prefixed_digest(const uint8_t digest[32], uint32_t pow_bits, uint8_t out[32]) {
+    uint8_t in[52];
+    uint32_t pre = OR_POW_PREFIX;
+    memcpy(in, &pre, 4);
+    memset(in + 4, 0, 12);
+    memcpy(in + 16, digest, 32);
+    memcpy(in + 48, &pow_bits, 4);
+    or_blake2s(in, 52, out);
+}
+/* Trailing zero bits of the first word of H(prefixed || le64(nonce))
+ * (blake2s.rs:132-137 counts them over the first 16 bytes; for pow_bits <= 32
+ * the first word decides, as SIMD/grind.rs:77-88 uses). */
+static uint32_t pow_zeros(const uint8_t prefixed[32], uint64_t nonce) {
+    uint8_t in[40], h[32];
+    memcpy(in, prefixed, 32);
+    memcpy(in + 32, &nonce, 8);
+    or_blake2s(in, 40, h);
+    uint32_t w; memcpy(&w, h, 4);
+    return w ? (uint32_t)__builtin_ctz(w) : 32;
+}
+/* SimdBackend's grind (SIMD/grind.rs:22-62): the smallest nonce (hi << 32) | lo
+ * with lo < 2^low_bits, hi = 0, 1, ..., whose hash has >= pow_bits trailing
+ * zeros.  Scans hi in order, lo in parallel.  Returns 0 on success. */
+int or_grind(const uint8_t digest[32], uint32_t pow_bits, uint32_t low_bits, uint64_t *nonce) {
+    if (pow_bits > 32 || low_bits > 30) return -1;
+    uint8_t pre[32];
+    or_pow_prefixed_digest(digest, pow_bits, pre);
+    for (uint64_t hi = 0; hi < 0x7fffffffull; hi++) {
+        uint64_t best = UINT64_MAX;
+        int64_t n = (int64_t)1 << low_bits;
+        #pragma omp parallel for schedule(static) reduction(min:best)
+        for (int64_t lo = 0; lo < n; lo++) {
+            uint64_t cand = (hi << 32) | (uint64_t)lo;
+            if (pow_zeros(pre, cand) >= pow_bits && cand < best) best = cand;
+        }
+        if (best != UINT64_MAX) { *nonce = best; return 0; }
+    }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Seeded synthetic inputs (not from the reference): SplitMix64 -> u64 % P.  */
 /* Shared with the library's bench generator so both sides see one input.    */
 /* ------------------------------------------------------------------------ */

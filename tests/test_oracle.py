@@ -355,3 +355,50 @@ def test_fri_rejects_high_degree():
     col = [oracle.fill_random(1 << n, 7, k) for k in range(4)]
     with pytest.raises(ValueError):
         oracle.fri_commit(oracle.Channel(), oracle.FriConfig(1, 1, 3, 1), col, n)
+
+
+# ------------------------------------------------------------ proof of work ---
+
+def verify_pow_nonce(digest, n_bits, nonce):
+    """Blake2sChannel::verify_pow_nonce (STWO/core/channel/blake2s.rs:122-139),
+    restated with hashlib: trailing zeros of the first 16 bytes as a u128."""
+    import struct
+    pre = hashlib.blake2s(struct.pack("<I", 0x12345678) + bytes(12) + bytes(digest) +
+                          struct.pack("<I", n_bits)).digest()
+    res = hashlib.blake2s(pre + struct.pack("<Q", nonce)).digest()
+    v = int.from_bytes(res[:16], "little")
+    tz = (v & -v).bit_length() - 1 if v else 128
+    return tz >= n_bits
+
+
+@pytest.mark.parametrize("seed,pow_bits", [(0, 2), (1, 5), (2, 9), (3, 12), (0x1111222233334344, 8)])
+def test_grind_is_smallest_valid_nonce(seed, pow_bits):
+    """SimdBackend's grind (SIMD/grind.rs:22-62) returns the smallest valid
+    nonce (hi << 32) | lo with lo < 2^20: checked by brute force with the
+    reference's verify rule; pow_bits 2 with mix_u64(0) is the shape of
+    test_grind_blake_is_deterministic."""
+    ch = oracle.Channel()
+    ch.mix_u64(seed)
+    d = ch.digest_bytes()
+    got = oracle.grind(d, pow_bits)
+    assert verify_pow_nonce(d, pow_bits, got)
+    assert got < (1 << 20)  # found at hi = 0 for these sizes
+    assert not any(verify_pow_nonce(d, pow_bits, n) for n in range(got))
+    assert oracle.pow_prefixed_digest(d, pow_bits) == hashlib.blake2s(
+        (0x12345678).to_bytes(4, "little") + bytes(12) + d + pow_bits.to_bytes(4, "little")).digest()
+
+
+def test_grind_hi_words_in_order():
+    """With a window of 2^2 low values per high word the search must move to
+    hi = 1, 2, ...: the answer is the smallest valid (hi, lo) in that lattice."""
+    ch = oracle.Channel()
+    ch.mix_u64(77)
+    d = ch.digest_bytes()
+    got = oracle.grind(d, 6, low_bits=2)
+    hi, lo = got >> 32, got & 0xFFFFFFFF
+    assert lo < 4 and verify_pow_nonce(d, 6, got)
+    for h in range(hi + 1):
+        for l in range(4):
+            n = (h << 32) | l
+            if n < got:
+                assert not verify_pow_nonce(d, 6, n)
