@@ -227,6 +227,31 @@ def serving(rates=(10.0, 30.0, 60.0), n_jobs=30, budget=96 << 30, n_slots=4):
     return out
 
 
+# Profiling names of the library's launches -> kernel symbols in rocprofv3 output.
+ROCPROF_NAMES = {
+    "rfft_pass_wg8": "void stwo_hip::fft_pass_kernel<false, 8>(stwo_hip::FftPass2)",
+    "rfft_pass_wg9": "void stwo_hip::fft_pass_kernel<false, 9>(stwo_hip::FftPass2)",
+    "ifft_pass_wg8": "void stwo_hip::fft_pass_kernel<true, 8>(stwo_hip::FftPass2)",
+    "ifft_pass_wg9": "void stwo_hip::fft_pass_kernel<true, 9>(stwo_hip::FftPass2)",
+    "merkle_commit_on_layer": "stwo_hip::merkle_layer_kernel(...)",
+    "merkle_nodes": "stwo_hip::merkle_nodes_kernel(...)",
+}
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_lde_merkle_256x2e20_b4.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
+    reading for a copy of known size), or None when none covers it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(sample_cols=N_COLS, runs=3):
     """The oracle (C, OpenMP over columns and nodes) on the same workload, on the host cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -301,22 +326,32 @@ def main():
 
     # Per-kernel device time (HIP events around every launch on the context stream).
     kernels = {k: {"launches": v[0], "ms_total": round(v[1], 3)} for k, v in prof.items() if k != "twiddle_tree"}
-    # Algorithmic bytes per step for each kernel class (twiddle reads excluded):
-    #   IFFT 8 B per input element; RFFT 4 B in + 4 B out per output element side;
-    #   Merkle leaves read 4 B per element per column + 32 B per leaf; nodes 96 B per node.
+    # Algorithmic HBM bytes per step for each kernel (twiddle reads excluded):
+    # an FFT pass reads its input domain once and writes its output domain
+    # once (the pass list comes from the library's planner); Merkle leaves
+    # read 4 B per element per column and write 32 B per leaf, node layers
+    # read 64 B and write 32 B per node.
+    alg = {}
+    for inverse, n, N in ((True, LOG_N, LOG_N), (False, LOG_N, LOG_N + LOG_BLOWUP)):
+        for p in sh.fft_plan(n, N, inverse):
+            name = "%s_pass_wg%d" % ("ifft" if inverse else "rfft", p["wg"])
+            rd = 4.0 * N_COLS * (1 << min(p["in_log"], N - p["rep"]))
+            wr = 4.0 * N_COLS * (1 << N)
+            alg[name] = alg.get(name, 0.0) + rd + wr
     n_out = 1 << (LOG_N + LOG_BLOWUP)
-    alg = {
-        "ifft_pass": 8.0 * elems,
-        "rfft_pass": 4.0 * elems + 4.0 * N_COLS * n_out,
-        "merkle_commit_on_layer": 4.0 * N_COLS * n_out + 32.0 * n_out,
-        "merkle_nodes": 96.0 * (n_out - 1),
-    }
-    dom_name = max((k for k in kernels if k in alg), key=lambda k: kernels[k]["ms_total"])
-    dom_ms = kernels[dom_name]["ms_total"] / args.steps
-    achieved = alg[dom_name] / (dom_ms * 1e-3) / 1e9
+    alg["merkle_commit_on_layer"] = 4.0 * N_COLS * n_out + 32.0 * n_out
+    alg["merkle_nodes"] = 96.0 * (n_out - 1)
     for k in kernels:
         if k in alg:
             kernels[k]["algorithmic_GBps"] = round(alg[k] / (kernels[k]["ms_total"] / args.steps * 1e-3) / 1e9, 1)
+    # The dominant kernel: the largest device time per step.  Its roofline
+    # point is per launch: algorithmic bytes of one launch / its mean duration.
+    dom_name = max((k for k in kernels if k in alg), key=lambda k: kernels[k]["ms_total"])
+    dom_launches_per_step = kernels[dom_name]["launches"] / args.steps
+    dom_launch_ms = kernels[dom_name]["ms_total"] / kernels[dom_name]["launches"]
+    dom_bytes = alg[dom_name] / dom_launches_per_step
+    achieved = dom_bytes / (dom_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom_name)
     log("kernels: %s" % json.dumps(kernels))
 
     serve = None
@@ -368,11 +403,14 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dom_name,
+            "rocprof_name": ROCPROF_NAMES.get(dom_name, dom_name),
+            "launch_ms": round(dom_launch_ms, 4),
+            "algorithmic_bytes": int(dom_bytes),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
         "kernels": kernels,

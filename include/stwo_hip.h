@@ -212,6 +212,88 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
                         stwo_hip_fri_config config, const uint32_t *const column[4], uint32_t log_size,
                         const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly);
 
+/* ---- decommitment ---------------------------------------------------------
+ * MerkleProverLifted::decommit (STWO/prover/vcs_lifted/prover.rs:58-201;
+ * Metal MB/decommit.rs:96-220) on a packed lifted tree `layers` of lifting
+ * log size L (as written by stwo_hip_merkle_lifted_commit for the same
+ * columns).  `queries`: n_queries positions in [0, 2^L), non-decreasing
+ * (host memory).  Outputs, host memory:
+ *   values[c * n_queries + q] = column c at lift(queries[q], L - log_sizes[c])
+ *     (the reference's queried_values, column order as given);
+ *   hash_witness: the witness hashes (8 words each) in the reference's order,
+ *     *n_witness of them (at most witness_cap);
+ *   aux_nodes (optional, MerkleDecommitmentLiftedAux::all_node_values): 10
+ *     words per node [layer log size, index, hash], both children of every
+ *     visited parent, from the leaf layer up, *n_aux nodes (<= aux_cap).
+ * One device gather; synchronizes. */
+int stwo_hip_merkle_lifted_decommit(stwo_hip_ctx *ctx, const uint32_t *layers, uint32_t lifting_log_size,
+                                    const uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                                    const uint32_t *queries, size_t n_queries, uint32_t *values,
+                                    uint32_t *hash_witness, size_t witness_cap, size_t *n_witness,
+                                    uint32_t *aux_nodes, size_t aux_cap, size_t *n_aux);
+
+/* FriProver (STWO/prover/fri.rs:117-347) kept for decommitment:
+ * stwo_hip_fri_prover_commit is stwo_hip_fri_commit that keeps every
+ * committed layer's tree and evaluation (one device allocation, about
+ * 8x the input column's size); the input column is borrowed and must stay
+ * valid until the prover is destroyed, as the reference's prover borrows
+ * its first-layer evaluation. */
+typedef struct stwo_hip_fri_prover stwo_hip_fri_prover;
+int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                               stwo_hip_fri_config config, const uint32_t *const column[4], uint32_t log_size,
+                               const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots,
+                               uint32_t *last_layer_poly, stwo_hip_fri_prover **out);
+/* FriProver::decommit (fri.rs:294-347): draws config.n_queries positions
+ * from the channel (draw_queries, STWO/core/queries.rs:11-28), updating
+ * channel_digest / channel_n_draws; query_positions receives the sorted,
+ * deduplicated positions (*n_positions of them, at most n_queries) and
+ * unsorted_queries the n_queries raw draws.  Decommits every layer (one
+ * device gather); the proofs are then read per layer with
+ * stwo_hip_fri_prover_layer_proof.  Once per prover. */
+int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *prover, uint8_t channel_digest[32],
+                                 uint32_t *channel_n_draws, uint32_t *query_positions, uint32_t *n_positions,
+                                 uint32_t *unsorted_queries);
+/* FriLayerProof of committed layer `layer` (0 = first/circle layer;
+ * fri.rs:372-453): commitment (root, 8 words); fri_witness (4 words per
+ * QM31, *n_fri of them); hash_witness (8 words per hash, *n_hash).  If
+ * `positions` is not NULL it receives the layer's decommitment positions
+ * (*n_pos <= pos_cap) and `values` (if not NULL) their QM31 values, 4 words
+ * each (FriLayerProofAux::all_values). */
+int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *prover, uint32_t layer,
+                                    uint32_t commitment[8], uint32_t *fri_witness, size_t fri_cap, size_t *n_fri,
+                                    uint32_t *hash_witness, size_t hash_cap, size_t *n_hash, uint32_t *positions,
+                                    uint32_t *values, size_t pos_cap, size_t *n_pos);
+int stwo_hip_fri_prover_destroy(stwo_hip_ctx *ctx, stwo_hip_fri_prover *prover);
+
+/* ---- proof of work -----------------------------------------------------------
+ * GrindOps::grind for the Blake2s channel as SimdBackend computes it
+ * (STWO/prover/backend/simd/grind.rs:22-62; Metal MB/blake2s.rs:399-443,
+ * SH/grind.metal): the smallest nonce (hi << 32) | lo with lo < 2^20, hi =
+ * 0, 1, ..., such that H(H(POW_PREFIX || 0^12 || digest || le32(pow_bits)) ||
+ * le64(nonce)) has at least pow_bits trailing zero bits
+ * (Blake2sChannel::verify_pow_nonce, STWO/core/channel/blake2s.rs:122-139).
+ * pow_bits <= 32.  Does not mix the nonce into the channel.  Synchronizes. */
+int stwo_hip_grind(stwo_hip_ctx *ctx, const uint8_t channel_digest[32], uint32_t pow_bits, uint64_t *nonce);
+
+/* ---- SimdBackend layout interop --------------------------------------------
+ * No single counterpart in the reference: conversions so that columns can
+ * move between this library and a SimdBackend-driven prover.
+ * stwo_hip_coeffs_simd_transpose: SimdBackend's IFFT leaves coefficients of
+ * log size > 16 with their 16-element vectors transposed (transpose_vecs,
+ * STWO/prover/backend/simd/fft/mod.rs:22-56, from simd/fft/ifft.rs:42-56;
+ * simd/circle.rs:503-512 transposes back before splitting); this library
+ * keeps natural order.  The map is its own inverse, in place, a no-op at
+ * log size <= 16; `col` must be 16-byte aligned.
+ * stwo_hip_twiddles_simd_doubled: SimdBackend's twiddle tree (doubled values
+ * 2t, same layout, simd/circle.rs:449-486) from a tree of this library;
+ * outputs hold twiddles_buffers' `len` words each.
+ * stwo_hip_canonicalize: maps the redundant value P (which SimdBackend may
+ * emit, simd/m31.rs:58) to 0, in place. */
+int stwo_hip_coeffs_simd_transpose(stwo_hip_ctx *ctx, uint32_t *col, uint32_t log_size);
+int stwo_hip_twiddles_simd_doubled(stwo_hip_ctx *ctx, const stwo_hip_twiddles *tw, uint32_t *out_twiddles,
+                                   uint32_t *out_itwiddles);
+int stwo_hip_canonicalize(stwo_hip_ctx *ctx, uint32_t *col, size_t n);
+
 /* ---- batched commitment of uneven columns -------------------------------
  * One plan for a whole set of columns of different log sizes, replacing the
  * per-column dispatch of the reference's TreeBuilder path

@@ -403,9 +403,19 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
   const uint64_t gtile = ((uint64_t)wg << (WG_LOG - tpt_log)) | slot;
   const uint32_t dom_log = p.N - p.rep_log;
   const uint32_t tpc_log = dom_log - p.T;  // log2 tiles per column
-  const uint64_t col_idx = gtile >> tpc_log;
-  const bool active = col_idx < p.n_cols;
-  const uint32_t tile = (uint32_t)(gtile & ((1ull << tpc_log) - 1));
+  // Tiles are numbered column-fastest: consecutive workgroups, which the
+  // dispatcher spreads round-robin over the XCDs, take the same tile of
+  // successive columns, so each XCD's L2 serves that tile's twiddles to all
+  // of its workgroups.  The host checks that tile numbers fit 32 bits.
+  const bool active = gtile < ((uint64_t)p.n_cols << tpc_log);
+  // Lanes past the last tile (a workgroup holds several small tiles) take
+  // tile 0 of column 0: they load no data and store nothing, but they do
+  // read twiddles, and tile 0's twiddle reads stay inside the table.
+  uint32_t tile = 0, col_idx = 0;
+  if (active) {
+    tile = u32div((uint32_t)gtile, p.cols_div);
+    col_idx = (uint32_t)gtile - tile * p.n_cols;
+  }
   const uint32_t *in = nullptr;
   uint32_t *out = nullptr;
   if (active) {
@@ -633,18 +643,21 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
     uint64_t total_tiles = (uint64_t)n_cols << (N - p.rep_log - p.T);
     uint32_t tiles_per_wg = 1u << (wg_log + 4 - p.T);
     uint64_t wgs = (total_tiles + tiles_per_wg - 1) / tiles_per_wg;
-    STWO_HIP_CHECK_ARG(ctx, wgs < (1ull << 31), "internal: grid too large");
+    STWO_HIP_CHECK_ARG(ctx, wgs * tiles_per_wg <= (1ull << 32), "FFT group of %u columns of log %u too large",
+                       n_cols, N);
+    p.cols_div = make_u32div(n_cols);
     out.push_back({p, (uint32_t)wgs, wg_log});
   }
   return STWO_HIP_OK;
 }
 
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log) {
+  // Profiling names follow the instantiation: *_wg9 = fft_pass_kernel<INV, 9>.
   if (wg_log == 9)
-    return inverse ? launch(ctx, "ifft_pass", fft_pass_kernel<true, 9>, dim3(wgs), dim3(512), 0, p)
-                   : launch(ctx, "rfft_pass", fft_pass_kernel<false, 9>, dim3(wgs), dim3(512), 0, p);
-  return inverse ? launch(ctx, "ifft_pass", fft_pass_kernel<true, 8>, dim3(wgs), dim3(256), 0, p)
-                 : launch(ctx, "rfft_pass", fft_pass_kernel<false, 8>, dim3(wgs), dim3(256), 0, p);
+    return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9>, dim3(wgs), dim3(512), 0, p)
+                   : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9>, dim3(wgs), dim3(512), 0, p);
+  return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8>, dim3(wgs), dim3(256), 0, p)
+                 : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8>, dim3(wgs), dim3(256), 0, p);
 }
 
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,

@@ -12,11 +12,14 @@
 // output held in VGPRs (alpha, alpha^2, ... per step, SIMD/fri.rs:52-111).
 #include <string.h>
 
+#include <algorithm>
+
 #include <vector>
 
 #include "blake2s.hpp"
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
@@ -344,12 +347,32 @@ int stwo_hip_fold_line(stwo_hip_ctx *ctx, const uint32_t *const src[4], stwo_hip
                          tw, dst);
 }
 
-int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
-                        stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
-                        const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly) {
-  if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws && column && roots && n_roots && last_layer_poly,
-                     "fri_commit: NULL argument");
+}  // extern "C"
+
+namespace stwo_hip {
+// The layer schedule of FriProver::commit (fri.rs:188-262): the first
+// (circle) layer of log N folds once into the line domain of log N - 1;
+// inner layers fold line_fold_step at a time, the last one exactly down to
+// the last layer of log (log_last_layer_degree_bound + log_blowup_factor).
+std::vector<FriLayerShape> fri_schedule(const stwo_hip_fri_config &cfg, uint32_t log_size) {
+  const uint32_t last_log = cfg.log_last_layer_degree_bound + cfg.log_blowup_factor;
+  std::vector<FriLayerShape> out;
+  out.push_back({log_size, 1});
+  uint32_t lg = log_size - 1;
+  if (lg != last_log) {
+    for (;;) {
+      bool last = !(lg > last_log + cfg.line_fold_step);
+      uint32_t step = last ? lg - last_log : cfg.line_fold_step;
+      out.push_back({lg, step});
+      lg -= step;
+      if (last) break;
+    }
+  }
+  return out;
+}
+
+static int check_fri_args(stwo_hip_ctx *ctx, const stwo_hip_fri_config &cfg, uint32_t log_size,
+                          const stwo_hip_twiddles *tw) {
   const uint32_t last_log = cfg.log_last_layer_degree_bound + cfg.log_blowup_factor;
   STWO_HIP_CHECK_ARG(ctx, cfg.line_fold_step >= 1, "Line fold step must be positive.");
   STWO_HIP_CHECK_ARG(ctx, cfg.log_last_layer_degree_bound <= 10, "log_last_layer_degree_bound must be <= 10");
@@ -357,9 +380,23 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   STWO_HIP_CHECK_ARG(ctx, log_size >= 3 && log_size - 1 >= last_log && log_size <= 28,
                      "fri_commit: log size %u incompatible with last layer log %u", log_size, last_log);
   stwo_hip_coset dom = {half_odds_initial(log_size - 1), log_size - 1};
-  int rc = check_line_domain(ctx, dom, tw);
-  if (rc) return rc;
+  return check_line_domain(ctx, dom, tw);
+}
 
+// FriProver::commit.  With `keep`, every committed layer's tree and every
+// folded evaluation stays in `keep->mem` (laid out by the caller from
+// fri_schedule); without, one scratch tree and two scratch layer buffers
+// are reused.
+static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                           const stwo_hip_fri_config &cfg, const uint32_t *const column[4], uint32_t log_size,
+                           const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly,
+                           stwo_hip_fri_prover *keep) {
+  int rc = check_fri_args(ctx, cfg, log_size, tw);
+  if (rc) return rc;
+  const uint32_t last_log = cfg.log_last_layer_degree_bound + cfg.log_blowup_factor;
+  const std::vector<FriLayerShape> sched = fri_schedule(cfg, log_size);
+  if (keep && keep->layers.size() != sched.size())
+    return set_error(ctx, STWO_HIP_EINVAL, "internal: FRI prover storage does not match the schedule");
   // The layer loop runs on the stream without host round trips: after each
   // layer's Merkle commit, fri_channel_step_kernel mixes the root into the
   // device copy of the channel and draws the next alpha, which the fold
@@ -371,15 +408,17 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   // One root per layer and at most log_size layers (log_size <= 28).
   static_assert(kMaxRoots >= 28, "root slots");
   void *tree_p = nullptr, *lay_a = nullptr, *lay_b = nullptr, *state_p = nullptr;
-  rc = get_scratch(ctx, "fri_tree", layer_off(log_size + 1) * 4, &tree_p);
-  if (rc) return rc;
-  rc = get_scratch(ctx, "fri_layer_a", ((size_t)4 << (log_size - 1)) * 4, &lay_a);
-  if (rc) return rc;
-  rc = get_scratch(ctx, "fri_layer_b", ((size_t)4 << (log_size - 1)) * 4, &lay_b);
-  if (rc) return rc;
+  if (!keep) {
+    rc = get_scratch(ctx, "fri_tree", layer_off(log_size + 1) * 4, &tree_p);
+    if (rc) return rc;
+    rc = get_scratch(ctx, "fri_layer_a", ((size_t)4 << (log_size - 1)) * 4, &lay_a);
+    if (rc) return rc;
+    rc = get_scratch(ctx, "fri_layer_b", ((size_t)4 << (log_size - 1)) * 4, &lay_b);
+    if (rc) return rc;
+  }
   rc = get_scratch(ctx, "fri_state", (kRootsOff + 8 * kMaxRoots) * 4, &state_p);
   if (rc) return rc;
-  uint32_t *tree = (uint32_t *)tree_p;
+  uint32_t *scratch_tree = (uint32_t *)tree_p;
   uint32_t *state = (uint32_t *)state_p;
   uint32_t *d_chan = state, *d_alpha = state + 10, *d_alpha_tmp = state + 14, *d_roots = state + kRootsOff;
   {
@@ -394,6 +433,10 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
 
   auto commit_root = [&](const uint32_t *const cols[4], uint32_t lg) -> int {
     uint32_t logs[4] = {lg, lg, lg, lg};
+    if (keep && nr >= keep->layers.size()) return set_error(ctx, STWO_HIP_EINVAL, "internal: FRI layer overflow");
+    uint32_t *tree = keep ? keep->layers[nr].tree : scratch_tree;
+    if (keep)
+      for (int k = 0; k < 4; k++) keep->layers[nr].cols[k] = cols[k];
     int r = stwo_hip_merkle_lifted_commit(ctx, cols, logs, 4, lg, tree, STWO_HIP_HASH_BLAKE2S);
     if (r) return r;
     r = launch(ctx, "fri_channel_step", fri_channel_step_kernel, dim3(1), dim3(64), 0, d_chan, (const uint32_t *)tree,
@@ -410,9 +453,13 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   uint32_t lg = log_size - 1;
   uint32_t line_init = half_odds_initial(lg);
   uint32_t *bufs[2] = {(uint32_t *)lay_a, (uint32_t *)lay_b};
-  int cur_b = 0;
   uint32_t *cur[4];
-  for (int k = 0; k < 4; k++) cur[k] = bufs[cur_b] + ((size_t)k << lg);
+  // Evaluation buffer of the layer of log `l` that fold number `f` writes.
+  auto layer_buf = [&](int f, uint32_t l, int k) -> uint32_t * {
+    return keep ? keep->evals[f] + ((size_t)k << l) : bufs[f & 1] + ((size_t)k << l);
+  };
+  for (int k = 0; k < 4; k++) cur[k] = layer_buf(0, lg, k);
+  int n_folds = 1;
   const qm31 unused = qm31_make(0, 0, 0, 0);
   rc = fold_circle(ctx, cur, column, log_size, unused, tw, false, d_alpha);
   if (rc) return rc;
@@ -423,10 +470,10 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
       rc = commit_root(cur, lg);
       if (rc) return rc;
       uint32_t *nxt[4];
-      for (int k = 0; k < 4; k++) nxt[k] = bufs[cur_b ^ 1] + ((size_t)k << (lg - step));
+      for (int k = 0; k < 4; k++) nxt[k] = layer_buf(n_folds, lg - step, k);
       rc = fold_line_steps(ctx, cur, lg, unused, step, tw, nxt, d_alpha, d_alpha_tmp);
       if (rc) return rc;
-      cur_b ^= 1;
+      n_folds++;
       for (int k = 0; k < 4; k++) cur[k] = nxt[k];
       line_init = (uint32_t)(((uint64_t)line_init << step) & IDX_MASK);
       lg -= step;
@@ -437,7 +484,10 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   rc = stwo_hip_download(ctx, st.data(), state, st.size());
   if (rc) return rc;
   if (st[9]) return set_error(ctx, STWO_HIP_EDEVICE, "fri_commit: channel draw did not terminate");
+  if (nr != sched.size()) return set_error(ctx, STWO_HIP_EDEVICE, "internal: FRI schedule mismatch");
   memcpy(roots, st.data() + kRootsOff, 32 * nr);
+  if (keep)
+    for (uint32_t k = 0; k < nr; k++) memcpy(keep->layers[k].root, roots + 8 * k, 32);
   HostChannel ch;
   memcpy(ch.digest, st.data(), 32);
   ch.n_draws = st[8];
@@ -469,7 +519,212 @@ int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t 
   memcpy(channel_digest, ch.digest, 32);
   *channel_n_draws = ch.n_draws;
   *n_roots = nr;
+  if (keep) {
+    keep->last_layer_poly.assign(last_layer_poly, last_layer_poly + 4 * bound);
+    keep->last_log = lg;
+    for (int k = 0; k < 4; k++) keep->last_layer_eval[k] = cur[k];
+  }
   return STWO_HIP_OK;
 }
+
+}  // namespace stwo_hip
+
+extern "C" {
+
+int stwo_hip_fri_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                        stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
+                        const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws && column && roots && n_roots && last_layer_poly,
+                     "fri_commit: NULL argument");
+  return fri_commit_impl(ctx, channel_digest, channel_n_draws, cfg, column, log_size, tw, roots, n_roots,
+                         last_layer_poly, nullptr);
+}
+
+int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                               stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
+                               const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots,
+                               uint32_t *last_layer_poly, stwo_hip_fri_prover **out) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws && column && roots && n_roots && last_layer_poly && out,
+                     "fri_prover_commit: NULL argument");
+  *out = nullptr;
+  for (int k = 0; k < 4; k++) STWO_HIP_CHECK_ARG(ctx, column[k] != nullptr, "fri_prover_commit: NULL column");
+  int rc = check_fri_args(ctx, cfg, log_size, tw);
+  if (rc) return rc;
+  auto *pr = new stwo_hip_fri_prover();
+  pr->device = ctx->device;
+  pr->cfg = cfg;
+  pr->log_size = log_size;
+  const std::vector<FriLayerShape> sched = fri_schedule(cfg, log_size);
+  // One allocation: a packed tree per committed layer, then the evaluation
+  // written by each fold (the last one is the uncommitted last layer).
+  size_t words = 0;
+  std::vector<size_t> tree_off, eval_off;
+  for (const FriLayerShape &l : sched) {
+    tree_off.push_back(words);
+    words += layer_off(l.log_size + 1);
+  }
+  uint32_t lg = log_size;
+  for (size_t f = 0; f < sched.size(); f++) {
+    lg -= sched[f].fold_step;
+    eval_off.push_back(words);
+    words += (size_t)4 << lg;
+  }
+  rc = stwo_hip_alloc(ctx, words, &pr->mem);
+  if (rc) {
+    delete pr;
+    return rc;
+  }
+  pr->mem_words = words;
+  pr->layers.resize(sched.size());
+  for (size_t k = 0; k < sched.size(); k++) {
+    pr->layers[k].shape = sched[k];
+    pr->layers[k].tree = pr->mem + tree_off[k];
+  }
+  for (size_t f = 0; f < sched.size(); f++) pr->evals.push_back(pr->mem + eval_off[f]);
+  rc = fri_commit_impl(ctx, channel_digest, channel_n_draws, cfg, column, log_size, tw, roots, n_roots,
+                       last_layer_poly, pr);
+  if (rc) {
+    stwo_hip_free(ctx, pr->mem);
+    delete pr;
+    return rc;
+  }
+  *out = pr;
+  return STWO_HIP_OK;
+}
+
+// FriProver::decommit (fri.rs:294-347): draw the query positions from the
+// channel (draw_queries, core/queries.rs:11-28; Queries::new sorts and
+// deduplicates), then decommit every committed layer:
+// compute_decommitment_positions_and_witness_evals (fri.rs:455-489) with the
+// layer's fold step (1 for the circle layer), and the Merkle decommitment
+// of the layer's tree on those positions; the queries fold by the layer's
+// step between layers (Queries::fold, queries.rs:56-63).  Every value and
+// hash read of all layers is one device gather.
+int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uint8_t channel_digest[32],
+                                 uint32_t *channel_n_draws, uint32_t *query_positions, uint32_t *n_positions,
+                                 uint32_t *unsorted_queries) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, pr && channel_digest && channel_n_draws && query_positions && n_positions &&
+                              unsorted_queries,
+                     "fri_prover_decommit: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, pr->device == ctx->device, "fri_prover_decommit: prover of another device");
+  STWO_HIP_CHECK_ARG(ctx, !pr->decommitted, "fri_prover_decommit: already decommitted");
+  const uint32_t n_queries = pr->cfg.n_queries;
+  STWO_HIP_CHECK_ARG(ctx, n_queries >= 1 && n_queries <= (1u << 20), "fri_prover_decommit: n_queries %u invalid",
+                     n_queries);
+  HostChannel ch;
+  memcpy(ch.digest, channel_digest, 32);
+  ch.n_draws = *channel_n_draws;
+  const uint32_t N = pr->log_size;
+  const uint32_t mask = (uint32_t)((1ull << N) - 1);
+  std::vector<uint32_t> raw;
+  while (raw.size() < n_queries) {
+    uint32_t w[8];
+    ch.draw_u32s(w);
+    for (int k = 0; k < 8 && raw.size() < n_queries; k++) raw.push_back(w[k] & mask);
+  }
+  std::vector<uint32_t> queries = raw;
+  std::sort(queries.begin(), queries.end());
+  queries.erase(std::unique(queries.begin(), queries.end()), queries.end());
+
+  struct LayerPlan {
+    std::vector<uint32_t> positions;
+    std::vector<size_t> val_off;   // gather offset of each coordinate read (4 per position)
+    std::vector<uint8_t> witness;  // 1 if the position is not a query
+    MerkleDecommitPlan merkle;
+  };
+  Gatherer g;
+  std::vector<LayerPlan> plans(pr->layers.size());
+  std::vector<uint32_t> lq = queries;
+  for (size_t k = 0; k < pr->layers.size(); k++) {
+    const FriLayerKeep &layer = pr->layers[k];
+    const uint32_t step = layer.shape.fold_step;
+    LayerPlan &pl = plans[k];
+    for (size_t i = 0; i < lq.size();) {
+      size_t j = i;
+      while (j < lq.size() && (lq[j] >> step) == (lq[i] >> step)) j++;
+      const uint32_t start = (lq[i] >> step) << step;
+      size_t next = i;
+      for (uint32_t pos = start; pos < start + (1u << step); pos++) {
+        pl.positions.push_back(pos);
+        for (int c = 0; c < 4; c++) pl.val_off.push_back(g.add(layer.cols[c] + pos, 1));
+        const bool queried = next < j && lq[next] == pos;
+        if (queried) next++;
+        pl.witness.push_back(queried ? 0 : 1);
+      }
+      i = j;
+    }
+    plan_merkle_decommit(g, layer.tree, layer.shape.log_size, pl.positions, pl.merkle);
+    std::vector<uint32_t> folded;
+    for (uint32_t q : lq) {
+      uint32_t f = q >> step;
+      if (folded.empty() || folded.back() != f) folded.push_back(f);
+    }
+    lq.swap(folded);
+  }
+  int rc = g.run(ctx);
+  if (rc) return rc;
+  pr->proofs.assign(pr->layers.size(), FriLayerDecommit());
+  for (size_t k = 0; k < plans.size(); k++) {
+    const LayerPlan &pl = plans[k];
+    FriLayerDecommit &d = pr->proofs[k];
+    d.positions = pl.positions;
+    for (size_t i = 0; i < pl.positions.size(); i++) {
+      for (int c = 0; c < 4; c++) d.values.push_back(g.out[pl.val_off[4 * i + c]]);
+      if (pl.witness[i])
+        for (int c = 0; c < 4; c++) d.fri_witness.push_back(g.out[pl.val_off[4 * i + c]]);
+    }
+    finish_merkle_decommit(g, pl.merkle, d.hash_witness, nullptr);
+  }
+  pr->query_positions = queries;
+  pr->unsorted_queries = raw;
+  pr->decommitted = true;
+  memcpy(query_positions, queries.data(), queries.size() * 4);
+  *n_positions = (uint32_t)queries.size();
+  memcpy(unsorted_queries, raw.data(), raw.size() * 4);
+  memcpy(channel_digest, ch.digest, 32);
+  *channel_n_draws = ch.n_draws;
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *pr, uint32_t layer,
+                                    uint32_t commitment[8], uint32_t *fri_witness, size_t fri_cap, size_t *n_fri,
+                                    uint32_t *hash_witness, size_t hash_cap, size_t *n_hash, uint32_t *positions,
+                                    uint32_t *values, size_t pos_cap, size_t *n_pos) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, pr && commitment && n_fri && n_hash && n_pos, "fri_prover_layer_proof: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, pr->decommitted, "fri_prover_layer_proof: not decommitted yet");
+  STWO_HIP_CHECK_ARG(ctx, layer < pr->proofs.size(), "fri_prover_layer_proof: layer %u of %zu", layer,
+                     pr->proofs.size());
+  const FriLayerDecommit &d = pr->proofs[layer];
+  memcpy(commitment, pr->layers[layer].root, 32);
+  *n_fri = d.fri_witness.size() / 4;
+  *n_hash = d.hash_witness.size() / 8;
+  *n_pos = d.positions.size();
+  STWO_HIP_CHECK_ARG(ctx, *n_fri <= fri_cap && *n_hash <= hash_cap && (!positions || *n_pos <= pos_cap),
+                     "fri_prover_layer_proof: capacity too small (%zu, %zu, %zu needed)", *n_fri, *n_hash, *n_pos);
+  STWO_HIP_CHECK_ARG(ctx, (fri_witness || !*n_fri) && (hash_witness || !*n_hash),
+                     "fri_prover_layer_proof: NULL output");
+  if (*n_fri) memcpy(fri_witness, d.fri_witness.data(), d.fri_witness.size() * 4);
+  if (*n_hash) memcpy(hash_witness, d.hash_witness.data(), d.hash_witness.size() * 4);
+  if (positions) {
+    memcpy(positions, d.positions.data(), d.positions.size() * 4);
+    if (values) memcpy(values, d.values.data(), d.values.size() * 4);
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_fri_prover_destroy(stwo_hip_ctx *ctx, stwo_hip_fri_prover *prover) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  if (!prover) return STWO_HIP_OK;
+  int rc = stwo_hip_sync(ctx);
+  int rc2 = stwo_hip_free(ctx, prover->mem);
+  delete prover;
+  return rc ? rc : rc2;
+}
+
+
 
 }  // extern "C"

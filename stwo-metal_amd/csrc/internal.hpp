@@ -2,6 +2,7 @@
 // part of the C ABI): FFT pass descriptors and the launchers the batched
 // path reuses with tables it uploads itself.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <vector>
@@ -16,6 +17,33 @@ struct FftCol {
 };
 
 constexpr uint32_t kMaxStages = 4;
+
+// Unsigned 32-bit division by a fixed divisor d >= 1 with a multiply-high
+// (Granlund and Montgomery, "Division by Invariant Integers using
+// Multiplication", PLDI 1994, figure 4.1): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0)
+// where t = mulhi(m, n), for every n < 2^32.
+struct U32Div {
+  uint32_t m, s1, s2, d;
+};
+inline U32Div make_u32div(uint32_t d) {
+  uint32_t l = 0;
+  while (l < 32 && (1ull << l) < d) l++;
+  U32Div r;
+  r.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+  r.s1 = l ? 1 : 0;
+  r.s2 = l ? l - 1 : 0;
+  r.d = d;
+  return r;
+}
+__host__ __device__ __forceinline__ uint32_t u32div(uint32_t n, const U32Div &v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t = __umulhi(v.m, n);
+#else
+  uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
+#endif
+  return (t + ((n - t) >> v.s1)) >> v.s2;
+}
 
 struct FftStage {
   uint32_t b0, r_lo, r_hi;
@@ -32,6 +60,8 @@ struct FftPass2 {
   uint32_t in_log;
   uint32_t scale;
   uint32_t rep_log;  // output blocks per tile (first RFFT pass of a blowup)
+  // Division of a tile number by n_cols (tiles are numbered column-fastest).
+  U32Div cols_div;
 };
 
 struct PassLaunch {
@@ -46,6 +76,85 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log);
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
                      uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log);
+
+// ---- decommitment: batched device gather -----------------------------------
+
+// Collects (device source, word count) reads; run() gathers them all with
+// one kernel launch into `out` (host), in the order they were added.
+struct Gatherer {
+  struct Op {
+    uintptr_t src;
+    uint32_t n, dst;
+  };
+  std::vector<Op> ops_;
+  size_t words = 0;
+  std::vector<uint32_t> out;
+  // Returns the word offset of this read in `out`.
+  size_t add(const uint32_t *src, uint32_t n);
+  int run(stwo_hip_ctx *ctx);
+};
+
+// Hash-witness traversal of a lifted Merkle decommitment: one entry per
+// visited parent, from the leaf layer up.
+struct MerkleDecommitPlan {
+  struct Parent {
+    uint32_t child_layer;  // log size of the layer holding the two children
+    uint32_t parent;       // parent index (children 2 * parent, 2 * parent + 1)
+    int witness_child;     // child (0 or 1) that is a witness, or -1
+    size_t off;            // gather offset of the two children (16 words)
+  };
+  std::vector<Parent> parents;
+};
+void plan_merkle_decommit(Gatherer &g, const uint32_t *tree, uint32_t L, const std::vector<uint32_t> &queries,
+                          MerkleDecommitPlan &plan);
+// witness: 8 words per hash; aux (optional): 10 words per node
+// [layer log size, index, hash].
+void finish_merkle_decommit(const Gatherer &g, const MerkleDecommitPlan &plan, std::vector<uint32_t> &witness,
+                            std::vector<uint32_t> *aux);
+
+// ---- FRI prover state kept for decommitment -------------------------------
+
+struct FriLayerShape {
+  uint32_t log_size;   // log size of the layer's evaluation
+  uint32_t fold_step;  // folds from this layer to the next
+};
+std::vector<FriLayerShape> fri_schedule(const stwo_hip_fri_config &cfg, uint32_t log_size);
+
+struct FriLayerKeep {
+  FriLayerShape shape;
+  const uint32_t *cols[4];  // coordinate columns of the committed evaluation
+  uint32_t *tree;           // packed lifted Merkle tree of the 4 columns
+  uint32_t root[8];
+};
+
+// One committed layer's decommitment (FriLayerProof + FriLayerProofAux).
+struct FriLayerDecommit {
+  std::vector<uint32_t> positions;     // decommitment positions (sorted)
+  std::vector<uint32_t> values;        // 4 words per position
+  std::vector<uint32_t> fri_witness;   // 4 words per witness value
+  std::vector<uint32_t> hash_witness;  // 8 words per hash
+};
+
+}  // namespace stwo_hip
+
+struct stwo_hip_fri_prover {
+  int device = 0;
+  stwo_hip_fri_config cfg{};
+  uint32_t log_size = 0;
+  uint32_t *mem = nullptr;  // trees and folded evaluations, one allocation
+  size_t mem_words = 0;
+  std::vector<stwo_hip::FriLayerKeep> layers;  // committed layers, first (circle) layer first
+  std::vector<uint32_t *> evals;               // output of fold f (4 columns of 2^log each)
+  std::vector<uint32_t> last_layer_poly;       // 4 words per coefficient
+  uint32_t last_log = 0;
+  uint32_t *last_layer_eval[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool decommitted = false;
+  std::vector<uint32_t> query_positions;  // sorted, deduplicated
+  std::vector<uint32_t> unsorted_queries;
+  std::vector<stwo_hip::FriLayerDecommit> proofs;
+};
+
+namespace stwo_hip {
 
 // Merkle column reference (pointer + lifting shift).
 struct ColRef {

@@ -130,6 +130,24 @@ def lib():
             "stwo_hip_prof_reset": [_vp],
             "stwo_hip_prof_read": [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64),
                                    ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_merkle_lifted_decommit": [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t,
+                                                _u32p, ctypes.c_size_t, _u32p, _u32p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t), _u32p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_fri_prover_commit": [_vp, ctypes.c_char_p, _u32p, FriConfig, ctypes.POINTER(_vp),
+                                           ctypes.c_uint32, _vp, _u32p, _u32p, _u32p, ctypes.POINTER(_vp)],
+            "stwo_hip_fri_prover_decommit": [_vp, _vp, ctypes.c_char_p, _u32p, _u32p, _u32p, _u32p],
+            "stwo_hip_fri_prover_layer_proof": [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t), _u32p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_fri_prover_destroy": [_vp, _vp],
+            "stwo_hip_coeffs_simd_transpose": [_vp, _vp, ctypes.c_uint32],
+            "stwo_hip_twiddles_simd_doubled": [_vp, _vp, _vp, _vp],
+            "stwo_hip_canonicalize": [_vp, _vp, ctypes.c_size_t],
+            "stwo_hip_grind": [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
+            "stwo_hip_debug_fft_plan": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.c_uint32,
+                                        _u32p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -350,6 +368,66 @@ class Context:
         n = int(nr[0])
         return roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), d.raw[:32], int(nd[0])
 
+    def fri_prover_commit(self, digest: bytes, n_draws: int, config: FriConfig, column4, log_size, tw):
+        """FriProver::commit keeping the layers; returns (FriProver, roots,
+        last_layer_poly, digest, n_draws).  column4 must outlive the prover."""
+        d = ctypes.create_string_buffer(bytes(digest), 32)
+        nd = np.array([n_draws], dtype=np.uint32)
+        roots = np.zeros(8 * (log_size + 1), dtype=np.uint32)
+        nr = np.zeros(1, dtype=np.uint32)
+        bound = 1 << config.log_last_layer_degree_bound
+        last = np.zeros(4 * bound, dtype=np.uint32)
+        h = _vp()
+        self._check(lib().stwo_hip_fri_prover_commit(self.h, d, nd.ctypes.data_as(_u32p), config, _ptrs(column4),
+                                                     log_size, tw.h, roots.ctypes.data_as(_u32p),
+                                                     nr.ctypes.data_as(_u32p), last.ctypes.data_as(_u32p),
+                                                     ctypes.byref(h)))
+        n = int(nr[0])
+        pr = FriProver(self, h.value, config, n, column4)
+        return pr, roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), d.raw[:32], int(nd[0])
+
+    # -- SimdBackend layout interop ---------------------------------------------
+    def coeffs_simd_transpose(self, col, log_size):
+        self._check(lib().stwo_hip_coeffs_simd_transpose(self.h, col.ptr if isinstance(col, DeviceBuf) else col,
+                                                         log_size))
+
+    def twiddles_simd_doubled(self, tw, out_tw, out_itw):
+        self._check(lib().stwo_hip_twiddles_simd_doubled(self.h, tw.h, out_tw.ptr, out_itw.ptr))
+
+    def canonicalize(self, col, n):
+        self._check(lib().stwo_hip_canonicalize(self.h, col.ptr if isinstance(col, DeviceBuf) else col, n))
+
+    def grind(self, digest: bytes, pow_bits):
+        """GrindOps::grind (SimdBackend rule) for the Blake2s channel state `digest`."""
+        n = ctypes.c_uint64()
+        self._check(lib().stwo_hip_grind(self.h, bytes(digest), pow_bits, ctypes.byref(n)))
+        return n.value
+
+    def merkle_lifted_decommit(self, layers, lifting_log, cols, log_sizes, queries, with_aux=False):
+        """MerkleProverLifted::decommit on a packed lifted tree; returns
+        (values [n_cols][n_queries], hash_witness [k][8], aux [(layer, index, hash[8])] or None)."""
+        q = np.ascontiguousarray(queries, dtype=np.uint32)
+        ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
+        nq, nc = len(q), len(cols)
+        values = np.zeros(max(1, nc * nq), dtype=np.uint32)
+        cap = max(1, nq * (lifting_log + 1))
+        wit = np.zeros(8 * cap, dtype=np.uint32)
+        nw = ctypes.c_size_t()
+        aux = np.zeros(10 * 2 * cap, dtype=np.uint32) if with_aux else None
+        na = ctypes.c_size_t()
+        self._check(lib().stwo_hip_merkle_lifted_decommit(
+            self.h, layers.ptr if isinstance(layers, DeviceBuf) else layers, lifting_log, _ptrs(cols),
+            ls.ctypes.data_as(_u32p), nc, q.ctypes.data_as(_u32p), nq, values.ctypes.data_as(_u32p),
+            wit.ctypes.data_as(_u32p), cap, ctypes.byref(nw),
+            aux.ctypes.data_as(_u32p) if with_aux else None, 2 * cap, ctypes.byref(na)))
+        vals = values[: nc * nq].reshape(nc, nq) if nc else np.zeros((0, nq), dtype=np.uint32)
+        w = wit[: 8 * nw.value].reshape(-1, 8)
+        a = None
+        if with_aux:
+            r = aux[: 10 * na.value].reshape(-1, 10)
+            a = [(int(x[0]), int(x[1]), tuple(int(v) for v in x[2:])) for x in r]
+        return vals, w, a
+
     # -- batched commitment -------------------------------------------------------
     def batch_create(self, cols, log_sizes, log_blowup, tw, outs, layers, tree_kind=TREE_MIXED,
                      mode=HASH_BLAKE2S, use_graph=True):
@@ -377,6 +455,71 @@ class Context:
         self._check(lib().stwo_hip_prof_read(self.h, names, len(names), counts, ms, 64, ctypes.byref(n)))
         keys = names.raw.split(b"\0")[: n.value]
         return {k.decode(): (int(counts[i]), float(ms[i])) for i, k in enumerate(keys)}
+
+
+class FriProver:
+    """A committed FRI prover (stwo_hip_fri_prover) kept for decommitment."""
+
+    def __init__(self, ctx, h, config, n_layers, column4):
+        self.ctx, self.h, self.config, self.n_layers = ctx, h, config, n_layers
+        self._column = column4  # borrowed by the prover
+
+    def decommit(self, digest: bytes, n_draws: int):
+        """FriProver::decommit; returns (sorted query positions, raw queries, digest, n_draws)."""
+        d = ctypes.create_string_buffer(bytes(digest), 32)
+        nd = np.array([n_draws], dtype=np.uint32)
+        nq = self.config.n_queries
+        pos = np.zeros(nq, dtype=np.uint32)
+        npos = np.zeros(1, dtype=np.uint32)
+        raw = np.zeros(nq, dtype=np.uint32)
+        self.ctx._check(lib().stwo_hip_fri_prover_decommit(self.ctx.h, self.h, d, nd.ctypes.data_as(_u32p),
+                                                           pos.ctypes.data_as(_u32p), npos.ctypes.data_as(_u32p),
+                                                           raw.ctypes.data_as(_u32p)))
+        return pos[: int(npos[0])], raw, d.raw[:32], int(nd[0])
+
+    def layer_proof(self, layer):
+        """FriLayerProof of committed layer `layer` as a dict: commitment [8],
+        fri_witness [k][4], hash_witness [k][8], positions, values [k][4]."""
+        com = np.zeros(8, dtype=np.uint32)
+        nf, nh, npos = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        # A first call with no capacity reports the counts.
+        lib().stwo_hip_fri_prover_layer_proof(self.ctx.h, self.h, layer, com.ctypes.data_as(_u32p), None, 0,
+                                              ctypes.byref(nf), None, 0, ctypes.byref(nh), None, None, 0,
+                                              ctypes.byref(npos))
+        fw = np.zeros(4 * nf.value + 1, dtype=np.uint32)
+        hw = np.zeros(8 * nh.value + 1, dtype=np.uint32)
+        ps = np.zeros(npos.value + 1, dtype=np.uint32)
+        vs = np.zeros(4 * npos.value + 1, dtype=np.uint32)
+        self.ctx._check(lib().stwo_hip_fri_prover_layer_proof(
+            self.ctx.h, self.h, layer, com.ctypes.data_as(_u32p), fw.ctypes.data_as(_u32p), nf.value,
+            ctypes.byref(nf), hw.ctypes.data_as(_u32p), nh.value, ctypes.byref(nh), ps.ctypes.data_as(_u32p),
+            vs.ctypes.data_as(_u32p), npos.value, ctypes.byref(npos)))
+        return dict(commitment=com, fri_witness=fw[: 4 * nf.value].reshape(-1, 4),
+                    hash_witness=hw[: 8 * nh.value].reshape(-1, 8), positions=ps[: npos.value],
+                    values=vs[: 4 * npos.value].reshape(-1, 4))
+
+    def destroy(self):
+        if self.h:
+            self.ctx._check(lib().stwo_hip_fri_prover_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
+def fft_plan(n, N, inverse):
+    """The FFT passes the library runs for a transform of 2^n values (RFFT:
+    coefficients) on a domain of log size N: one dict per pass, in launch
+    order (stwo_hip_debug_fft_plan; launches nothing)."""
+    buf = (ctypes.c_uint32 * (16 * 8))()
+    k = ctypes.c_uint32()
+    rc = lib().stwo_hip_debug_fft_plan(n, N, int(inverse), buf, 8, ctypes.byref(k))
+    if rc != 0:
+        raise StwoHipError(rc, "fft plan (n=%d, N=%d) rejected" % (n, N))
+    out = []
+    for i in range(k.value):
+        w = list(buf[16 * i: 16 * i + 16])
+        stages = [(v >> 16, (v >> 8) & 0xFF, v & 0xFF) for v in w[9: 9 + w[8]]]
+        out.append(dict(T=w[0], lo=w[1], w=w[2], m=w[3], rep=w[4], wg=w[5], in_log=w[6], scaled=w[7],
+                        stages=stages, wgs=w[13]))
+    return out
 
 
 def layer_offset(l):
