@@ -265,6 +265,31 @@ int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover
                                     uint32_t *values, size_t pos_cap, size_t *n_pos);
 int stwo_hip_fri_prover_destroy(stwo_hip_ctx *ctx, stwo_hip_fri_prover *prover);
 
+/* ---- FRI quotients ------------------------------------------------------------
+ * compute_fri_quotients (STWO/prover/pcs/quotient_ops.rs:131-232) with the
+ * QuotientOps/AccumulationOps semantics of the reference backends
+ * (CPUB/quotients.rs, CPUB/accumulation.rs; Metal MB/quotient.rs:388,
+ * SH/quotients.metal).  Columns are the flattened trees' evaluations
+ * (bit-reversed, CanonicCoset(log_sizes[i]), 1 <= log <= lifting_log_size);
+ * `samples` holds, column after column, n_samples[i] PointSamples of column
+ * i (a column with exactly two gets the periodicity sample, as
+ * build_samples_with_randomness_and_periodicity does,
+ * STWO/core/pcs/quotients.rs:289-336); random powers alpha^0, alpha^1, ... run
+ * over columns in order.  `out` receives the 4 coordinate columns of the
+ * quotient evaluation on CanonicCoset(lifting_log_size), bit-reversed.  One
+ * kernel; the sample tables go through the context's upload ring. */
+typedef struct {
+  uint32_t x[4];
+  uint32_t y[4];
+} stwo_hip_secure_point; /* CirclePoint<SecureField> */
+typedef struct {
+  stwo_hip_secure_point point;
+  uint32_t value[4];
+} stwo_hip_point_sample; /* PointSample, STWO/core/pcs/quotients.rs */
+int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, const uint32_t *log_sizes,
+                                   size_t n_cols, const stwo_hip_point_sample *samples, const uint32_t *n_samples,
+                                   const uint32_t random_coeff[4], uint32_t lifting_log_size, uint32_t *const out[4]);
+
 /* ---- proof of work -----------------------------------------------------------
  * GrindOps::grind for the Blake2s channel as SimdBackend computes it
  * (STWO/prover/backend/simd/grind.rs:22-62; Metal MB/blake2s.rs:399-443,

@@ -65,6 +65,8 @@ def lib():
                                               _u32p, ctypes.c_size_t, ctypes.POINTER(_u32p)]
         L.or_blake2s.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p]
         L.or_num_threads.restype = ctypes.c_int
+        L.or_compute_fri_quotients.argtypes = [ctypes.POINTER(_u32p), _u32p, ctypes.c_size_t, ctypes.c_void_p, _u32p,
+                                               _u32p, ctypes.c_uint32, ctypes.POINTER(_u32p)]
         L.or_pow_prefixed_digest.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p]
         L.or_grind.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
@@ -300,6 +302,32 @@ def fri_commit(channel, cfg, col4, log_size, keep_layers=False):
     if n < 0:
         raise ValueError("fri_commit failed: %d" % n)
     return roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), layer_bufs
+
+
+# --- FRI quotients -------------------------------------------------------------
+
+def pack_samples(samples_per_col):
+    """[(x[4], y[4], value[4]), ...] per column -> (flat u32 array of 12 words per
+    sample, counts) in the C layout {point.x, point.y, value}."""
+    flat, counts = [], []
+    for ss in samples_per_col:
+        counts.append(len(ss))
+        for x, y, v in ss:
+            flat.extend(list(x) + list(y) + list(v))
+    return np.array(flat or [0], dtype=np.uint32), np.array(counts or [0], dtype=np.uint32)
+
+
+def compute_fri_quotients(cols, samples_per_col, random_coeff, lifting_log):
+    """compute_fri_quotients over the flattened columns; returns 4 coordinate columns of 2^lifting_log."""
+    logs = np.array([int(len(c)).bit_length() - 1 for c in cols], dtype=np.uint32)
+    flat, counts = pack_samples(samples_per_col)
+    rc = np.array(random_coeff, dtype=np.uint32)
+    out = [np.zeros(1 << lifting_log, dtype=np.uint32) for _ in range(4)]
+    r = lib().or_compute_fri_quotients(_ptr_array(cols), _p(logs), len(cols), flat.ctypes.data, _p(counts), _p(rc),
+                                       lifting_log, _ptr_array(out))
+    if r != 0:
+        raise ValueError("compute_fri_quotients rejected its input")
+    return out
 
 
 # --- proof of work -------------------------------------------------------------

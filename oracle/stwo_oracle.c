@@ -744,6 +744,159 @@ int or_fri_commit(or_channel *ch, const or_fri_config *cfg, const uint32_t *cons
 }
 
 /* ------------------------------------------------------------------------ */
+/* FRI quotients: compute_fri_quotients (STWO/prover/pcs/quotient_ops.rs:131-232) */
+/* with CpuBackend's QuotientOps (CPUB/quotients.rs) and AccumulationOps::       */
+/* lift_and_accumulate (CPUB/accumulation.rs), helpers from                      */
+/* STWO/core/pcs/quotients.rs and STWO/core/constraints.rs:119-134.              */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint32_t x[4], y[4]; } or_spoint;             /* CirclePoint<SecureField> */
+typedef struct { or_spoint point; uint32_t value[4]; } or_sample; /* PointSample */
+
+static qm31 qm31_conj(qm31 v) { qm31 r = {v.a, {m31_neg(v.b.a), m31_neg(v.b.b)}}; return r; }
+static int qm31_eq(qm31 x, qm31 y) { return x.a.a == y.a.a && x.a.b == y.a.b && x.b.a == y.b.a && x.b.b == y.b.b; }
+
+typedef struct { qm31 px, py; qm31 value; qm31 rand; int col; } q_entry;  /* (sample, alpha^k, column) */
+typedef struct { qm31 px, py; size_t n; q_entry *e; } q_batch;
+typedef struct { qm31 px, py; uint32_t log; qm31 *partial; qm31 first; } q_acc;
+
+int or_compute_fri_quotients(const uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                             const or_sample *samples, const uint32_t *n_samples, const uint32_t random_coeff[4],
+                             uint32_t lifting_log, uint32_t *const out[4]) {
+    if (lifting_log < 1 || lifting_log > 30) return -1;
+    for (size_t c = 0; c < n_cols; c++) if (log_sizes[c] < 1 || log_sizes[c] > lifting_log || n_samples[c] > 64) return -1;
+    /* 1. samples with randomness and periodicity (build_samples_with_randomness_and_periodicity,
+     *    quotient_ops.rs:80-127): alpha^0, alpha^1, ... over columns in order; a column with
+     *    exactly two samples first gets (second point + period generator, second value). */
+    qm31 alpha = qm31_from4(random_coeff), pw = {{1, 0}, {0, 0}};
+    size_t total = 0;
+    for (size_t c = 0; c < n_cols; c++) total += n_samples[c] + (n_samples[c] == 2);
+    q_entry *all = (q_entry *)calloc(total ? total : 1, sizeof(q_entry));
+    size_t k = 0, si = 0;
+    for (size_t c = 0; c < n_cols; c++) {
+        const or_sample *sm = samples + si;
+        if (n_samples[c] == 2) {
+            /* CanonicCoset(L).step() repeated_double(log_size): index 2^(31 - L + log) */
+            uint64_t idx = ((uint64_t)1 << (31 - lifting_log + log_sizes[c])) & IDX_MASK;
+            cpoint g = index_to_point((uint32_t)idx);
+            qm31 x1 = qm31_from4(sm[1].point.x), y1 = qm31_from4(sm[1].point.y);
+            q_entry e;
+            e.px = qm31_sub(qm31_mul_m31(x1, g.x), qm31_mul_m31(y1, g.y));
+            e.py = qm31_add(qm31_mul_m31(y1, g.x), qm31_mul_m31(x1, g.y));
+            e.value = qm31_from4(sm[1].value); e.rand = pw; e.col = (int)c;
+            all[k++] = e; pw = qm31_mul(pw, alpha);
+        }
+        for (uint32_t j = 0; j < n_samples[c]; j++) {
+            q_entry e;
+            e.px = qm31_from4(sm[j].point.x); e.py = qm31_from4(sm[j].point.y);
+            e.value = qm31_from4(sm[j].value); e.rand = pw; e.col = (int)c;
+            all[k++] = e; pw = qm31_mul(pw, alpha);
+        }
+        si += n_samples[c];
+    }
+    /* 2-4. per log size ascending: batches by point in first-appearance order
+     *      (ColumnSampleBatch::new_vec), line coefficients, partial numerators
+     *      (accumulate_numerators, CPUB/quotients.rs:22-46). */
+    q_acc *accs = (q_acc *)calloc(total ? total : 1, sizeof(q_acc));
+    size_t n_accs = 0;
+    for (uint32_t lg = 1; lg <= lifting_log; lg++) {
+        q_batch *batches = (q_batch *)calloc(total ? total : 1, sizeof(q_batch));
+        size_t nb = 0;
+        for (size_t i = 0; i < total; i++) {
+            if (log_sizes[all[i].col] != lg) continue;
+            size_t b = 0;
+            while (b < nb && !(qm31_eq(batches[b].px, all[i].px) && qm31_eq(batches[b].py, all[i].py))) b++;
+            if (b == nb) {
+                batches[nb].px = all[i].px; batches[nb].py = all[i].py; batches[nb].n = 0;
+                batches[nb].e = (q_entry *)calloc(total, sizeof(q_entry)); nb++;
+            }
+            batches[b].e[batches[b].n++] = all[i];
+        }
+        size_t n = (size_t)1 << lg;
+        for (size_t b = 0; b < nb; b++) {
+            q_acc *A = &accs[n_accs++];
+            A->px = batches[b].px; A->py = batches[b].py; A->log = lg;
+            A->partial = (qm31 *)calloc(n, sizeof(qm31));
+            qm31 first = {{0, 0}, {0, 0}};
+            qm31 *la = (qm31 *)malloc(batches[b].n * sizeof(qm31)), *lb = (qm31 *)malloc(batches[b].n * sizeof(qm31)),
+                 *lc = (qm31 *)malloc(batches[b].n * sizeof(qm31));
+            for (size_t j = 0; j < batches[b].n; j++) {
+                const q_entry *e = &batches[b].e[j];
+                /* complex_conjugate_line_coeffs (constraints.rs:119-134) */
+                qm31 a = qm31_sub(qm31_conj(e->value), e->value);
+                qm31 cc = qm31_sub(qm31_conj(e->py), e->py);
+                qm31 bb = qm31_sub(qm31_mul(e->value, cc), qm31_mul(a, e->py));
+                la[j] = qm31_mul(e->rand, a); lb[j] = qm31_mul(e->rand, bb); lc[j] = qm31_mul(e->rand, cc);
+                first = qm31_add(first, la[j]);
+            }
+            A->first = first;
+            #pragma omp parallel for schedule(static)
+            for (size_t r = 0; r < n; r++) {
+                qm31 acc = {{0, 0}, {0, 0}};
+                for (size_t j = 0; j < batches[b].n; j++) {
+                    uint32_t v = cols[batches[b].e[j].col][r];
+                    acc = qm31_add(acc, qm31_sub(qm31_mul_m31(lc[j], v), lb[j]));
+                }
+                A->partial[r] = acc;
+            }
+            free(la); free(lb); free(lc);
+        }
+        for (size_t b = 0; b < nb; b++) free(batches[b].e);
+        free(batches);
+    }
+    /* 5. per sample point: sum the a-terms, lift_and_accumulate the partials (ascending sizes). */
+    int *used = (int *)calloc(n_accs ? n_accs : 1, sizeof(int));
+    q_acc *pts = (q_acc *)calloc(n_accs ? n_accs : 1, sizeof(q_acc));
+    size_t n_pts = 0;
+    for (size_t i = 0; i < n_accs; i++) {
+        if (used[i]) continue;
+        uint32_t clog = 1;
+        qm31 *cur = (qm31 *)calloc(2, sizeof(qm31));
+        qm31 first = {{0, 0}, {0, 0}};
+        for (size_t j = i; j < n_accs; j++) {
+            if (used[j] || !qm31_eq(accs[j].px, accs[i].px) || !qm31_eq(accs[j].py, accs[i].py)) continue;
+            used[j] = 1;
+            first = qm31_add(first, accs[j].first);
+            uint32_t lr = accs[j].log - clog;
+            size_t n = (size_t)1 << accs[j].log;
+            qm31 *nx = (qm31 *)malloc(n * sizeof(qm31));
+            for (size_t r = 0; r < n; r++) nx[r] = qm31_add(accs[j].partial[r], cur[((r >> (lr + 1)) << 1) + (r & 1)]);
+            free(cur); cur = nx; clog = accs[j].log;
+        }
+        pts[n_pts].px = accs[i].px; pts[n_pts].py = accs[i].py; pts[n_pts].log = clog;
+        pts[n_pts].partial = cur; pts[n_pts].first = first; n_pts++;
+    }
+    /* 6. compute_quotients_and_combine (CPUB/quotients.rs:48-76) with denominator_inverses
+     *    (core/pcs/quotients.rs:262-287). */
+    size_t N = (size_t)1 << lifting_log;
+    uint32_t half_init = half_odds_initial(lifting_log - 1);
+    uint32_t half_step = (uint32_t)(((uint64_t)1 << (31 - (lifting_log - 1))) & IDX_MASK);
+    #pragma omp parallel for schedule(static)
+    for (size_t row = 0; row < N; row++) {
+        size_t i = bitrev(row, lifting_log);
+        size_t h = N / 2;
+        uint32_t idx = (uint32_t)((half_init + (uint64_t)half_step * (i < h ? i : i - h)) & IDX_MASK);
+        if (i >= h) idx = (uint32_t)((((uint64_t)1 << 31) - idx) & IDX_MASK);
+        cpoint dp = index_to_point(idx);
+        qm31 q = {{0, 0}, {0, 0}};
+        for (size_t s = 0; s < n_pts; s++) {
+            cm31 prx = pts[s].px.a, pix = pts[s].px.b, pry = pts[s].py.a, piy = pts[s].py.b;
+            cm31 dx = {m31_sub(prx.a, dp.x), prx.b}, dy = {m31_sub(pry.a, dp.y), pry.b};
+            cm31 den = cm31_sub(cm31_mul(dx, piy), cm31_mul(dy, pix));
+            cm31 dinv = cm31_inv(den);
+            uint32_t lr = lifting_log - pts[s].log;
+            qm31 full = qm31_sub(pts[s].partial[((row >> (lr + 1)) << 1) + (row & 1)], qm31_mul_m31(pts[s].first, dp.y));
+            qm31 t = {cm31_mul(full.a, dinv), cm31_mul(full.b, dinv)};
+            q = qm31_add(q, t);
+        }
+        out[0][row] = q.a.a; out[1][row] = q.a.b; out[2][row] = q.b.a; out[3][row] = q.b.b;
+    }
+    for (size_t s = 0; s < n_pts; s++) free(pts[s].partial);
+    for (size_t i = 0; i < n_accs; i++) free(accs[i].partial);
+    free(pts); free(used); free(accs); free(all);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Proof of work (GrindOps for the Blake2s channel)                          */
 /* ------------------------------------------------------------------------ */
 #define OR_POW_PREFIX 0x12345678u

@@ -140,7 +140,7 @@ __host__ __device__ __forceinline__ uint32_t bitrev32(uint32_t i, uint32_t log_s
 }
 // Coset::half_odds(n).initial_index = 2^(29 - n) (STWO/core/circle.rs:328-330);
 // CanonicCoset(n).circle_domain() = CircleDomain(half_odds(n - 1)).
-inline uint32_t half_odds_initial(uint32_t n) { return 1u << (29 - n); }
+__host__ __device__ __forceinline__ uint32_t half_odds_initial(uint32_t n) { return 1u << (29 - n); }
 
 }  // namespace stwo_hip
 

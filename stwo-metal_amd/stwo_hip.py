@@ -145,6 +145,8 @@ def lib():
             "stwo_hip_coeffs_simd_transpose": [_vp, _vp, ctypes.c_uint32],
             "stwo_hip_twiddles_simd_doubled": [_vp, _vp, _vp, _vp],
             "stwo_hip_canonicalize": [_vp, _vp, ctypes.c_size_t],
+            "stwo_hip_compute_fri_quotients": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_void_p,
+                                               _u32p, _u32p, ctypes.c_uint32, ctypes.POINTER(_vp)],
             "stwo_hip_grind": [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
             "stwo_hip_debug_fft_plan": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.c_uint32,
                                         _u32p],
@@ -396,6 +398,22 @@ class Context:
 
     def canonicalize(self, col, n):
         self._check(lib().stwo_hip_canonicalize(self.h, col.ptr if isinstance(col, DeviceBuf) else col, n))
+
+    def compute_fri_quotients(self, cols, log_sizes, samples_per_col, random_coeff, lifting_log, out4):
+        """compute_fri_quotients; samples_per_col: per column a list of
+        (x[4], y[4], value[4]) PointSamples; out4: 4 device columns of 2^lifting_log."""
+        flat, counts = [], []
+        for ss in samples_per_col:
+            counts.append(len(ss))
+            for x, y, v in ss:
+                flat.extend(list(x) + list(y) + list(v))
+        fl = np.array(flat or [0], dtype=np.uint32)
+        cn = np.array(counts or [0], dtype=np.uint32)
+        ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
+        rc = np.ascontiguousarray(random_coeff, dtype=np.uint32)
+        self._check(lib().stwo_hip_compute_fri_quotients(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols),
+                                                         fl.ctypes.data, cn.ctypes.data_as(_u32p),
+                                                         rc.ctypes.data_as(_u32p), lifting_log, _ptrs(out4)))
 
     def grind(self, digest: bytes, pow_bits):
         """GrindOps::grind (SimdBackend rule) for the Blake2s channel state `digest`."""

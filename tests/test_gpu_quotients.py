@@ -1,0 +1,102 @@
+"""GPU parity of stwo_hip_compute_fri_quotients against the oracle's
+restatement of compute_fri_quotients (STWO/prover/pcs/quotient_ops.rs:131-232),
+bit-exact, plus the low-degree property the reference's
+test_quotients_are_low_degree checks, on the library's own output."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
+sys.path.insert(0, HERE)
+import oracle as O  # noqa: E402
+import stwo_hip as sh  # noqa: E402
+from test_quotients_oracle import ALPHA, domain_eval, eval_at, interp, lifted_poly, secure_point  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+U = np.uint32
+P = O.P
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sh.Context(0)
+    yield c
+    c.close()
+
+
+def run(ctx, cols, samples, L):
+    d = [ctx.to_device(c) for c in cols]
+    out = [ctx.alloc(1 << L) for _ in range(4)]
+    ctx.compute_fri_quotients(d, [len(c).bit_length() - 1 for c in cols], samples, ALPHA, L, out)
+    res = [o.download() for o in out]
+    for x in d + out:
+        x.free()
+    return res
+
+
+def test_quotients_reference_shape(ctx):
+    L = 7
+    coeffs = np.arange(8, dtype=U)
+    ev = domain_eval(coeffs, L)
+    pts = [secure_point(k) for k in (0x1234567, 0xABCDEF01)]
+    samples = [[(p[0], p[1], eval_at(coeffs, p)) for p in pts]]
+    got = run(ctx, [ev], samples, L)
+    want = O.compute_fri_quotients([ev], samples, ALPHA, L)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+    for g in got:
+        assert not interp(g, L)[(1 << 3) - 1:].any()
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_quotients_lifted_mixed(ctx, seed):
+    rng = np.random.default_rng(seed)
+    L, b = 8, 2
+    logs = [8, 5, 6, 8, 4, 7, 1]
+    cols, samples = [], []
+    z = [secure_point(int(k)) for k in rng.integers(1, 1 << 62, 3)]
+    for i, lg in enumerate(logs):
+        if lg > b:
+            c = rng.integers(0, P, 1 << (lg - b), dtype=np.uint64).astype(U)
+            ev = domain_eval(c, lg)
+        else:
+            ev = rng.integers(0, P, 1 << lg, dtype=np.uint64).astype(U)
+        cols.append(ev)
+        lp = lifted_poly(ev, L)
+        n = i % 4  # 0..3 samples, 2 gets the periodicity sample
+        samples.append([(p[0], p[1], eval_at(lp, p)) for p in z[:n]])
+    got = run(ctx, cols, samples, L)
+    want = O.compute_fri_quotients(cols, samples, ALPHA, L)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_quotients_wide_vs_oracle(ctx):
+    """96 columns of log 14..16 lifted to 16, two samples each (an OODS point
+    and its predecessor, as a trace with a mask of two rows has)."""
+    rng = np.random.default_rng(11)
+    L = 16
+    logs = [14 + (i % 3) for i in range(96)]
+    cols = [rng.integers(0, P, 1 << lg, dtype=np.uint64).astype(U) for lg in logs]
+    z1, z2 = secure_point(0x55AA55AA), secure_point(0x1234)
+    samples = []
+    for i in range(96):
+        v1 = tuple(int(x) for x in rng.integers(0, P, 4))
+        v2 = tuple(int(x) for x in rng.integers(0, P, 4))
+        samples.append([(z1[0], z1[1], v1), (z2[0], z2[1], v2)] if i % 5 else [(z1[0], z1[1], v1)])
+    got = run(ctx, cols, samples, L)
+    want = O.compute_fri_quotients(cols, samples, ALPHA, L)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_quotients_reject_bad_columns(ctx):
+    d = ctx.to_device(np.zeros(16, U))
+    out = [ctx.alloc(1 << 3) for _ in range(4)]
+    with pytest.raises(sh.StwoHipError):
+        ctx.compute_fri_quotients([d], [4], [[]], ALPHA, 3, out)  # column longer than the lifting size
+    for x in [d] + out:
+        x.free()
