@@ -290,6 +290,15 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
                                    size_t n_cols, const stwo_hip_point_sample *samples, const uint32_t *n_samples,
                                    const uint32_t random_coeff[4], uint32_t lifting_log_size, uint32_t *const out[4]);
 
+/* PolyOps::eval_at_point for a batch of polynomials at one secure point
+ * (STWO/prover/poly/circle/ops.rs; CPUB/circle.rs:82-99; Metal
+ * batch_eval_at_point, SH/eval_at_point.metal): the out-of-domain samples.
+ * coeffs[i] holds 2^log_sizes[i] coefficients in natural FFT-basis order
+ * (as interpolate returns them); values receives 4 words per polynomial
+ * (host memory).  Synchronizes. */
+int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
+                           const stwo_hip_secure_point *point, uint32_t *values);
+
 /* ---- proof of work -----------------------------------------------------------
  * GrindOps::grind for the Blake2s channel as SimdBackend computes it
  * (STWO/prover/backend/simd/grind.rs:22-62; Metal MB/blake2s.rs:399-443,

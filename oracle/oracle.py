@@ -67,6 +67,7 @@ def lib():
         L.or_num_threads.restype = ctypes.c_int
         L.or_compute_fri_quotients.argtypes = [ctypes.POINTER(_u32p), _u32p, ctypes.c_size_t, ctypes.c_void_p, _u32p,
                                                _u32p, ctypes.c_uint32, ctypes.POINTER(_u32p)]
+        L.or_eval_at_point.argtypes = [_u32p, ctypes.c_uint32, _u32p, _u32p, _u32p]
         L.or_pow_prefixed_digest.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p]
         L.or_grind.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
@@ -302,6 +303,16 @@ def fri_commit(channel, cfg, col4, log_size, keep_layers=False):
     if n < 0:
         raise ValueError("fri_commit failed: %d" % n)
     return roots[: 8 * n].reshape(n, 8), last.reshape(bound, 4), layer_bufs
+
+
+def eval_at_point(coeffs, point):
+    """CpuBackend::eval_at_point at a secure point ((x[4], y[4]))."""
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32)
+    x = np.array(point[0], dtype=np.uint32)
+    y = np.array(point[1], dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().or_eval_at_point(_p(c), int(len(c)).bit_length() - 1, _p(x), _p(y), _p(out))
+    return tuple(int(v) for v in out)
 
 
 # --- FRI quotients -------------------------------------------------------------

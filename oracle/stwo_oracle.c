@@ -743,6 +743,29 @@ int or_fri_commit(or_channel *ch, const or_fri_config *cfg, const uint32_t *cons
     return n_roots;
 }
 
+/* CpuBackend::eval_at_point at a secure point (CPUB/circle.rs:82-99): fold
+ * (STWO/core/poly/utils.rs:27-38) with factors [pi^(n-2)(x), ..., x, y],
+ * evaluated bottom-up: adjacent pairs with y, then x, then pi(x), ... */
+void or_eval_at_point(const uint32_t *coeffs, uint32_t log_size, const uint32_t px[4], const uint32_t py[4],
+                      uint32_t out[4]) {
+    size_t n = (size_t)1 << log_size;
+    qm31 *lv = (qm31 *)malloc(n * sizeof(qm31));
+    for (size_t j = 0; j < n; j++) { qm31 v = {{coeffs[j], 0}, {0, 0}}; lv[j] = v; }
+    qm31 x = qm31_from4(px), g = qm31_from4(py);
+    for (uint32_t k = 0; k < log_size; k++) {
+        if (k >= 1) {
+            g = x;
+            qm31 sq = qm31_mul(x, x);
+            qm31 one = {{1, 0}, {0, 0}};
+            x = qm31_sub(qm31_add(sq, sq), one);
+        }
+        size_t m = n >> (k + 1);
+        for (size_t t = 0; t < m; t++) lv[t] = qm31_add(lv[2 * t], qm31_mul(lv[2 * t + 1], g));
+    }
+    qm31_to4(lv[0], out);
+    free(lv);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FRI quotients: compute_fri_quotients (STWO/prover/pcs/quotient_ops.rs:131-232) */
 /* with CpuBackend's QuotientOps (CPUB/quotients.rs) and AccumulationOps::       */

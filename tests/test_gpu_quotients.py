@@ -100,3 +100,19 @@ def test_quotients_reject_bad_columns(ctx):
         ctx.compute_fri_quotients([d], [4], [[]], ALPHA, 3, out)  # column longer than the lifting size
     for x in [d] + out:
         x.free()
+
+
+def test_eval_at_point_vs_oracle(ctx):
+    """stwo_hip_eval_at_point vs the oracle (pinned by the FFT on domain
+    points and the reference's 4-coefficient KAT), logs 0..22, one call."""
+    rng = np.random.default_rng(5)
+    logs = [0, 1, 2, 3, 10, 11, 12, 13, 17, 22, 11, 4]
+    cols = [rng.integers(0, P, 1 << lg, dtype=np.uint64).astype(U) for lg in logs]
+    d = [ctx.to_device(c) for c in cols]
+    for k in (0xDEADBEEF, 42):
+        pt = secure_point(k)
+        got = ctx.eval_at_point(d, logs, pt)
+        for c, g in zip(cols, got):
+            assert tuple(int(v) for v in g) == O.eval_at_point(c, pt)
+    for x in d:
+        x.free()

@@ -167,3 +167,44 @@ def test_periodicity_point_matters():
                                    ALPHA, L)
     for q in quot:
         assert not interp(q, L)[1 << (L - b):].any()
+
+
+# ---------------------------------------------------------- eval_at_point ---
+
+def test_eval_at_point_reference_kat():
+    """test_eval_at_point_with_4_coeffs (CPUB/circle.rs tests): coefficients
+    [1, 3, 2, 4] give c0 + c1 y + c2 x + c3 x y."""
+    x, y = O.index_to_point(123456)
+    got = O.eval_at_point(np.array([1, 3, 2, 4], U), ((x, 0, 0, 0), (y, 0, 0, 0)))
+    want = (1 + 3 * y + 2 * x + 4 * x * y) % P
+    assert got == (want, 0, 0, 0)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 5, 9])
+def test_eval_at_point_matches_fft_on_the_domain(n):
+    """At every point of CanonicCoset(n) the value equals the FFT evaluation
+    (which brute force pins) at that point's bit-reversed position."""
+    rng = np.random.default_rng(n)
+    coeffs = rng.integers(0, P, 1 << n, dtype=np.uint64).astype(U)
+    if n == 0:
+        assert O.eval_at_point(coeffs, ((5, 0, 0, 0), (7, 0, 0, 0))) == (int(coeffs[0]), 0, 0, 0)
+        return
+    ev = domain_eval(coeffs, n)
+    half_init = O.half_odds_initial(n - 1)
+    for j in range(1 << n):
+        i = int(format(j, "0%db" % n)[::-1], 2) if n else 0
+        h = 1 << (n - 1)
+        step = (1 << (31 - (n - 1))) % (1 << 31)
+        idx = (half_init + step * (i % h)) % (1 << 31)
+        if i >= h:
+            idx = (-idx) % (1 << 31)
+        x, y = O.index_to_point(idx)
+        assert O.eval_at_point(coeffs, ((x, 0, 0, 0), (y, 0, 0, 0))) == (int(ev[j]), 0, 0, 0), j
+
+
+def test_eval_at_point_secure_points_match_fold():
+    rng = np.random.default_rng(3)
+    for n in (1, 4, 7):
+        coeffs = rng.integers(0, P, 1 << n, dtype=np.uint64).astype(U)
+        pt = secure_point(int(rng.integers(1, 1 << 60)))
+        assert O.eval_at_point(coeffs, pt) == eval_at(coeffs, pt)
