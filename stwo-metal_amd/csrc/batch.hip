@@ -94,6 +94,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     uint32_t n;
     size_t ifft_off, rfft_off, inplace_off;
     uint32_t count;
+    bool aligned16;  // every column pointer of the group is 16-byte aligned
   };
   std::vector<GroupTabs> gt;
   for (auto &g : groups) {
@@ -104,6 +105,9 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
       c.push_back({outs[i], outs[i]});
     }
     GroupTabs t;
+    t.aligned16 = true;
+    for (size_t i : g.second)
+      if ((((uintptr_t)cols[i]) | ((uintptr_t)outs[i])) & 15) t.aligned16 = false;
     t.n = g.first;
     t.count = (uint32_t)g.second.size();
     t.ifft_off = st.put(a.data(), a.size() * sizeof(FftCol));
@@ -156,7 +160,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
       const FftCol *first = (const FftCol *)(dbase + (inverse ? t.ifft_off : t.rfft_off));
       const FftCol *inpl = (const FftCol *)(dbase + t.inplace_off);
       int rc = make_group_passes(ctx, inverse ? N : n, N, inverse, first, inpl, t.count, inverse ? itw_end : tw_end,
-                                 passes);
+                                 t.aligned16, passes);
       if (rc) {
         (void)hipFree(dmem);
         return rc;

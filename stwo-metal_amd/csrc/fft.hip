@@ -257,6 +257,12 @@ __device__ __forceinline__ uint32_t gld(const uint32_t *base, uint32_t byte_off)
 __device__ __forceinline__ void gst(uint32_t *base, uint32_t byte_off, uint32_t v) {
   *(g_u32 *)((g_u8 *)as_global(base) + byte_off) = v;
 }
+__device__ __forceinline__ u32x4 gld4(const uint32_t *base, uint32_t byte_off) {
+  return *(gconst_u4 *)((gconst_u8 *)as_global(base) + byte_off);
+}
+__device__ __forceinline__ void gst4(uint32_t *base, uint32_t byte_off, u32x4 v) {
+  *(g_u4 *)((g_u8 *)as_global(base) + byte_off) = v;
+}
 
 __device__ __forceinline__ void load_level(uint32_t *dst, int cnt, const uint32_t *tw_end, uint32_t N, uint32_t s,
                                            uint32_t gb) {
@@ -442,9 +448,25 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     auto group_base = [&](uint32_t b0) {
       return ((local >> b0) << (b0 + 4)) | (local & ((1u << b0) - 1));
     };
-    // --- load in the top layout
     uint32_t cur = top_b0;
-    {
+    if (p.vec_io & 1) {
+      // --- load straight into layout b0 = 0: 16 consecutive words per lane
+      // (in_log >= 4, so the input mask keeps them consecutive)
+      if (rep > 0) __syncthreads();
+      if (active) {
+        const uint32_t vo = (gidx(local << 4) & in_mask) << 2;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const u32x4 v = gld4(in, vo + 16 * k);
+          x[4 * k] = v.x;
+          x[4 * k + 1] = v.y;
+          x[4 * k + 2] = v.z;
+          x[4 * k + 3] = v.w;
+        }
+      }
+      cur = 0;
+    } else {
+      // --- load in the top layout
       // (gb | j << sh) & in_mask = (gb & in_mask) | ((j << sh) & in_mask):
       // a lane offset plus a wave-uniform one.
       const uint32_t gb = gidx(local), sh = top_b0 - p.w + p.lo;
@@ -475,6 +497,18 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
 #pragma unroll
       for (int j = 0; j < 16; j++) x[j] = m31_mul(x[j], p.scale);
     }
+    if ((p.vec_io & 2) && cur == 0) {
+      // --- store straight from layout b0 = 0
+      if (active) {
+        const uint32_t vo = gidx(local << 4) << 2;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const u32x4 v = {x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]};
+          gst4(out, vo + 16 * k, v);
+        }
+      }
+      continue;
+    }
     // --- store in a lane-consecutive layout
     if (cur != top_b0 && cur < 5) {
       __syncthreads();
@@ -493,7 +527,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
 }
 
 template <bool INV, int WG_LOG>
-__global__ void __launch_bounds__(1 << WG_LOG) fft_pass_kernel(FftPass2 p) {
+__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(5))) fft_pass_kernel(FftPass2 p) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
   fft_pass_body<INV, WG_LOG>(p, blockIdx.x, lds);
 }
@@ -502,7 +536,7 @@ __global__ void __launch_bounds__(1 << WG_LOG) fft_pass_kernel(FftPass2 p) {
 // owns workgroups [wg_start[g], wg_start[g + 1]).  Used by the batched path,
 // where every column size contributes its pass of the same level.
 template <bool INV, int WG_LOG>
-__global__ void __launch_bounds__(1 << WG_LOG) fft_pass_multi_kernel(const FftPass2 *groups,
+__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(5))) fft_pass_multi_kernel(const FftPass2 *groups,
                                                                     const uint32_t *wg_start, uint32_t n_groups) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
   uint32_t g = 0;
@@ -605,7 +639,7 @@ static int check_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_tw
 // first pass; later forward passes run in place on the outputs and use
 // `d_tab_inplace` (entries {out, out}).
 int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, const FftCol *d_tab,
-                      const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end,
+                      const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
                       std::vector<PassLaunch> &out) {
   out.clear();
   STWO_HIP_CHECK_ARG(ctx, N >= 4 && N <= 28 && n <= N, "internal: radix-16 path needs 4 <= N <= 28 (N = %u)", N);
@@ -646,6 +680,11 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
     STWO_HIP_CHECK_ARG(ctx, wgs * tiles_per_wg <= (1ull << 32), "FFT group of %u columns of log %u too large",
                        n_cols, N);
     p.cols_div = make_u32div(n_cols);
+    p.vec_io = 0;
+    if (aligned16 && p.w == 0 && p.st[0].r_hi > 0) {
+      if (p.st[0].b0 == 0 && p.in_log >= 4) p.vec_io |= 1;
+      if (p.st[p.n_stages - 1].b0 == 0) p.vec_io |= 2;
+    }
     out.push_back({p, (uint32_t)wgs, wg_log});
   }
   return STWO_HIP_OK;
@@ -683,9 +722,12 @@ static int run_fft_group2(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, vo
     int rc = upload_table(ctx, c2.data(), c2.size() * sizeof(FftCol), &dtab_out);
     if (rc) return rc;
   }
+  bool aligned16 = true;
+  for (const FftCol &c : cols)
+    if ((((uintptr_t)c.in) | ((uintptr_t)c.out)) & 15) aligned16 = false;
   std::vector<PassLaunch> passes;
   int rc = make_group_passes(ctx, n, N, inverse, (const FftCol *)dtab, (const FftCol *)dtab_out,
-                             (uint32_t)cols.size(), tw_end, passes);
+                             (uint32_t)cols.size(), tw_end, aligned16, passes);
   if (rc) return rc;
   for (const PassLaunch &pl : passes) {
     rc = launch_fft_pass(ctx, inverse, pl.p, pl.wgs, pl.wg_log);
@@ -884,7 +926,7 @@ int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, 
   if (!out || !n_passes || N < 4 || N > 28 || n > N) return STWO_HIP_EINVAL;
   std::vector<PassLaunch> passes;
   stwo_hip_ctx dummy;
-  int rc = make_group_passes(&dummy, inverse ? N : n, N, inverse != 0, nullptr, nullptr, 1, nullptr, passes);
+  int rc = make_group_passes(&dummy, inverse ? N : n, N, inverse != 0, nullptr, nullptr, 1, nullptr, true, passes);
   if (rc) return rc;
   if (passes.size() > max_passes) return STWO_HIP_EINVAL;
   for (size_t k = 0; k < passes.size(); k++) {
@@ -895,7 +937,8 @@ int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, 
     for (uint32_t i = 0; i < kMaxStages; i++)
       o[9 + i] = i < p.n_stages ? (p.st[i].b0 << 16) | (p.st[i].r_lo << 8) | p.st[i].r_hi : 0;
     o[13] = passes[k].wgs;
-    o[14] = o[15] = 0;
+    o[14] = p.vec_io;
+    o[15] = 0;
   }
   *n_passes = (uint32_t)passes.size();
   return STWO_HIP_OK;

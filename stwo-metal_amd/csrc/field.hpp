@@ -153,6 +153,9 @@ typedef __attribute__((address_space(1))) const uint32_t gconst_u32;
 typedef __attribute__((address_space(1))) uint32_t g_u32;
 typedef __attribute__((address_space(1))) const uint8_t gconst_u8;
 typedef __attribute__((address_space(1))) uint8_t g_u8;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 gconst_u4;
+typedef __attribute__((address_space(1))) u32x4 g_u4;
 __device__ __forceinline__ gconst_u32 *as_global(const uint32_t *p) { return (gconst_u32 *)p; }
 __device__ __forceinline__ g_u32 *as_global(uint32_t *p) { return (g_u32 *)p; }
 }  // namespace stwo_hip

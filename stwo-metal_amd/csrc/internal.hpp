@@ -62,6 +62,11 @@ struct FftPass2 {
   uint32_t rep_log;  // output blocks per tile (first RFFT pass of a blowup)
   // Division of a tile number by n_cols (tiles are numbered column-fastest).
   U32Div cols_div;
+  // Bit 0: the first stage's layout (b0 = 0, 16 consecutive words per lane)
+  // is loaded straight from HBM with 128-bit loads; bit 1: the last stage's
+  // layout (b0 = 0) is stored the same way.  Set only for contiguous tiles
+  // (w = 0) of 16-byte aligned columns.
+  uint32_t vec_io;
 };
 
 struct PassLaunch {
@@ -70,8 +75,9 @@ struct PassLaunch {
   uint32_t wg_log;  // log2 lanes per workgroup (8 or 9)
 };
 
+// `aligned16`: every column pointer of the group is 16-byte aligned.
 int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, const FftCol *d_tab,
-                      const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end,
+                      const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
                       std::vector<PassLaunch> &out);
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log);
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,

@@ -547,7 +547,7 @@ def fft_plan(n, N, inverse):
         w = list(buf[16 * i: 16 * i + 16])
         stages = [(v >> 16, (v >> 8) & 0xFF, v & 0xFF) for v in w[9: 9 + w[8]]]
         out.append(dict(T=w[0], lo=w[1], w=w[2], m=w[3], rep=w[4], wg=w[5], in_log=w[6], scaled=w[7],
-                        stages=stages, wgs=w[13]))
+                        stages=stages, wgs=w[13], vec_io=w[14]))
     return out
 
 

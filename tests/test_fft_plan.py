@@ -35,7 +35,7 @@ def plan(lib, n, N, inverse):
             v = w[9 + j]
             stages.append((v >> 16, (v >> 8) & 0xFF, v & 0xFF))
         out.append(dict(T=w[0], lo=w[1], w=w[2], m=w[3], rep=w[4], wg=w[5], in_log=w[6], scaled=w[7],
-                        stages=stages, wgs=w[13]))
+                        stages=stages, wgs=w[13], vec_io=w[14]))
     return out
 
 
@@ -67,3 +67,21 @@ def test_every_layer_once_in_order(lib, N):
             assert sorted(seen) == list(range(L)), (n, N, inverse, passes)
             expect = list(range(L)) if inverse else list(range(L - 1, -1, -1))
             assert seen == expect, (n, N, inverse, seen)
+
+
+@pytest.mark.parametrize("N", list(range(4, 29)))
+def test_vector_io_only_on_contiguous_layout_zero(lib, N):
+    """128-bit loads/stores are used only where a lane's 16 elements are 16
+    consecutive words: layout b0 = 0 of a contiguous tile (w = 0), and for
+    loads an input index mask that keeps the low 4 bits (in_log >= 4)."""
+    for inverse in (False, True):
+        for n in ([N] if inverse else range(0, N + 1)):
+            for p in plan(lib, n, N, inverse):
+                v = p["vec_io"]
+                if v & 1:
+                    assert p["w"] == 0 and p["lo"] == 0 and p["stages"][0][0] == 0 and p["in_log"] >= 4, (n, N, p)
+                if v & 2:
+                    assert p["w"] == 0 and p["lo"] == 0 and p["stages"][-1][0] == 0, (n, N, p)
+                if p["w"] == 0 and p["stages"][0][2] > 0:
+                    assert bool(v & 1) == (p["stages"][0][0] == 0 and p["in_log"] >= 4)
+                    assert bool(v & 2) == (p["stages"][-1][0] == 0)
