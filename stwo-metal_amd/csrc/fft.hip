@@ -409,18 +409,20 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
   const uint64_t gtile = ((uint64_t)wg << (WG_LOG - tpt_log)) | slot;
   const uint32_t dom_log = p.N - p.rep_log;
   const uint32_t tpc_log = dom_log - p.T;  // log2 tiles per column
-  // Tiles are numbered column-fastest: consecutive workgroups, which the
-  // dispatcher spreads round-robin over the XCDs, take the same tile of
-  // successive columns, so each XCD's L2 serves that tile's twiddles to all
-  // of its workgroups.  The host checks that tile numbers fit 32 bits.
+  // Tiles are numbered column-major: concurrent workgroups stream through
+  // consecutive 16 KB chunks of one column, spread over the HBM channels.
+  // (Numbering them column-fastest, so that an XCD's L2 serves one tile's
+  // twiddles to many columns, cut the low pass's fetched bytes by a third
+  // but measured 2% slower over the whole step: the 256 columns' same offsets
+  // were then read at once.)
   const bool active = gtile < ((uint64_t)p.n_cols << tpc_log);
   // Lanes past the last tile (a workgroup holds several small tiles) take
   // tile 0 of column 0: they load no data and store nothing, but they do
   // read twiddles, and tile 0's twiddle reads stay inside the table.
   uint32_t tile = 0, col_idx = 0;
   if (active) {
-    tile = u32div((uint32_t)gtile, p.cols_div);
-    col_idx = (uint32_t)gtile - tile * p.n_cols;
+    tile = (uint32_t)(gtile & ((1ull << tpc_log) - 1));
+    col_idx = (uint32_t)(gtile >> tpc_log);
   }
   const uint32_t *in = nullptr;
   uint32_t *out = nullptr;
@@ -679,7 +681,6 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
     uint64_t wgs = (total_tiles + tiles_per_wg - 1) / tiles_per_wg;
     STWO_HIP_CHECK_ARG(ctx, wgs * tiles_per_wg <= (1ull << 32), "FFT group of %u columns of log %u too large",
                        n_cols, N);
-    p.cols_div = make_u32div(n_cols);
     p.vec_io = 0;
     if (aligned16 && p.w == 0 && p.st[0].r_hi > 0) {
       if (p.st[0].b0 == 0 && p.in_log >= 4) p.vec_io |= 1;

@@ -18,33 +18,6 @@ struct FftCol {
 
 constexpr uint32_t kMaxStages = 4;
 
-// Unsigned 32-bit division by a fixed divisor d >= 1 with a multiply-high
-// (Granlund and Montgomery, "Division by Invariant Integers using
-// Multiplication", PLDI 1994, figure 4.1): with l = ceil(log2 d) and
-// m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0)
-// where t = mulhi(m, n), for every n < 2^32.
-struct U32Div {
-  uint32_t m, s1, s2, d;
-};
-inline U32Div make_u32div(uint32_t d) {
-  uint32_t l = 0;
-  while (l < 32 && (1ull << l) < d) l++;
-  U32Div r;
-  r.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
-  r.s1 = l ? 1 : 0;
-  r.s2 = l ? l - 1 : 0;
-  r.d = d;
-  return r;
-}
-__host__ __device__ __forceinline__ uint32_t u32div(uint32_t n, const U32Div &v) {
-#ifdef __HIP_DEVICE_COMPILE__
-  uint32_t t = __umulhi(v.m, n);
-#else
-  uint32_t t = (uint32_t)(((uint64_t)v.m * n) >> 32);
-#endif
-  return (t + ((n - t) >> v.s1)) >> v.s2;
-}
-
 struct FftStage {
   uint32_t b0, r_lo, r_hi;
 };
@@ -60,8 +33,6 @@ struct FftPass2 {
   uint32_t in_log;
   uint32_t scale;
   uint32_t rep_log;  // output blocks per tile (first RFFT pass of a blowup)
-  // Division of a tile number by n_cols (tiles are numbered column-fastest).
-  U32Div cols_div;
   // Bit 0: the first stage's layout (b0 = 0, 16 consecutive words per lane)
   // is loaded straight from HBM with 128-bit loads; bit 1: the last stage's
   // layout (b0 = 0) is stored the same way.  Set only for contiguous tiles
