@@ -299,6 +299,24 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
 int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
                            const stwo_hip_secure_point *point, uint32_t *values);
 
+/* ---- wide-Fibonacci AIR (the prove benchmark's component) ----------------------
+ * vendor/stwo-upstream-dev-62b228e/crates/examples/src/wide_fibonacci/mod.rs.
+ * stwo_hip_wide_fib_trace: generate_trace (mod.rs:27-50) -- row i of the n_cols
+ * columns (2^log_n words each, device) is the sequence a[i], b[i], then
+ * x_k = x_{k-2}^2 + x_{k-1}^2; a and b are device arrays of 2^log_n values.
+ * stwo_hip_wide_fib_constraint_quotients: the component's
+ * evaluate_constraint_quotients_on_domain (constraint-framework/src/prover/
+ * component_prover.rs:170-274) for a single component: `cols` are the trace
+ * columns evaluated on CanonicCoset(log_n + 1) (bit-reversed, the trace
+ * committed at blowup 1); accum (4 coordinate columns of 2^(log_n + 1)) is
+ * incremented by the random-linear combination of the N - 2 constraints
+ * (constraint j weighted alpha^(N - 3 - j)) divided by the trace domain's
+ * vanishing polynomial. */
+int stwo_hip_wide_fib_trace(stwo_hip_ctx *ctx, uint32_t *const *cols, uint32_t n_cols, uint32_t log_n,
+                            const uint32_t *a, const uint32_t *b);
+int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
+                                           uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4]);
+
 /* ---- proof of work -----------------------------------------------------------
  * GrindOps::grind for the Blake2s channel as SimdBackend computes it
  * (STWO/prover/backend/simd/grind.rs:22-62; Metal MB/blake2s.rs:399-443,

@@ -68,6 +68,9 @@ def lib():
         L.or_compute_fri_quotients.argtypes = [ctypes.POINTER(_u32p), _u32p, ctypes.c_size_t, ctypes.c_void_p, _u32p,
                                                _u32p, ctypes.c_uint32, ctypes.POINTER(_u32p)]
         L.or_eval_at_point.argtypes = [_u32p, ctypes.c_uint32, _u32p, _u32p, _u32p]
+        L.or_wide_fib_trace.argtypes = [ctypes.POINTER(_u32p), ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
+        L.or_wide_fib_constraint_quotients.argtypes = [ctypes.POINTER(_u32p), ctypes.c_uint32, ctypes.c_uint32,
+                                                       _u32p, ctypes.POINTER(_u32p)]
         L.or_pow_prefixed_digest.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_char_p]
         L.or_grind.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
@@ -313,6 +316,27 @@ def eval_at_point(coeffs, point):
     out = np.zeros(4, dtype=np.uint32)
     lib().or_eval_at_point(_p(c), int(len(c)).bit_length() - 1, _p(x), _p(y), _p(out))
     return tuple(int(v) for v in out)
+
+
+# --- wide-Fibonacci AIR ----------------------------------------------------------
+
+def wide_fib_trace(a, b, n_cols):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    b = np.ascontiguousarray(b, dtype=np.uint32)
+    log_n = int(len(a)).bit_length() - 1
+    cols = [np.zeros(len(a), dtype=np.uint32) for _ in range(n_cols)]
+    lib().or_wide_fib_trace(_ptr_array(cols), n_cols, log_n, _p(a), _p(b))
+    return cols
+
+
+def wide_fib_constraint_quotients(ext_cols, log_n, alpha, accum=None):
+    """Adds the component's constraint quotients to accum (4 columns of 2^(log_n+1)); returns accum."""
+    if accum is None:
+        accum = [np.zeros(2 << log_n, dtype=np.uint32) for _ in range(4)]
+    al = np.array(alpha, dtype=np.uint32)
+    rc = lib().or_wide_fib_constraint_quotients(_ptr_array(ext_cols), len(ext_cols), log_n, _p(al), _ptr_array(accum))
+    assert rc == 0
+    return accum
 
 
 # --- FRI quotients -------------------------------------------------------------

@@ -767,6 +767,76 @@ void or_eval_at_point(const uint32_t *coeffs, uint32_t log_size, const uint32_t 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Wide-Fibonacci AIR: generate_trace (examples/src/wide_fibonacci/mod.rs:27-50) */
+/* and the single-component constraint quotients (constraint-framework/src/     */
+/* prover/component_prover.rs:170-274, CpuDomainEvaluator::add_constraint,      */
+/* DomainEvaluationAccumulator, STWO/prover/air/accumulation.rs:30-68;           */
+/* coset_vanishing, STWO/core/constraints.rs:12-35).                             */
+/* ------------------------------------------------------------------------ */
+void or_wide_fib_trace(uint32_t **cols, uint32_t n_cols, uint32_t log_n, const uint32_t *a, const uint32_t *b) {
+    size_t n = (size_t)1 << log_n;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t x0 = a[i], x1 = b[i];
+        cols[0][i] = x0; cols[1][i] = x1;
+        for (uint32_t k = 2; k < n_cols; k++) {
+            uint32_t c = m31_add(m31_mul(x0, x0), m31_mul(x1, x1));
+            cols[k][i] = c; x0 = x1; x1 = c;
+        }
+    }
+}
+
+/* coset_vanishing(coset, p) for p in the base field: shift p by -initial +
+ * step/2 (indices), then double its x log_size - 1 times. */
+static uint32_t coset_vanishing_m31(uint32_t coset_initial, uint32_t coset_log, cpoint p) {
+    uint32_t shift = (uint32_t)((((uint64_t)1 << 31) - coset_initial + ((1ull << (31 - coset_log)) >> 1)) & IDX_MASK);
+    cpoint q = cp_add(p, index_to_point(shift));
+    uint32_t x = q.x;
+    for (uint32_t d = 1; d < coset_log; d++) x = m31_sub(m31_add(m31_mul(x, x), m31_mul(x, x)), 1);
+    return x;
+}
+
+int or_wide_fib_constraint_quotients(uint32_t **cols, uint32_t n_cols, uint32_t log_n, const uint32_t alpha4[4],
+                                     uint32_t **accum) {
+    if (n_cols < 3) return -1;
+    uint32_t eval_log = log_n + 1, n_cons = n_cols - 2;
+    /* DomainEvaluationAccumulator::new: powers alpha^0 .. alpha^(total-1); the
+     * component takes the last n_cons (all of them) and reverses them. */
+    qm31 alpha = qm31_from4(alpha4);
+    qm31 *pw = (qm31 *)malloc(n_cons * sizeof(qm31));
+    qm31 p = {{1, 0}, {0, 0}};
+    for (uint32_t e = 0; e < n_cons; e++) { pw[e] = p; p = qm31_mul(p, alpha); }
+    for (uint32_t e = 0; e < n_cons / 2; e++) { qm31 t = pw[e]; pw[e] = pw[n_cons - 1 - e]; pw[n_cons - 1 - e] = t; }
+    /* denominators at eval_domain.at(i), i < 2^(eval_log - log_n), bit-reversed */
+    uint32_t log_expand = eval_log - log_n;
+    size_t ne = (size_t)1 << log_expand;
+    uint32_t *dinv = (uint32_t *)malloc(ne * 4);
+    uint32_t coset_initial = (uint32_t)((1ull << (31 - (log_n + 1))) & IDX_MASK); /* Coset::odds(log_n) */
+    uint32_t half_init = half_odds_initial(eval_log - 1);
+    uint32_t half_step = (uint32_t)((1ull << (31 - (eval_log - 1))) & IDX_MASK);
+    for (size_t i = 0; i < ne; i++) {
+        cpoint dp = index_to_point((uint32_t)((half_init + (uint64_t)half_step * i) & IDX_MASK));
+        dinv[bitrev(i, log_expand)] = m31_inv(coset_vanishing_m31(coset_initial, log_n, dp));
+    }
+    size_t n = (size_t)1 << eval_log;
+    #pragma omp parallel for schedule(static)
+    for (size_t r = 0; r < n; r++) {
+        qm31 res = {{0, 0}, {0, 0}};
+        uint32_t x0 = cols[0][r], x1 = cols[1][r];
+        for (uint32_t k = 2; k < n_cols; k++) {
+            uint32_t c = cols[k][r];
+            uint32_t con = m31_sub(c, m31_add(m31_mul(x0, x0), m31_mul(x1, x1)));
+            res = qm31_add(res, qm31_mul_m31(pw[k - 2], con));
+            x0 = x1; x1 = c;
+        }
+        res = qm31_mul_m31(res, dinv[r >> log_n]);
+        accum[0][r] = m31_add(accum[0][r], res.a.a); accum[1][r] = m31_add(accum[1][r], res.a.b);
+        accum[2][r] = m31_add(accum[2][r], res.b.a); accum[3][r] = m31_add(accum[3][r], res.b.b);
+    }
+    free(pw); free(dinv);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* FRI quotients: compute_fri_quotients (STWO/prover/pcs/quotient_ops.rs:131-232) */
 /* with CpuBackend's QuotientOps (CPUB/quotients.rs) and AccumulationOps::       */
 /* lift_and_accumulate (CPUB/accumulation.rs), helpers from                      */

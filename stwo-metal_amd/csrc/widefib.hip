@@ -1,0 +1,135 @@
+// The wide-Fibonacci AIR of the reference's prove benchmark
+// (vendor/stwo-upstream-dev-62b228e/crates/examples/src/wide_fibonacci/mod.rs):
+// trace generation and the constraint-quotient evaluation that the
+// composition polynomial is built from.
+//
+// Trace (generate_trace, mod.rs:27-50): row i holds one sequence of N
+// values, a, b, then x_k = x_{k-2}^2 + x_{k-1}^2; column k is stored with
+// row i at index i.
+//
+// Constraint quotients (FrameworkComponent::evaluate_constraint_quotients_on_domain,
+// constraint-framework/src/prover/component_prover.rs:170-274): on
+// CanonicCoset(log_n + 1) (the trace committed at blowup 1), row r gets
+//   accum[r] + (sum_k alpha^(N-3-k) (x_{k+2} - x_k^2 - x_{k+1}^2)) * denom_inv[r >> log_n]
+// where the powers come from DomainEvaluationAccumulator (generated
+// alpha^0.., reversed for the component, STWO/prover/air/accumulation.rs:30-68,
+// CpuDomainEvaluator::add_constraint) and denom_inv holds the inverses of
+// the trace coset's vanishing polynomial at the first 2^(eval - trace)
+// points of the evaluation domain, bit-reversed (coset_vanishing,
+// STWO/core/constraints.rs:12-35).
+#include <string.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "field.hpp"
+
+namespace stwo_hip {
+
+__global__ void __launch_bounds__(256) wide_fib_trace_kernel(uint32_t *const *cols, uint32_t n_cols, uint32_t log_n,
+                                                             const uint32_t *a_in, const uint32_t *b_in) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >> log_n) return;
+  uint32_t a = a_in[i], b = b_in[i];
+  as_global(cols[0])[i] = a;
+  as_global(cols[1])[i] = b;
+  for (uint32_t k = 2; k < n_cols; k++) {
+    const uint32_t c = m31_add(m31_mul(a, a), m31_mul(b, b));
+    as_global(cols[k])[i] = c;
+    a = b;
+    b = c;
+  }
+}
+
+__global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t *const *cols, uint32_t n_cols,
+                                                                 uint32_t eval_log, uint32_t log_n,
+                                                                 const uint32_t *powers, const uint32_t *denom_inv,
+                                                                 uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
+                                                                 uint32_t *acc3) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >> eval_log) return;
+  uint32_t a = as_global(cols[0])[r], b = as_global(cols[1])[r];
+  qm31 res = qm31_make(0, 0, 0, 0);
+  for (uint32_t k = 2; k < n_cols; k++) {
+    const uint32_t c = as_global(cols[k])[r];
+    const uint32_t con = m31_sub(c, m31_add(m31_mul(a, a), m31_mul(b, b)));
+    const uint32_t *pw = powers + 4 * (k - 2);  // alpha^(N-3-(k-2)) for constraint k-2
+    res = qm31_add(res, qm31_mul_m31(qm31_make(pw[0], pw[1], pw[2], pw[3]), con));
+    a = b;
+    b = c;
+  }
+  res = qm31_mul_m31(res, denom_inv[r >> log_n]);
+  acc0[r] = m31_add(acc0[r], res.v[0]);
+  acc1[r] = m31_add(acc1[r], res.v[1]);
+  acc2[r] = m31_add(acc2[r], res.v[2]);
+  acc3[r] = m31_add(acc3[r], res.v[3]);
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_wide_fib_trace(stwo_hip_ctx *ctx, uint32_t *const *cols, uint32_t n_cols, uint32_t log_n,
+                            const uint32_t *a, const uint32_t *b) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, cols && a && b && n_cols >= 2 && n_cols <= 4096 && log_n >= 1 && log_n <= 28,
+                     "wide_fib_trace: bad arguments");
+  for (uint32_t k = 0; k < n_cols; k++) STWO_HIP_CHECK_ARG(ctx, cols[k] != nullptr, "wide_fib_trace: NULL column");
+  void *d_tab = nullptr;
+  int rc = upload_table(ctx, cols, n_cols * sizeof(uint32_t *), &d_tab);
+  if (rc) return rc;
+  const uint64_t n = 1ull << log_n;
+  return launch(ctx, "wide_fib_trace", wide_fib_trace_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+                (uint32_t *const *)d_tab, n_cols, log_n, a, b);
+}
+
+int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
+                                           uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4]) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, cols && random_coeff && accum && n_cols >= 3 && n_cols <= 4096 && log_n >= 1 &&
+                              log_n <= 27,
+                     "wide_fib_constraint_quotients: bad arguments");
+  for (uint32_t k = 0; k < n_cols; k++)
+    STWO_HIP_CHECK_ARG(ctx, cols[k] != nullptr, "wide_fib_constraint_quotients: NULL column");
+  for (int k = 0; k < 4; k++) STWO_HIP_CHECK_ARG(ctx, accum[k] != nullptr, "wide_fib_constraint_quotients: NULL accumulator");
+  const uint32_t eval_log = log_n + 1;
+  const uint32_t n_cons = n_cols - 2;
+  // Powers alpha^0 .. alpha^(n_cons - 1), reversed: constraint j uses alpha^(n_cons - 1 - j).
+  std::vector<uint32_t> powers(4 * n_cons);
+  {
+    const qm31 alpha = qm31_make(random_coeff[0], random_coeff[1], random_coeff[2], random_coeff[3]);
+    qm31 p = qm31_make(1, 0, 0, 0);
+    for (uint32_t e = 0; e < n_cons; e++) {
+      memcpy(&powers[4 * (n_cons - 1 - e)], p.v, 16);
+      p = qm31_mul(p, alpha);
+    }
+  }
+  // Vanishing-polynomial inverses at the first 2^(eval_log - log_n) = 2 points of
+  // CanonicCoset(eval_log).circle_domain(), bit-reversed.  The trace coset is
+  // canonic, so its shift (-initial + step / 2) is zero and v(p) = pi^(log_n - 1)(p.x).
+  uint32_t dinv[2];
+  for (uint32_t i = 0; i < 2; i++) {
+    // domain.at(i) for i < half: half coset half_odds(eval_log - 1) at i
+    const uint32_t idx =
+        (uint32_t)((half_odds_initial(eval_log - 1) + (uint64_t)coset_step(eval_log - 1) * i) & IDX_MASK);
+    uint32_t x = host_index_to_point(idx).x;
+    for (uint32_t d = 1; d < log_n; d++) x = m31_sub(m31_add(m31_mul(x, x), m31_mul(x, x)), 1);
+    STWO_HIP_CHECK_ARG(ctx, x != 0, "wide_fib_constraint_quotients: vanishing polynomial is zero");
+    dinv[bitrev32(i, 1)] = m31_inv(x);
+  }
+  void *d_cols = nullptr, *d_pw = nullptr, *d_dinv = nullptr;
+  int rc = upload_table(ctx, cols, n_cols * sizeof(uint32_t *), &d_cols);
+  if (rc) return rc;
+  rc = upload_table(ctx, powers.data(), powers.size() * 4, &d_pw);
+  if (rc) return rc;
+  rc = upload_table(ctx, dinv, sizeof dinv, &d_dinv);
+  if (rc) return rc;
+  const uint64_t n = 1ull << eval_log;
+  return launch(ctx, "wide_fib_constraint_quotients", wide_fib_quotients_kernel, dim3((uint32_t)((n + 255) / 256)),
+                dim3(256), 0, (const uint32_t *const *)d_cols, n_cols, eval_log, log_n, (const uint32_t *)d_pw,
+                (const uint32_t *)d_dinv, accum[0], accum[1], accum[2], accum[3]);
+}
+
+}  // extern "C"

@@ -148,6 +148,9 @@ def lib():
             "stwo_hip_compute_fri_quotients": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_void_p,
                                                _u32p, _u32p, ctypes.c_uint32, ctypes.POINTER(_vp)],
             "stwo_hip_eval_at_point": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, _u32p, _u32p],
+            "stwo_hip_wide_fib_trace": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
+            "stwo_hip_wide_fib_constraint_quotients": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
+                                                       _u32p, ctypes.POINTER(_vp)],
             "stwo_hip_grind": [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
             "stwo_hip_debug_fft_plan": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.c_uint32,
                                         _u32p],
@@ -425,6 +428,16 @@ class Context:
                                                  len(coeff_cols), pt.ctypes.data_as(_u32p),
                                                  out.ctypes.data_as(_u32p)))
         return out[: 4 * len(coeff_cols)].reshape(-1, 4)
+
+    def wide_fib_trace(self, cols, log_n, a, b):
+        self._check(lib().stwo_hip_wide_fib_trace(self.h, _ptrs(cols), len(cols), log_n,
+                                                  a.ptr if isinstance(a, DeviceBuf) else a,
+                                                  b.ptr if isinstance(b, DeviceBuf) else b))
+
+    def wide_fib_constraint_quotients(self, cols, log_n, random_coeff, accum4):
+        rc = np.ascontiguousarray(random_coeff, dtype=np.uint32)
+        self._check(lib().stwo_hip_wide_fib_constraint_quotients(self.h, _ptrs(cols), len(cols), log_n,
+                                                                 rc.ctypes.data_as(_u32p), _ptrs(accum4)))
 
     def grind(self, digest: bytes, pow_bits):
         """GrindOps::grind (SimdBackend rule) for the Blake2s channel state `digest`."""

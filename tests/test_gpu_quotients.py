@@ -116,3 +116,39 @@ def test_eval_at_point_vs_oracle(ctx):
             assert tuple(int(v) for v in g) == O.eval_at_point(c, pt)
     for x in d:
         x.free()
+
+
+@pytest.mark.parametrize("log_n,n_cols", [(5, 8), (12, 100), (16, 256)])
+def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols):
+    """Trace generation and the constraint quotients, bit-exact against the
+    oracle; the GPU's quotients are low degree (checked at log 5)."""
+    rng = np.random.default_rng(log_n)
+    a = rng.integers(0, P, 1 << log_n, dtype=np.uint64).astype(U)
+    b = rng.integers(0, P, 1 << log_n, dtype=np.uint64).astype(U)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    cols = [ctx.alloc(1 << log_n) for _ in range(n_cols)]
+    ctx.wide_fib_trace(cols, log_n, da, db)
+    tr = O.wide_fib_trace(a, b, n_cols)
+    for c, t in zip(cols, tr):
+        assert np.array_equal(c.download(), t)
+    # extend on the host oracle (the GPU FFT is checked elsewhere), then constrain on both
+    tw, itw = O.precompute_twiddles(O.half_odds_initial(log_n), log_n)
+    coeffs = [t.copy() for t in tr]
+    O.interpolate_columns(coeffs, itw)
+    ext = O.evaluate_polynomials(coeffs, 1, tw)
+    dext = [ctx.to_device(e) for e in ext]
+    base = [rng.integers(0, P, 2 << log_n, dtype=np.uint64).astype(U) for _ in range(4)]
+    acc = [ctx.to_device(x) for x in base]
+    ctx.wide_fib_constraint_quotients(dext, log_n, ALPHA, acc)
+    want = O.wide_fib_constraint_quotients(ext, log_n, ALPHA, [x.copy() for x in base])
+    got = [x.download() for x in acc]
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+    if log_n == 5:
+        zero = [ctx.to_device(np.zeros(2 << log_n, U)) for _ in range(4)]
+        ctx.wide_fib_constraint_quotients(dext, log_n, ALPHA, zero)
+        for z in zero:
+            assert not interp(z.download(), log_n + 1)[(1 << log_n) + 1:].any()
+            z.free()
+    for x in [da, db] + cols + dext + acc:
+        x.free()
