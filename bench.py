@@ -229,10 +229,10 @@ def serving(rates=(10.0, 30.0, 60.0), n_jobs=30, budget=96 << 30, n_slots=4):
 
 # Profiling names of the library's launches -> kernel symbols in rocprofv3 output.
 ROCPROF_NAMES = {
-    "rfft_pass_wg8": "void stwo_hip::fft_pass_kernel<false, 8>(stwo_hip::FftPass2)",
-    "rfft_pass_wg9": "void stwo_hip::fft_pass_kernel<false, 9>(stwo_hip::FftPass2)",
-    "ifft_pass_wg8": "void stwo_hip::fft_pass_kernel<true, 8>(stwo_hip::FftPass2)",
-    "ifft_pass_wg9": "void stwo_hip::fft_pass_kernel<true, 9>(stwo_hip::FftPass2)",
+    "rfft_pass_wg8": "void stwo_hip::fft_pass_kernel<false, 8, 3, 8, 4, 0>(stwo_hip::FftPass2)",
+    "rfft_pass_wg9": "void stwo_hip::fft_pass_kernel<false, 9, 2, 9, 5, 0>(stwo_hip::FftPass2)",
+    "ifft_pass_wg8": "void stwo_hip::fft_pass_kernel<true, 8, 3, 0, 4, 8>(stwo_hip::FftPass2)",
+    "ifft_pass_wg9": "void stwo_hip::fft_pass_kernel<true, 9, 2, 5, 9, 0>(stwo_hip::FftPass2)",
     "merkle_commit_on_layer": "stwo_hip::merkle_layer_kernel(...)",
     "merkle_nodes": "stwo_hip::merkle_nodes_kernel(...)",
 }

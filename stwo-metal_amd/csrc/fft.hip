@@ -401,7 +401,51 @@ __device__ __forceinline__ void lds_get(const uint32_t *L, uint32_t b0, uint32_t
 // the coefficient domain 2^(N - rep_log) and each workgroup writes the tile
 // of every one of the 2^rep_log output blocks: the coefficients are read
 // once from HBM (repeats hit L2) and the blocks differ only in twiddles.
-template <bool INV, int WG_LOG>
+// One stage of a compile-time schedule: the register group at tile bit B0,
+// all 4 layers of it.  PREV is the previous stage's group, or -1 when it is
+// only known at run time (the first stage follows the load layout).
+template <bool INV, int B0, int PREV, typename GB, typename GI>
+__device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16], uint32_t *L, uint32_t &cur,
+                                            uint32_t local, GB group_base, GI gidx) {
+  // Keep the scheduler from hoisting later stages' twiddle loads above this
+  // point: with the whole schedule visible it would otherwise keep several
+  // stages' twiddles live at once and spill.
+  __builtin_amdgcn_sched_barrier(0);
+  const uint32_t base = ((local >> B0) << (B0 + 4)) | (local & ((1u << B0) - 1));
+  const uint32_t gb = gidx(base);
+  const uint32_t sh = B0 - p.w + p.lo;
+  StageTw t;
+  load_stage_twiddles(t, p.tw_end, p.N, sh, gb, 0, 4);
+  if constexpr (PREV >= 0) {
+    if constexpr (PREV != B0) {
+      const uint32_t cb = ((local >> PREV) << (PREV + 4)) | (local & ((1u << PREV) - 1));
+      __syncthreads();
+      lds_put16<PREV>(L + cb + (cb >> 4), x);
+      __syncthreads();
+      lds_get16<B0>(L + base + (base >> 4), x);
+    }
+  } else {
+    if (cur != B0) {
+      __syncthreads();
+      lds_put(L, cur, group_base(cur), x);
+      __syncthreads();
+      lds_get16<B0>(L + base + (base >> 4), x);
+    }
+  }
+  cur = B0;
+  stage_layers<INV, 0, 4>(x, t);
+}
+
+// The fixed schedules need about 125 VGPRs (4 waves per SIMD); asking for 5
+// makes them spill ~30 VGPRs and run at half speed.  Measured on one box,
+// 256 x 2^20 -> 2^22: 9.06 ms per step against 9.48 ms with the run-time
+// schedule at 5 waves (the 8-layer strided RFFT pass 1.95 ms against 2.28).
+//
+// NS > 0: the pass's stages are the compile-time schedule B0A, B0B, B0C
+// (first NS of them), each applying all 4 layers of its group; the host only
+// picks such a kernel for passes whose plan is exactly that schedule.
+// NS = 0: the schedule is read from p.st at run time.
+template <bool INV, int WG_LOG, int NS, int B0A, int B0B, int B0C>
 __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, uint32_t *lds) {
   const uint32_t tpt_log = p.T - 4;  // log2 lanes per tile
   const uint32_t local = threadIdx.x & ((1u << tpt_log) - 1);
@@ -479,6 +523,11 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         for (int j = 0; j < 16; j++) x[j] = gld(in + (((uint32_t)j << sh) & in_mask), vo);
       }
     }
+    if constexpr (NS > 0) {
+      fixed_stage<INV, B0A, -1>(p, x, L, cur, local, group_base, gidx);
+      if constexpr (NS > 1) fixed_stage<INV, B0B, B0A>(p, x, L, cur, local, group_base, gidx);
+      if constexpr (NS > 2) fixed_stage<INV, B0C, B0B>(p, x, L, cur, local, group_base, gidx);
+    } else
     for (uint32_t si = 0; si < p.n_stages; si++) {
       const FftStage S = p.st[si];
       const uint32_t base = group_base(S.b0);
@@ -528,10 +577,10 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
   }
 }
 
-template <bool INV, int WG_LOG>
-__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(5))) fft_pass_kernel(FftPass2 p) {
+template <bool INV, int WG_LOG, int NS = 0, int B0A = 0, int B0B = 0, int B0C = 0>
+__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(NS > 0 ? 4 : 5))) fft_pass_kernel(FftPass2 p) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
-  fft_pass_body<INV, WG_LOG>(p, blockIdx.x, lds);
+  fft_pass_body<INV, WG_LOG, NS, B0A, B0B, B0C>(p, blockIdx.x, lds);
 }
 
 // Several passes of different column groups (sizes) in one launch: group g
@@ -544,7 +593,7 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
   uint32_t g = 0;
   while (g + 1 < n_groups && wg_start[g + 1] <= blockIdx.x) g++;
   const FftPass2 p = groups[g];
-  fft_pass_body<INV, WG_LOG>(p, blockIdx.x - wg_start[g], lds);
+  fft_pass_body<INV, WG_LOG, 0, 0, 0, 0>(p, blockIdx.x - wg_start[g], lds);
 }
 
 // Stages covering layer tile bits [A, B) of a tile of T bits, forward order.
@@ -691,8 +740,26 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
   return STWO_HIP_OK;
 }
 
+// The compile-time schedules: every stage applies its group's 4 layers, and
+// the groups are T-4, T-8, ... (forward) or the reverse (inverse).  These are
+// the passes of columns of log size >= 16 split 12 + 8 + ..., the low pass
+// (T = 12) and the 8-layer strided passes (T = 13).
+static bool full_schedule(const FftPass2 &p, const uint32_t *b0s, uint32_t ns) {
+  if (p.n_stages != ns) return false;
+  for (uint32_t i = 0; i < ns; i++)
+    if (p.st[i].b0 != b0s[i] || p.st[i].r_lo != 0 || p.st[i].r_hi != 4) return false;
+  return true;
+}
+
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log) {
-  // Profiling names follow the instantiation: *_wg9 = fft_pass_kernel<INV, 9>.
+  static const uint32_t kFwd12[3] = {8, 4, 0}, kInv12[3] = {0, 4, 8}, kFwd13[2] = {9, 5}, kInv13[2] = {5, 9};
+  if (wg_log == 8 && p.T == 12 && full_schedule(p, inverse ? kInv12 : kFwd12, 3))
+    return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 3, 0, 4, 8>, dim3(wgs), dim3(256), 0, p)
+                   : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 3, 8, 4, 0>, dim3(wgs), dim3(256), 0, p);
+  if (wg_log == 9 && p.T == 13 && full_schedule(p, inverse ? kInv13 : kFwd13, 2))
+    return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, 2, 5, 9>, dim3(wgs), dim3(512), 0, p)
+                   : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, 2, 9, 5>, dim3(wgs), dim3(512), 0, p);
+  // Profiling names follow the instantiation: *_wg9 = fft_pass_kernel<INV, 9, ...>.
   if (wg_log == 9)
     return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9>, dim3(wgs), dim3(512), 0, p)
                    : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9>, dim3(wgs), dim3(512), 0, p);
