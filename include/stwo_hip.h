@@ -47,6 +47,7 @@ extern "C" {
 #define STWO_HIP_EDEVICE (-3)  /* HIP runtime error                         */
 #define STWO_HIP_EDEGREE (-4)  /* FRI last layer is not of low degree       */
 #define STWO_HIP_EBUSY (-5)    /* serving queue full / budget exceeded      */
+#define STWO_HIP_ECONSTRAINTS (-6) /* prove: constraints not satisfied     */
 
 /* Hash output mode: Blake2sMerkleHasher or Blake2sM31MerkleHasher
  * (STWO/core/vcs/blake2_merkle.rs:8-10): the M31 variant reduces each output
@@ -258,7 +259,8 @@ int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *prover,
  * QM31, *n_fri of them); hash_witness (8 words per hash, *n_hash).  If
  * `positions` is not NULL it receives the layer's decommitment positions
  * (*n_pos <= pos_cap) and `values` (if not NULL) their QM31 values, 4 words
- * each (FriLayerProofAux::all_values). */
+ * each (FriLayerProofAux::all_values).  With fri_witness, hash_witness and
+ * positions all NULL the call only reports the three counts. */
 int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *prover, uint32_t layer,
                                     uint32_t commitment[8], uint32_t *fri_witness, size_t fri_cap, size_t *n_fri,
                                     uint32_t *hash_witness, size_t hash_cap, size_t *n_hash, uint32_t *positions,
@@ -316,6 +318,48 @@ int stwo_hip_wide_fib_trace(stwo_hip_ctx *ctx, uint32_t *const *cols, uint32_t n
                             const uint32_t *a, const uint32_t *b);
 int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
                                            uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4]);
+
+/* ---- end-to-end prove (wide-Fibonacci AIR) -------------------------------------
+ * stwo's prove (STWO/prover/mod.rs:32-144) of one wide-Fibonacci component
+ * with a Blake2sChannel and Blake2sMerkleChannel, preceded by the commits the
+ * reference's benchmark makes before calling it (an empty preprocessed tree,
+ * then the trace tree): the flow of run_one_sample in
+ * fixtures/standalone-benchmarks/src/bin/wide_fibonacci_simd_prove.rs, and of
+ * the Metal V1 prove runtime (MB/prove_runtime_v1.rs:1079 execute_prove_core_v1,
+ * :1181 commit_trace_with_breakdown) that the reference benchmarks against it.
+ *
+ * The trace is generate_trace (examples wide_fibonacci/mod.rs:27-50) of the
+ * device inputs a, b (2^log_n values each): n_cols columns of 2^log_n rows.
+ * config is PcsConfig (STWO/core/pcs/mod.rs:27-40) with lifting_log_size None;
+ * PcsConfig::default() is {10, {1, 0, 3, 1}}.  tw must cover
+ * CanonicCoset(log_n + log_blowup_factor).  channel_digest / channel_n_draws
+ * are the channel's state, read and updated (zeros = Blake2sChannel::default()).
+ * Returns STWO_HIP_ECONSTRAINTS when prove_ex's own OODS self-check fails (a
+ * trace that does not satisfy the AIR).  Synchronizes.
+ *
+ * stwo_hip_stark_proof_bytes writes the StarkProof<Blake2sMerkleHasher> in the
+ * layout bincode 1's default options give the reference's serde derives
+ * (fixed-width little-endian integers, u64 lengths and usize, Option as a
+ * one-byte tag, hashes as their 32 bytes, QM31 as 4 M31 words), fields in
+ * declaration order: CommitmentSchemeProof (STWO/core/pcs/quotients.rs:26-34),
+ * FriProof / FriLayerProof (STWO/core/fri.rs:360-397), LinePoly
+ * (STWO/core/poly/line.rs:113-123).  With buf NULL only *len is set.
+ * stwo_hip_stark_proof_info returns host wall times of the prove's phases
+ * (trace commit, composition commit, OODS samples, quotients, FRI commit,
+ * proof of work, decommitment) and the sorted query positions. */
+typedef struct {
+  uint32_t pow_bits;
+  stwo_hip_fri_config fri;
+} stwo_hip_pcs_config;
+#define STWO_HIP_PROVE_PHASES 7
+typedef struct stwo_hip_stark_proof stwo_hip_stark_proof;
+int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols, uint32_t log_n,
+                            stwo_hip_pcs_config config, const stwo_hip_twiddles *tw, uint8_t channel_digest[32],
+                            uint32_t *channel_n_draws, stwo_hip_stark_proof **out);
+int stwo_hip_stark_proof_bytes(const stwo_hip_stark_proof *proof, uint8_t *buf, size_t cap, size_t *len);
+int stwo_hip_stark_proof_info(const stwo_hip_stark_proof *proof, double phase_ms[STWO_HIP_PROVE_PHASES],
+                              uint32_t *query_positions, size_t cap, size_t *n_positions);
+int stwo_hip_stark_proof_destroy(stwo_hip_stark_proof *proof);
 
 /* ---- proof of work -----------------------------------------------------------
  * GrindOps::grind for the Blake2s channel as SimdBackend computes it

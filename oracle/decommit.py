@@ -363,11 +363,14 @@ def _rebuild(queries, query_evals, witness, fold_step, log_domain_size, err):
     return positions, subsets, initials
 
 
-def fri_verify(channel, cfg, proof, column_log_size, first_layer_query_evals_of):
+def fri_verify(channel, cfg, proof, column_log_size, first_layer_query_evals_of, before_queries=None):
     """FriVerifier::commit + sample_query_positions + decommit.  `proof` is
     (layer proofs as from fri_decommit, last layer poly as [k][4]);
     `first_layer_query_evals_of(queries)` returns the committed column's
-    QM31 values at the sorted query positions.  Raises VerificationError."""
+    QM31 values at the sorted query positions; `before_queries(channel)`, if
+    given, runs between the commit phase and the query draw (the PCS
+    verifier's proof-of-work check, STWO/core/pcs/verifier.rs:73-78).
+    Raises VerificationError."""
     layer_proofs, last_poly = proof
     log_blowup = cfg.log_blowup_factor
     column_bound = column_log_size - log_blowup
@@ -399,6 +402,8 @@ def fri_verify(channel, cfg, proof, column_log_size, first_layer_query_evals_of)
     if len(last_poly) > (1 << cfg.log_last_layer_degree_bound):
         raise VerificationError("LastLayerDegreeInvalid")
     channel.mix_felts([list(c) for c in last_poly])
+    if before_queries is not None:
+        before_queries(channel)
     # queries (fri.rs:281-287)
     raw = draw_queries(channel, column_log_size, cfg.n_queries)
     queries = queries_new(raw)

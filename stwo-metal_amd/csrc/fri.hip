@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "blake2s.hpp"
+#include "channel.hpp"
 #include "common.hpp"
 #include "field.hpp"
 #include "internal.hpp"
@@ -231,58 +232,6 @@ static int fold_line_steps(stwo_hip_ctx *ctx, const uint32_t *const src[4], uint
   }
   return STWO_HIP_OK;
 }
-
-// ---- host-side Blake2s channel (STWO/core/channel/blake2s.rs) -------------
-
-struct HostChannel {
-  uint8_t digest[32];
-  uint32_t n_draws;
-
-  void update(const uint8_t d[32]) {
-    memcpy(digest, d, 32);
-    n_draws = 0;
-  }
-  // MerkleChannel::mix_root: digest = H(digest || root), vcs_lifted/blake2_merkle.rs:49-55.
-  void mix_root(const uint32_t root[8]) {
-    b2_host h;
-    h.update(digest, 32);
-    h.update(root, 32);
-    uint8_t o[32];
-    h.finalize(o);
-    update(o);
-  }
-  // mix_felts, blake2s.rs:58-68.
-  void mix_felts(const uint32_t *felts4, size_t n) {
-    b2_host h;
-    h.update(digest, 32);
-    h.update(felts4, 16 * n);
-    uint8_t o[32];
-    h.finalize(o);
-    update(o);
-  }
-  // draw_u32s: H(digest || le32(n_draws) || 0), blake2s.rs:103-117.
-  void draw_u32s(uint32_t out[8]) {
-    uint8_t in[37];
-    memcpy(in, digest, 32);
-    memcpy(in + 32, &n_draws, 4);
-    in[36] = 0;
-    n_draws++;
-    b2_host h;
-    h.update(in, 37);
-    h.finalize((uint8_t *)out);
-  }
-  // draw_secure_felt via draw_base_felts (retry unless all words < 2P), blake2s.rs:36-51,85-88.
-  qm31 draw_secure_felt() {
-    uint32_t w[8];
-    for (;;) {
-      draw_u32s(w);
-      bool ok = true;
-      for (int i = 0; i < 8; i++) ok = ok && w[i] < 2 * P;
-      if (ok) break;
-    }
-    return qm31_make(b2_reduce_word(w[0]), b2_reduce_word(w[1]), b2_reduce_word(w[2]), b2_reduce_word(w[3]));
-  }
-};
 
 // LineEvaluation::interpolate on the host (STWO/prover/line.rs:54-99): the
 // last FRI layer has at most 2^(10 + 16) values and usually 2^1..2^4.
@@ -703,6 +652,7 @@ int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover
   *n_fri = d.fri_witness.size() / 4;
   *n_hash = d.hash_witness.size() / 8;
   *n_pos = d.positions.size();
+  if (!fri_witness && !hash_witness && !positions) return STWO_HIP_OK;  // size query
   STWO_HIP_CHECK_ARG(ctx, *n_fri <= fri_cap && *n_hash <= hash_cap && (!positions || *n_pos <= pos_cap),
                      "fri_prover_layer_proof: capacity too small (%zu, %zu, %zu needed)", *n_fri, *n_hash, *n_pos);
   STWO_HIP_CHECK_ARG(ctx, (fri_witness || !*n_fri) && (hash_witness || !*n_hash),

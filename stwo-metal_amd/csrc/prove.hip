@@ -1,0 +1,512 @@
+// End-to-end STARK prove of the wide-Fibonacci AIR on the device kernels: the
+// host flow of the reference's benchmark
+// (fixtures/standalone-benchmarks/src/bin/wide_fibonacci_simd_prove.rs
+// run_one_sample: Blake2sChannel, Blake2sMerkleChannel, an empty preprocessed
+// tree, the trace tree, then stwo's prove) with every bulk step on the GPU.
+//
+// Steps, in channel order (STWO = vendor/stwo-upstream-dev-62b228e/crates/stwo/src):
+//   1. preprocessed tree: no columns, root = Blake2s("") (CPUB/merkle_lifted.rs:44-48),
+//      mixed into the channel (CommitmentSchemeProver::commit, STWO/prover/pcs/mod.rs:368-380);
+//   2. trace tree: generate_trace, interpolate, evaluate at the blowup, lifted
+//      Merkle commit, mix the root (TreeBuilder::extend_evals + commit, :707-735);
+//   3. prove_ex (STWO/prover/mod.rs:41-144): draw the composition coefficient,
+//      evaluate the constraint quotients on CanonicCoset(log_n + 1), interpolate
+//      (DomainEvaluationAccumulator::finalize, STWO/prover/air/accumulation.rs:96-120),
+//      split the 4 coordinate polynomials at the middle (CpuBackend::split_at_mid,
+//      CPUB/circle.rs:283-291), commit the 8 halves, draw the OODS point
+//      (CirclePoint::get_random_point, STWO/core/circle.rs:169-181);
+//   4. prove_values (STWO/prover/pcs/mod.rs:528-698): every trace column and
+//      composition half sampled at the OODS point, values mixed into the
+//      channel, FRI quotients with a fresh coefficient, FRI commit, grind, mix
+//      the nonce, FRI decommit (query positions), then each tree decommitted
+//      at the query positions (prepare_query_positions_for_tree is the identity
+//      for trees at the lifting size and empty for the preprocessed tree,
+//      STWO/core/pcs/utils.rs:195-213);
+//   5. prove_ex's self-check: the composition value rebuilt from the sampled
+//      halves (StarkProof::extract_composition_oods_eval, STWO/core/proof.rs:26-56)
+//      must equal the constraints evaluated at the OODS point from the sampled
+//      trace values (FrameworkComponent::evaluate_constraint_quotients_at_point,
+//      constraint-framework/src/component.rs:271-300; PointEvaluationAccumulator,
+//      STWO/core/air/accumulation.rs).
+//
+// The OODS samples are computed from the coefficients (eval_at_point); the
+// reference's prover without stored coefficients uses barycentric weights on
+// the evaluations, which give the same field element for these polynomials.
+#include <string.h>
+
+#include <array>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "channel.hpp"
+#include "common.hpp"
+#include "field.hpp"
+#include "internal.hpp"
+
+namespace stwo_hip {
+
+// ---- host secure-field helpers ------------------------------------------------
+
+static qm31 q_from_m31(uint32_t a) { return qm31_make(a, 0, 0, 0); }
+
+// CM31 inverse: (a + bi)^-1 = (a - bi) / (a^2 + b^2).
+static void cm31_inv(uint32_t a, uint32_t b, uint32_t &ra, uint32_t &rb) {
+  const uint32_t n = m31_inv(m31_add(m31_mul(a, a), m31_mul(b, b)));
+  ra = m31_mul(a, n);
+  rb = m31_mul(m31_neg(b), n);
+}
+
+// QM31 inverse: (x + yu)^-1 = (x - yu) / (x^2 - (2 + i) y^2), the denominator in CM31.
+static qm31 qm31_inv_host(const qm31 &v) {
+  uint32_t x2r, x2i, y2r, y2i;
+  cm31_mul(v.v[0], v.v[1], v.v[0], v.v[1], x2r, x2i);
+  cm31_mul(v.v[2], v.v[3], v.v[2], v.v[3], y2r, y2i);
+  // (2 + i)(y2r + y2i i) = (2 y2r - y2i) + (y2r + 2 y2i) i
+  const uint32_t ry = m31_sub(m31_add(y2r, y2r), y2i), iy = m31_add(y2r, m31_add(y2i, y2i));
+  uint32_t dr, di;
+  cm31_inv(m31_sub(x2r, ry), m31_sub(x2i, iy), dr, di);
+  uint32_t ar, ai, br, bi;
+  cm31_mul(v.v[0], v.v[1], dr, di, ar, ai);
+  cm31_mul(m31_neg(v.v[2]), m31_neg(v.v[3]), dr, di, br, bi);
+  return qm31_make(ar, ai, br, bi);
+}
+
+// CirclePoint::double_x: 2x^2 - 1.
+static qm31 q_double_x(const qm31 &x) {
+  const qm31 sq = qm31_mul(x, x);
+  return qm31_sub(qm31_add(sq, sq), q_from_m31(1));
+}
+
+struct SecurePoint {
+  qm31 x, y;
+};
+
+// CirclePoint::<SecureField>::get_random_point (STWO/core/circle.rs:169-181).
+static SecurePoint random_point(HostChannel &ch) {
+  const qm31 t = ch.draw_secure_felt();
+  const qm31 t2 = qm31_mul(t, t);
+  const qm31 inv = qm31_inv_host(qm31_add(t2, q_from_m31(1)));
+  SecurePoint p;
+  p.x = qm31_mul(qm31_sub(q_from_m31(1), t2), inv);
+  p.y = qm31_mul(qm31_add(t, t), inv);
+  return p;
+}
+
+// SecureField::from_partial_evals (STWO/core/fields/qm31.rs:51-57).
+static qm31 from_partial_evals(const qm31 e[4]) {
+  qm31 r = e[0];
+  r = qm31_add(r, qm31_mul(e[1], qm31_make(0, 1, 0, 0)));
+  r = qm31_add(r, qm31_mul(e[2], qm31_make(0, 0, 1, 0)));
+  r = qm31_add(r, qm31_mul(e[3], qm31_make(0, 0, 0, 1)));
+  return r;
+}
+
+// ---- proof layout (bincode 1 defaults: fixed-width little-endian integers,
+// u64 sequence lengths, Option as a one-byte tag) ----------------------------------
+
+struct ByteWriter {
+  std::vector<uint8_t> b;
+  void u8(uint8_t v) { b.push_back(v); }
+  void u32(uint32_t v) {
+    for (int i = 0; i < 4; i++) b.push_back((uint8_t)(v >> (8 * i)));
+  }
+  void u64(uint64_t v) {
+    for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
+  }
+  void words(const uint32_t *w, size_t n) {
+    for (size_t i = 0; i < n; i++) u32(w[i]);
+  }
+};
+
+struct ProofLayer {
+  std::vector<uint32_t> fri_witness;   // 4 words per QM31
+  std::vector<uint32_t> hash_witness;  // 8 words per hash
+  uint32_t commitment[8];
+};
+
+}  // namespace stwo_hip
+
+struct stwo_hip_stark_proof {
+  stwo_hip_pcs_config config{};
+  std::vector<std::array<uint32_t, 8>> commitments;
+  // [tree][column] -> 4 words per sample
+  std::vector<std::vector<std::vector<uint32_t>>> sampled_values;
+  std::vector<std::vector<uint32_t>> decommitments;  // [tree] -> 8 words per hash
+  std::vector<std::vector<std::vector<uint32_t>>> queried_values;  // [tree][column] -> values
+  uint64_t proof_of_work = 0;
+  stwo_hip::ProofLayer first_layer;
+  std::vector<stwo_hip::ProofLayer> inner_layers;
+  std::vector<uint32_t> last_layer_poly;  // 4 words per coefficient
+  uint32_t last_layer_log_size = 0;
+  std::vector<uint32_t> query_positions;
+  double phase_ms[STWO_HIP_PROVE_PHASES] = {};
+};
+
+namespace stwo_hip {
+
+// CommitmentSchemeProof field order (STWO/core/pcs/quotients.rs:26-34).
+static void write_proof(const stwo_hip_stark_proof &p, ByteWriter &w) {
+  // PcsConfig { pow_bits, fri_config { log_blowup_factor, log_last_layer_degree_bound,
+  // n_queries: usize, line_fold_step }, lifting_log_size: None } (STWO/core/pcs/mod.rs:27-40,
+  // STWO/core/fri.rs:31-36)
+  w.u32(p.config.pow_bits);
+  w.u32(p.config.fri.log_blowup_factor);
+  w.u32(p.config.fri.log_last_layer_degree_bound);
+  w.u64(p.config.fri.n_queries);
+  w.u32(p.config.fri.line_fold_step);
+  w.u8(0);
+  w.u64(p.commitments.size());
+  for (const auto &c : p.commitments) w.words(c.data(), 8);
+  w.u64(p.sampled_values.size());
+  for (const auto &tree : p.sampled_values) {
+    w.u64(tree.size());
+    for (const auto &col : tree) {
+      w.u64(col.size() / 4);
+      w.words(col.data(), col.size());
+    }
+  }
+  w.u64(p.decommitments.size());
+  for (const auto &d : p.decommitments) {
+    w.u64(d.size() / 8);
+    w.words(d.data(), d.size());
+  }
+  w.u64(p.queried_values.size());
+  for (const auto &tree : p.queried_values) {
+    w.u64(tree.size());
+    for (const auto &col : tree) {
+      w.u64(col.size());
+      w.words(col.data(), col.size());
+    }
+  }
+  w.u64(p.proof_of_work);
+  // FriProof { first_layer, inner_layers, last_layer_poly } (STWO/core/fri.rs:360-397)
+  auto layer = [&](const ProofLayer &l) {
+    w.u64(l.fri_witness.size() / 4);
+    w.words(l.fri_witness.data(), l.fri_witness.size());
+    w.u64(l.hash_witness.size() / 8);
+    w.words(l.hash_witness.data(), l.hash_witness.size());
+    w.words(l.commitment, 8);
+  };
+  layer(p.first_layer);
+  w.u64(p.inner_layers.size());
+  for (const auto &l : p.inner_layers) layer(l);
+  // LinePoly { coeffs, log_size } (STWO/core/poly/line.rs:113-123)
+  w.u64(p.last_layer_poly.size() / 4);
+  w.words(p.last_layer_poly.data(), p.last_layer_poly.size());
+  w.u32(p.last_layer_log_size);
+}
+
+using Clock = std::chrono::steady_clock;
+static double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+// Everything the prove allocates on the device, freed on every exit path.
+struct ProveMem {
+  stwo_hip_ctx *ctx = nullptr;
+  uint32_t *mem = nullptr;
+  stwo_hip_fri_prover *fri = nullptr;
+  ~ProveMem() {
+    if (fri) (void)stwo_hip_fri_prover_destroy(ctx, fri);
+    if (mem) (void)stwo_hip_free(ctx, mem);
+  }
+};
+
+static int fetch_layer(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *fri, uint32_t k, ProofLayer &out) {
+  size_t nf = 0, nh = 0, npos = 0;
+  int rc = stwo_hip_fri_prover_layer_proof(ctx, fri, k, out.commitment, nullptr, 0, &nf, nullptr, 0, &nh, nullptr,
+                                           nullptr, 0, &npos);
+  if (rc) return rc;
+  out.fri_witness.assign(4 * nf, 0);
+  out.hash_witness.assign(8 * nh, 0);
+  return stwo_hip_fri_prover_layer_proof(ctx, fri, k, out.commitment, out.fri_witness.data(), nf, &nf,
+                                         out.hash_witness.data(), nh, &nh, nullptr, nullptr, 0, &npos);
+}
+
+// Lifted-tree decommitment of `cols` (all of log size L) at the query positions.
+static int decommit_tree(stwo_hip_ctx *ctx, const uint32_t *tree, uint32_t L, const std::vector<const uint32_t *> &cols,
+                         const std::vector<uint32_t> &queries, std::vector<std::vector<uint32_t>> &values,
+                         std::vector<uint32_t> &witness) {
+  const size_t nq = queries.size(), nc = cols.size();
+  std::vector<uint32_t> logs(nc, L), vals(nc * nq);
+  const size_t cap = nq * (L + 1) + 1;
+  witness.assign(8 * cap, 0);
+  size_t nw = 0, naux = 0;
+  int rc = stwo_hip_merkle_lifted_decommit(ctx, tree, L, cols.data(), logs.data(), nc, queries.data(), nq,
+                                           vals.data(), witness.data(), cap, &nw, nullptr, 0, &naux);
+  if (rc) return rc;
+  witness.resize(8 * nw);
+  values.assign(nc, {});
+  for (size_t c = 0; c < nc; c++) values[c].assign(vals.begin() + c * nq, vals.begin() + (c + 1) * nq);
+  return STWO_HIP_OK;
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols,
+                            uint32_t log_n, stwo_hip_pcs_config config, const stwo_hip_twiddles *tw,
+                            uint8_t channel_digest[32], uint32_t *channel_n_draws, stwo_hip_stark_proof **out) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, a && b && tw && channel_digest && channel_n_draws && out, "wide_fib_prove: NULL argument");
+  *out = nullptr;
+  const uint32_t B = config.fri.log_blowup_factor;
+  STWO_HIP_CHECK_ARG(ctx, n_cols >= 3 && n_cols <= 4096, "wide_fib_prove: %u columns (3..4096)", n_cols);
+  STWO_HIP_CHECK_ARG(ctx, B >= 1 && log_n >= 4 && log_n + B <= 27,
+                     "wide_fib_prove: log_n %u with log blowup %u (log_n >= 4, blowup >= 1, log_n + blowup <= 27)",
+                     log_n, B);
+  STWO_HIP_CHECK_ARG(ctx, config.pow_bits <= 32, "wide_fib_prove: pow_bits %u > 32", config.pow_bits);
+  const uint32_t E = log_n + B;  // committed evaluation (and lifting) log size
+  const uint32_t C = log_n + 1;  // composition (constraint) domain log size
+  const size_t n = (size_t)1 << log_n, e = (size_t)1 << E, nc = (size_t)1 << C;
+  const size_t tree_words = tree_layer_off(E + 1);
+
+  // Device layout: trace coefficients | trace evaluations | trace tree |
+  // trace on CanonicCoset(C) (blowup > 1 only) | composition coordinates |
+  // composition-half evaluations | composition tree | quotient coordinates.
+  size_t off = 0;
+  const size_t o_coef = off;
+  off += n_cols * n;
+  const size_t o_eval = off;
+  off += n_cols * e;
+  const size_t o_tree1 = off;
+  off += tree_words;
+  const size_t o_ext = off;
+  if (B > 1) off += n_cols * nc;
+  const size_t o_comp = off;
+  off += 4 * nc;
+  const size_t o_ceval = off;
+  off += 8 * e;
+  const size_t o_tree2 = off;
+  off += tree_words;
+  const size_t o_quot = off;
+  off += 4 * e;
+
+  ProveMem pm;
+  pm.ctx = ctx;
+  int rc = stwo_hip_alloc(ctx, off, &pm.mem);
+  if (rc) return rc;
+  uint32_t *mem = pm.mem;
+  std::vector<uint32_t *> coef(n_cols), eval(n_cols), ext(n_cols);
+  for (uint32_t k = 0; k < n_cols; k++) {
+    coef[k] = mem + o_coef + k * n;
+    eval[k] = mem + o_eval + k * e;
+    ext[k] = B > 1 ? mem + o_ext + k * nc : eval[k];
+  }
+  uint32_t *comp[4], *quot[4];
+  std::vector<uint32_t *> ceval(8);
+  for (int k = 0; k < 4; k++) {
+    comp[k] = mem + o_comp + k * nc;
+    quot[k] = mem + o_quot + k * e;
+  }
+  for (int k = 0; k < 8; k++) ceval[k] = mem + o_ceval + k * e;
+  uint32_t *tree1 = mem + o_tree1, *tree2 = mem + o_tree2;
+
+  auto proof = std::make_unique<stwo_hip_stark_proof>();
+  proof->config = config;
+  HostChannel ch;
+  memcpy(ch.digest, channel_digest, 32);
+  ch.n_draws = *channel_n_draws;
+
+  // 1. preprocessed tree: no columns.
+  auto t0 = Clock::now();
+  {
+    b2_host h;
+    std::array<uint32_t, 8> root;
+    h.finalize((uint8_t *)root.data());
+    proof->commitments.push_back(root);
+    ch.mix_root(root.data());
+  }
+
+  // 2. trace tree.
+  const std::vector<uint32_t> logs_n(n_cols, log_n), logs_e(n_cols + 8, E), logs_8(8, log_n);
+  rc = stwo_hip_wide_fib_trace(ctx, coef.data(), n_cols, log_n, a, b);
+  if (!rc) rc = stwo_hip_interpolate_columns(ctx, coef.data(), logs_n.data(), n_cols, tw);
+  if (!rc) rc = stwo_hip_evaluate_polynomials(ctx, coef.data(), logs_n.data(), n_cols, B, tw, eval.data());
+  if (!rc) rc = stwo_hip_merkle_lifted_commit(ctx, eval.data(), logs_e.data(), n_cols, E, tree1, STWO_HIP_HASH_BLAKE2S);
+  std::array<uint32_t, 8> root1;
+  if (!rc) rc = stwo_hip_download(ctx, root1.data(), tree1, 8);
+  if (rc) return rc;
+  proof->commitments.push_back(root1);
+  ch.mix_root(root1.data());
+  proof->phase_ms[0] = ms_since(t0);
+
+  // 3. composition polynomial.
+  t0 = Clock::now();
+  const qm31 comp_coeff = ch.draw_secure_felt();
+  if (B > 1) rc = stwo_hip_evaluate_polynomials(ctx, coef.data(), logs_n.data(), n_cols, 1, tw, ext.data());
+  if (!rc) rc = stwo_hip_memset_zero(ctx, comp[0], 4 * nc);
+  if (!rc) rc = stwo_hip_wide_fib_constraint_quotients(ctx, ext.data(), n_cols, log_n, comp_coeff.v, comp);
+  const std::vector<uint32_t> logs_c(4, C);
+  if (!rc) rc = stwo_hip_interpolate_columns(ctx, comp, logs_c.data(), 4, tw);
+  // split_at_mid: left halves (coefficients [0, n)), then right halves ([n, 2n)).
+  std::vector<const uint32_t *> halves(8);
+  for (int k = 0; k < 4; k++) {
+    halves[k] = comp[k];
+    halves[4 + k] = comp[k] + n;
+  }
+  if (!rc) rc = stwo_hip_evaluate_polynomials(ctx, halves.data(), logs_8.data(), 8, B, tw, ceval.data());
+  if (!rc) rc = stwo_hip_merkle_lifted_commit(ctx, ceval.data(), logs_e.data(), 8, E, tree2, STWO_HIP_HASH_BLAKE2S);
+  std::array<uint32_t, 8> root2;
+  if (!rc) rc = stwo_hip_download(ctx, root2.data(), tree2, 8);
+  if (rc) return rc;
+  proof->commitments.push_back(root2);
+  ch.mix_root(root2.data());
+  proof->phase_ms[1] = ms_since(t0);
+
+  // 4a. out-of-domain samples: every trace column and composition half at the OODS point.
+  t0 = Clock::now();
+  const SecurePoint oods = random_point(ch);
+  stwo_hip_secure_point sp;
+  memcpy(sp.x, oods.x.v, 16);
+  memcpy(sp.y, oods.y.v, 16);
+  std::vector<const uint32_t *> sampled(n_cols + 8);
+  std::vector<uint32_t> sample_logs(n_cols + 8, log_n);
+  for (uint32_t k = 0; k < n_cols; k++) sampled[k] = coef[k];
+  for (int k = 0; k < 8; k++) sampled[n_cols + k] = halves[k];
+  std::vector<uint32_t> values(4 * (n_cols + 8));
+  rc = stwo_hip_eval_at_point(ctx, sampled.data(), sample_logs.data(), n_cols + 8, &sp, values.data());
+  if (rc) return rc;
+  proof->sampled_values.resize(3);
+  proof->sampled_values[1].resize(n_cols);
+  proof->sampled_values[2].resize(8);
+  for (uint32_t k = 0; k < n_cols + 8; k++) {
+    auto &dst = k < n_cols ? proof->sampled_values[1][k] : proof->sampled_values[2][k - n_cols];
+    dst.assign(values.begin() + 4 * k, values.begin() + 4 * k + 4);
+  }
+  ch.mix_felts(values.data(), n_cols + 8);
+
+  // prove_ex's self-check: composition value from the halves against the
+  // constraints at the OODS point.
+  {
+    qm31 ev[8];
+    for (int k = 0; k < 8; k++)
+      ev[k] = qm31_make(values[4 * (n_cols + k)], values[4 * (n_cols + k) + 1], values[4 * (n_cols + k) + 2],
+                        values[4 * (n_cols + k) + 3]);
+    qm31 xr = oods.x;  // repeated_double(log_n - 1).x
+    for (uint32_t d = 1; d < log_n; d++) xr = q_double_x(xr);
+    const qm31 from_halves = qm31_add(from_partial_evals(ev), qm31_mul(xr, from_partial_evals(ev + 4)));
+    // coset_vanishing(CanonicCoset(log_n).coset, p): the shift is zero for a
+    // canonic coset, then log_n - 1 doublings of x.
+    const qm31 denom_inv = qm31_inv_host(xr);
+    qm31 acc = qm31_make(0, 0, 0, 0);
+    auto tv = [&](uint32_t k) {
+      return qm31_make(values[4 * k], values[4 * k + 1], values[4 * k + 2], values[4 * k + 3]);
+    };
+    qm31 va = tv(0), vb = tv(1);
+    for (uint32_t k = 2; k < n_cols; k++) {
+      const qm31 vc = tv(k);
+      const qm31 con = qm31_sub(vc, qm31_add(qm31_mul(va, va), qm31_mul(vb, vb)));
+      acc = qm31_add(qm31_mul(acc, comp_coeff), qm31_mul(denom_inv, con));
+      va = vb;
+      vb = vc;
+    }
+    if (memcmp(acc.v, from_halves.v, 16) != 0)
+      return set_error(ctx, STWO_HIP_ECONSTRAINTS,
+                       "wide_fib_prove: constraints not satisfied (the composition value at the OODS point does "
+                       "not match the trace samples)");
+  }
+  proof->phase_ms[2] = ms_since(t0);
+
+  // 4b. FRI quotients over the flattened trees (trace columns, then composition halves).
+  t0 = Clock::now();
+  const qm31 quot_coeff = ch.draw_secure_felt();
+  std::vector<const uint32_t *> qcols(n_cols + 8);
+  for (uint32_t k = 0; k < n_cols; k++) qcols[k] = eval[k];
+  for (int k = 0; k < 8; k++) qcols[n_cols + k] = ceval[k];
+  std::vector<stwo_hip_point_sample> samples(n_cols + 8);
+  std::vector<uint32_t> n_samples(n_cols + 8, 1);
+  for (uint32_t k = 0; k < n_cols + 8; k++) {
+    samples[k].point = sp;
+    memcpy(samples[k].value, &values[4 * k], 16);
+  }
+  rc = stwo_hip_compute_fri_quotients(ctx, qcols.data(), logs_e.data(), n_cols + 8, samples.data(), n_samples.data(),
+                                      quot_coeff.v, E, quot);
+  if (!rc) rc = stwo_hip_sync(ctx);
+  if (rc) return rc;
+  proof->phase_ms[3] = ms_since(t0);
+
+  // 4c. FRI commit.
+  t0 = Clock::now();
+  std::vector<uint32_t> roots(8 * (E + 1));
+  uint32_t n_roots = 0;
+  std::vector<uint32_t> last(4u << config.fri.log_last_layer_degree_bound);
+  rc = stwo_hip_fri_prover_commit(ctx, ch.digest, &ch.n_draws, config.fri, quot, E, tw, roots.data(), &n_roots,
+                                  last.data(), &pm.fri);
+  if (rc) return rc;
+  proof->last_layer_poly = last;
+  proof->last_layer_log_size = config.fri.log_last_layer_degree_bound;
+  proof->phase_ms[4] = ms_since(t0);
+
+  // 4d. proof of work.
+  t0 = Clock::now();
+  rc = stwo_hip_grind(ctx, ch.digest, config.pow_bits, &proof->proof_of_work);
+  if (rc) return rc;
+  ch.mix_u64(proof->proof_of_work);
+  proof->phase_ms[5] = ms_since(t0);
+
+  // 4e. FRI decommitment, then the trees at the query positions.
+  t0 = Clock::now();
+  std::vector<uint32_t> qpos(config.fri.n_queries), unsorted(config.fri.n_queries);
+  uint32_t npos = 0;
+  rc = stwo_hip_fri_prover_decommit(ctx, pm.fri, ch.digest, &ch.n_draws, qpos.data(), &npos, unsorted.data());
+  if (rc) return rc;
+  qpos.resize(npos);
+  proof->query_positions = qpos;
+  rc = fetch_layer(ctx, pm.fri, 0, proof->first_layer);
+  if (rc) return rc;
+  proof->inner_layers.resize(n_roots - 1);
+  for (uint32_t k = 1; k < n_roots; k++) {
+    rc = fetch_layer(ctx, pm.fri, k, proof->inner_layers[k - 1]);
+    if (rc) return rc;
+  }
+  proof->decommitments.resize(3);
+  proof->queried_values.resize(3);
+  {
+    std::vector<const uint32_t *> c1(eval.begin(), eval.end()), c2(ceval.begin(), ceval.end());
+    rc = decommit_tree(ctx, tree1, E, c1, qpos, proof->queried_values[1], proof->decommitments[1]);
+    if (!rc) rc = decommit_tree(ctx, tree2, E, c2, qpos, proof->queried_values[2], proof->decommitments[2]);
+    if (rc) return rc;
+  }
+  proof->phase_ms[6] = ms_since(t0);
+
+  memcpy(channel_digest, ch.digest, 32);
+  *channel_n_draws = ch.n_draws;
+  *out = proof.release();
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_stark_proof_bytes(const stwo_hip_stark_proof *proof, uint8_t *buf, size_t cap, size_t *len) {
+  if (!proof || !len) return STWO_HIP_EINVAL;
+  ByteWriter w;
+  write_proof(*proof, w);
+  *len = w.b.size();
+  if (buf) {
+    if (cap < w.b.size()) return STWO_HIP_EINVAL;
+    memcpy(buf, w.b.data(), w.b.size());
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_stark_proof_info(const stwo_hip_stark_proof *proof, double phase_ms[STWO_HIP_PROVE_PHASES],
+                              uint32_t *query_positions, size_t cap, size_t *n_positions) {
+  if (!proof) return STWO_HIP_EINVAL;
+  if (phase_ms) memcpy(phase_ms, proof->phase_ms, sizeof proof->phase_ms);
+  if (n_positions) *n_positions = proof->query_positions.size();
+  if (query_positions) {
+    if (cap < proof->query_positions.size()) return STWO_HIP_EINVAL;
+    memcpy(query_positions, proof->query_positions.data(), 4 * proof->query_positions.size());
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_stark_proof_destroy(stwo_hip_stark_proof *proof) {
+  delete proof;
+  return STWO_HIP_OK;
+}
+
+}  // extern "C"

@@ -42,6 +42,19 @@ class FriConfig(ctypes.Structure):
                 ("n_queries", ctypes.c_uint32), ("line_fold_step", ctypes.c_uint32)]
 
 
+class PcsConfig(ctypes.Structure):
+    """PcsConfig (lifting_log_size None); the default is PcsConfig::default()."""
+    _fields_ = [("pow_bits", ctypes.c_uint32), ("fri", FriConfig)]
+
+    @staticmethod
+    def default():
+        return PcsConfig(10, FriConfig(1, 0, 3, 1))
+
+
+PROVE_PHASES = ("trace_commit", "composition_commit", "oods_samples", "fri_quotients", "fri_commit",
+                "proof_of_work", "decommit")
+
+
 class JobDesc(ctypes.Structure):
     _fields_ = [("n_cols", ctypes.c_uint32), ("log_size", ctypes.c_uint32), ("log_blowup", ctypes.c_uint32),
                 ("pad", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("fri", FriConfig)]
@@ -152,6 +165,12 @@ def lib():
             "stwo_hip_wide_fib_constraint_quotients": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
                                                        _u32p, ctypes.POINTER(_vp)],
             "stwo_hip_grind": [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
+            "stwo_hip_wide_fib_prove": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, PcsConfig, _vp,
+                                        ctypes.c_char_p, _u32p, ctypes.POINTER(_vp)],
+            "stwo_hip_stark_proof_bytes": [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_stark_proof_info": [_vp, ctypes.POINTER(ctypes.c_double), _u32p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_stark_proof_destroy": [_vp],
             "stwo_hip_debug_fft_plan": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.c_uint32,
                                         _u32p],
         }
@@ -444,6 +463,34 @@ class Context:
         n = ctypes.c_uint64()
         self._check(lib().stwo_hip_grind(self.h, bytes(digest), pow_bits, ctypes.byref(n)))
         return n.value
+
+    def wide_fib_prove(self, a, b, n_cols, log_n, tw, config=None, digest=bytes(32), n_draws=0):
+        """The wide-Fibonacci prove (empty preprocessed tree, trace tree, prove)
+        from the device inputs a, b (2^log_n words each).  Returns (proof
+        bytes, phase times in ms by name, sorted query positions, channel
+        digest, channel n_draws)."""
+        cfg = config if config is not None else PcsConfig.default()
+        d = ctypes.create_string_buffer(bytes(digest), 32)
+        nd = np.array([n_draws], dtype=np.uint32)
+        h = _vp()
+        self._check(lib().stwo_hip_wide_fib_prove(self.h, a.ptr if isinstance(a, DeviceBuf) else a,
+                                                  b.ptr if isinstance(b, DeviceBuf) else b, n_cols, log_n, cfg,
+                                                  tw.h, d, nd.ctypes.data_as(_u32p), ctypes.byref(h)))
+        try:
+            n = ctypes.c_size_t()
+            self._check(lib().stwo_hip_stark_proof_bytes(h, None, 0, ctypes.byref(n)))
+            buf = ctypes.create_string_buffer(n.value)
+            self._check(lib().stwo_hip_stark_proof_bytes(h, buf, n.value, ctypes.byref(n)))
+            ms = (ctypes.c_double * len(PROVE_PHASES))()
+            npos = ctypes.c_size_t()
+            self._check(lib().stwo_hip_stark_proof_info(h, ms, None, 0, ctypes.byref(npos)))
+            pos = np.zeros(max(1, npos.value), dtype=np.uint32)
+            self._check(lib().stwo_hip_stark_proof_info(h, ms, pos.ctypes.data_as(_u32p), npos.value,
+                                                        ctypes.byref(npos)))
+        finally:
+            lib().stwo_hip_stark_proof_destroy(h)
+        return (buf.raw[: n.value], dict(zip(PROVE_PHASES, list(ms))), [int(x) for x in pos[: npos.value]],
+                d.raw[:32], int(nd[0]))
 
     def merkle_lifted_decommit(self, layers, lifting_log, cols, log_sizes, queries, with_aux=False):
         """MerkleProverLifted::decommit on a packed lifted tree; returns
