@@ -239,6 +239,66 @@ ROCPROF_NAMES = {
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_lde_merkle_256x2e20_b4.json")
 
 
+PROVE_LOG_N, PROVE_COLS = 18, 256
+
+
+def prove_wide_fib(ctx, runs=3, cpu=True):
+    """The end-to-end wide-Fibonacci prove: 256 columns x 2^18 rows, the first
+    two columns seeded random, PcsConfig::default() (blowup 2, 3 queries,
+    10 PoW bits), Blake2s channel -- trace generation, trace commit,
+    composition, OODS samples, FRI quotients, FRI commit, grind and
+    decommitment, with the inputs resident on the device.  Median of `runs`
+    proves after one warm-up; one more prove with per-kernel device times.
+    The CPU leg runs the oracle's prove of the same inputs on the host."""
+    n = 1 << PROVE_LOG_N
+    a, b = ctx.alloc(n), ctx.alloc(n)
+    ctx.fill_random(a, n, 4, 0)
+    ctx.fill_random(b, n, 4, 1)
+    cfg = sh.PcsConfig.default()
+    tw = ctx.precompute_twiddles(sh.half_odds(PROVE_LOG_N + cfg.fri.log_blowup_factor - 1))
+    ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg)
+    times, phases, data = [], [], None
+    for _ in range(runs):
+        ctx.sync()
+        t0 = time.perf_counter()
+        data, ph, _, _, _ = ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg)
+        times.append((time.perf_counter() - t0) * 1e3)
+        phases.append(ph)
+    order = sorted(range(runs), key=lambda i: times[i])
+    mid = order[runs // 2]
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg)
+    prof = ctx.prof_read()
+    ctx.prof_enable(False)
+    res = {
+        "workload": "wide-Fibonacci prove: 256 columns x 2^18 rows (first two seeded random), PcsConfig::default() "
+                    "(log blowup 1, 3 queries, 10 PoW bits, fold step 1), Blake2s channel, bincode proof",
+        "ms": round(times[mid], 3),
+        "runs_ms": [round(t, 3) for t in times],
+        "phases_ms": {k: round(v, 3) for k, v in phases[mid].items()},
+        "proof_bytes": len(data),
+        "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+    }
+    a.free()
+    b.free()
+    tw.destroy()
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+        import prove as oracle_prove
+        ha, hb = oracle.fill_random(n, 4, 0), oracle.fill_random(n, 4, 1)
+        ct = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            want = oracle_prove.wide_fib_prove(ha, hb, PROVE_COLS, PROVE_LOG_N)
+            ct.append((time.perf_counter() - t0) * 1e3)
+        res["cpu_oracle_ms"] = round(min(ct), 1)
+        res["cpu_oracle_cores"] = oracle.num_threads()
+        res["proof_equal_to_oracle"] = oracle_prove.proof_bytes(want) == data
+    return res
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
@@ -288,6 +348,7 @@ def main():
     ap.add_argument("--no-fri", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-prove", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
@@ -360,6 +421,11 @@ def main():
         batched = batched_vs_plain(ctx)
         log("batched vs plain: %s" % json.dumps(batched))
 
+    prove = None
+    if not args.no_prove:
+        prove = prove_wide_fib(ctx, cpu=not args.no_cpu)
+        log("prove: %s" % json.dumps(prove))
+
     fri_ms, fri_layers = (None, None)
     if not args.no_fri:
         fri_ms, fri_layers = fri_chain(ctx, tw)
@@ -416,6 +482,7 @@ def main():
         "kernels": kernels,
         "batched_uneven": batched,
         "serving": serve,
+        "prove_wide_fib": prove,
         "fri_chain_ms": round(fri_ms, 3) if fri_ms is not None else None,
         "fri_committed_layers": fri_layers,
     }
