@@ -98,9 +98,11 @@ int stwo_hip_fill_random(stwo_hip_ctx *ctx, uint32_t *dst, size_t n, uint64_t se
  * PolyOps::precompute_twiddles(coset) (STWO/prover/poly/circle/ops.rs:144;
  * CpuBackend layout CPUB/circle.rs:294-311): for each doubling of the root
  * coset, the x-coordinates of its first half in bit-reversed order, then a
- * pad of 1; plus the elementwise inverses.  Built on the device and kept
- * resident (replaces MB/poly.rs:185-224 and SH/runtime.m:1026).  A tree may
- * be shared by every context on its device. */
+ * pad of 1; plus the elementwise inverses, and both trees doubled (2t as a
+ * u32, the FFT's multiplication operands, as SimdBackend keeps them): 16 bytes
+ * per root point.  Built on the device and kept resident (replaces
+ * MB/poly.rs:185-224 and SH/runtime.m:1026).  A tree may be shared by every
+ * context on its device. */
 int stwo_hip_twiddles_create(stwo_hip_ctx *ctx, stwo_hip_coset root, stwo_hip_twiddles **out);
 int stwo_hip_twiddles_destroy(stwo_hip_ctx *ctx, stwo_hip_twiddles *tw);
 /* Device pointers and length (= root size) of the tree and its inverse. */

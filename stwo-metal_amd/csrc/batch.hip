@@ -83,7 +83,8 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   uint32_t rl = tw->root.log_size;
   STWO_HIP_CHECK_ARG(ctx, max_log - 1 <= rl && tw->root.initial_index == half_odds_initial(rl),
                      "batch_create: twiddle root must be half_odds(r) with r >= %u", max_log - 1);
-  const uint32_t *itw_end = tw->d_itw + tw->len, *tw_end = tw->d_tw + tw->len;
+  // The radix-16 passes take the doubled trees.
+  const uint32_t *itw_end = tw->d_itw2 + tw->len, *tw_end = tw->d_tw2 + tw->len;
 
   // Groups by log size, largest first (their workgroups start first).
   std::map<uint32_t, std::vector<size_t>, std::greater<uint32_t>> groups;

@@ -20,6 +20,8 @@ struct stwo_hip_twiddles {
   size_t len;          // root size (2^root.log_size)
   uint32_t *d_tw;      // twiddle tree, CpuBackend layout
   uint32_t *d_itw;     // elementwise inverses
+  uint32_t *d_tw2;     // both trees doubled (2 * t < 2^32), the radix-16 FFT's operands
+  uint32_t *d_itw2;
 };
 
 namespace stwo_hip {

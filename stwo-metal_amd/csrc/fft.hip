@@ -229,14 +229,15 @@ __global__ void __launch_bounds__(256) bit_reverse_tiled_kernel(uint32_t *v, uin
 // stage, loaded once.  The circle layer (s = 0) reads 4 words (two (x, y)
 // pairs of line layer 0) for its 8 pairs.
 
+// `t2` is the twiddle doubled (the passes read the doubled trees).
 template <bool INV>
-__device__ __forceinline__ void bfly(uint32_t &a, uint32_t &b, uint32_t t) {
+__device__ __forceinline__ void bfly(uint32_t &a, uint32_t &b, uint32_t t2) {
   if (INV) {
     uint32_t s = m31_add(a, b);
-    b = m31_mul(m31_sub(a, b), t);
+    b = m31_mul_dbl(m31_sub(a, b), t2);
     a = s;
   } else {
-    uint32_t tmp = m31_mul(b, t);
+    uint32_t tmp = m31_mul_dbl(b, t2);
     b = m31_sub(a, tmp);
     a = m31_add(a, tmp);
   }
@@ -277,7 +278,8 @@ __device__ __forceinline__ void load_level(uint32_t *dst, int cnt, const uint32_
     for (int u = 0; u < 8; u++) {
       uint32_t xx = w4[2 * (u >> 2)], yy = w4[2 * (u >> 2) + 1];
       uint32_t v = (u & 2) ? xx : yy;
-      dst[u] = ((u & 3) == 1 || (u & 3) == 2) ? m31_neg(v) : v;
+      // doubled negation: 2 (P - v / 2) = 2P - v < 2^32 (for v = 0 it is 2P, still 0 mod P)
+      dst[u] = ((u & 3) == 1 || (u & 3) == 2) ? 2u * P - v : v;
     }
     return;
   }
@@ -836,10 +838,10 @@ static int run_fft_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, uin
     }
     return launch(ctx, "fft_small", fft_small_kernel, dim3((uint32_t)cols.size()), dim3(64), 0, a);
   }
+  if (N >= 4) return run_fft_group2(ctx, cols, dtab, n, N, (inverse ? tw->d_itw2 : tw->d_tw2) + tw->len, inverse);
   const uint32_t *tree = inverse ? tw->d_itw : tw->d_tw;
   const uint32_t *tw_end = tree + tw->len;
   uint32_t L = inverse ? N : n;
-  if (N >= 4) return run_fft_group2(ctx, cols, dtab, n, N, tw_end, inverse);
   std::vector<PassPlan> plan = plan_passes(N, L);
   if (inverse) std::reverse(plan.begin(), plan.end());
   // Output-side table (in place after the first forward pass).

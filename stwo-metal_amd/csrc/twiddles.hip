@@ -27,14 +27,18 @@ __device__ __forceinline__ cpoint dev_index_to_point(uint32_t idx) {
   return r;
 }
 
-__global__ void __launch_bounds__(256) twiddle_tree_kernel(uint32_t *tw, uint32_t *itw, uint32_t root_initial,
-                                                           uint32_t r) {
+// Also writes both trees doubled (tw2 = 2 tw, itw2 = 2 itw as u32), the
+// operands of the radix-16 FFT's multiplications.
+__global__ void __launch_bounds__(256) twiddle_tree_kernel(uint32_t *tw, uint32_t *itw, uint32_t *tw2, uint32_t *itw2,
+                                                           uint32_t root_initial, uint32_t r) {
   uint32_t len = 1u << r;
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= len) return;
   if (k == len - 1) {
     tw[k] = 1;
     itw[k] = 1;
+    tw2[k] = 2;
+    itw2[k] = 2;
     return;
   }
   uint32_t d = len - k;                  // >= 2
@@ -44,8 +48,11 @@ __global__ void __launch_bounds__(256) twiddle_tree_kernel(uint32_t *tw, uint32_
   uint32_t initial = (uint32_t)(((uint64_t)root_initial << level) & IDX_MASK);
   uint32_t idx = (initial + bitrev32(j, cl - 1) * coset_step(cl)) & IDX_MASK;
   uint32_t x = dev_index_to_point(idx).x;
+  uint32_t xi = m31_inv(x);
   tw[k] = x;
-  itw[k] = m31_inv(x);
+  itw[k] = xi;
+  tw2[k] = x << 1;
+  itw2[k] = xi << 1;
 }
 
 static int upload_gen_table(stwo_hip_ctx *ctx) {
@@ -77,20 +84,21 @@ int stwo_hip_twiddles_create(stwo_hip_ctx *ctx, stwo_hip_coset root, stwo_hip_tw
   tw->device = ctx->device;
   tw->root = {root.initial_index & IDX_MASK, root.log_size};
   tw->len = (size_t)1 << root.log_size;
-  if (hipMalloc((void **)&tw->d_tw, tw->len * 4) != hipSuccess ||
-      hipMalloc((void **)&tw->d_itw, tw->len * 4) != hipSuccess) {
+  // One allocation: tree | inverse tree | both doubled.
+  if (hipMalloc((void **)&tw->d_tw, tw->len * 16) != hipSuccess) {
     hipGetLastError();
-    if (tw->d_tw) hipFree(tw->d_tw);
     delete tw;
     return set_error(ctx, STWO_HIP_ENOMEM, "twiddle tree of %zu entries: out of device memory", (size_t)1 << root.log_size);
   }
+  tw->d_itw = tw->d_tw + tw->len;
+  tw->d_tw2 = tw->d_tw + 2 * tw->len;
+  tw->d_itw2 = tw->d_tw + 3 * tw->len;
   rc = launch(ctx, "twiddle_tree", twiddle_tree_kernel, dim3((uint32_t)((tw->len + 255) / 256)), dim3(256), 0,
-              tw->d_tw, tw->d_itw, tw->root.initial_index, root.log_size);
+              tw->d_tw, tw->d_itw, tw->d_tw2, tw->d_itw2, tw->root.initial_index, root.log_size);
   if (rc == STWO_HIP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
     rc = set_error(ctx, STWO_HIP_EDEVICE, "twiddle tree build failed");
   if (rc) {
-    hipFree(tw->d_tw);
-    hipFree(tw->d_itw);
+    (void)hipFree(tw->d_tw);
     delete tw;
     return rc;
   }
@@ -101,8 +109,7 @@ int stwo_hip_twiddles_create(stwo_hip_ctx *ctx, stwo_hip_coset root, stwo_hip_tw
 int stwo_hip_twiddles_destroy(stwo_hip_ctx *ctx, stwo_hip_twiddles *tw) {
   if (!ctx || !tw) return STWO_HIP_EINVAL;
   STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  hipFree(tw->d_tw);
-  hipFree(tw->d_itw);
+  (void)hipFree(tw->d_tw);
   delete tw;
   return STWO_HIP_OK;
 }
