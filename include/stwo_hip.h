@@ -337,7 +337,10 @@ int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *co
  * CanonicCoset(log_n + log_blowup_factor).  channel_digest / channel_n_draws
  * are the channel's state, read and updated (zeros = Blake2sChannel::default()).
  * Returns STWO_HIP_ECONSTRAINTS when prove_ex's own OODS self-check fails (a
- * trace that does not satisfy the AIR).  Synchronizes.
+ * trace that does not satisfy the AIR).  Synchronizes.  The prove's device
+ * buffers (4 + 4 * 2^log_blowup_factor bytes per trace cell, plus the trees
+ * and the composition and FRI columns) are kept by the context and
+ * reused by its next prove; stwo_hip_ctx_destroy releases them.
  *
  * stwo_hip_stark_proof_bytes writes the StarkProof<Blake2sMerkleHasher> in the
  * layout bincode 1's default options give the reference's serde derives

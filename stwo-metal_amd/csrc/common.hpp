@@ -61,6 +61,11 @@ struct stwo_hip_ctx {
   std::vector<stwo_hip::ProfPending> prof_pending;
   std::map<std::string, std::pair<uint64_t, double>> prof_totals;
 
+  // Pinned staging for small synchronous copies (a pageable hipMemcpy costs
+  // tens of microseconds more per call).
+  uint8_t *h_stage = nullptr;
+  size_t stage_cap = 0;
+
   // Scratch device buffers kept across calls, keyed by purpose.
   std::map<std::string, std::pair<void *, size_t>> scratch;
 };
@@ -89,6 +94,14 @@ int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr);
 // Returns a device scratch buffer of at least `bytes` for `key` (grown on
 // demand; growing synchronizes the stream before freeing the old buffer).
 int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr);
+// Several device-to-host reads with one stream synchronization (through the
+// pinned staging buffer when they fit).
+struct DownloadReq {
+  uint32_t *dst;
+  const uint32_t *src;
+  size_t n_words;
+};
+int download_many(stwo_hip_ctx *ctx, const DownloadReq *reqs, size_t n_reqs);
 
 hipEvent_t take_event(stwo_hip_ctx *ctx);
 void prof_begin(stwo_hip_ctx *ctx, const char *name, hipEvent_t *start);

@@ -5,13 +5,16 @@
 //
 // For coefficients c_0..c_{2^n - 1} (natural FFT-basis order) and the point
 // (x, y), eval_at_point = fold(c, [pi^(n-2)(x), ..., pi(x), x, y])
-// (STWO/core/poly/utils.rs:27-38), i.e. sum_j c_j * prod_{bit k of j} g_k with
-// g_0 = y, g_1 = x, g_k = pi^(k-1)(x), pi(x) = 2x^2 - 1.  A workgroup folds a
-// block of 2^B consecutive coefficients through LDS with g_0..g_{B-1} (the
-// fold's bottom levels) and multiplies the result by the product of g_k over
-// the set bits of the block index (the top levels); a second kernel sums the
-// blocks of each column.  Sums of field elements are exact, so the grouping
-// does not change the result.
+// (STWO/core/poly/utils.rs:27-38), i.e. sum_j c_j * basis_j with
+// basis_j = prod of g_k over the set bits k of j, g_0 = y, g_1 = x,
+// g_k = pi^(k-1)(x), pi(x) = 2x^2 - 1.  The basis does not depend on the
+// column, so one table of the low 2^11 basis values serves every column: a
+// workgroup takes one 2^11-coefficient block of up to 8 columns of the same
+// size, forms the dot products with the table (64-bit accumulation, one
+// multiply-add per limb and coefficient), and multiplies by the product of
+// g_k over the block index's bits; a second kernel sums each column's
+// blocks.  Sums of field elements are exact, so the grouping does not change
+// the result.
 #include <string.h>
 
 #include <algorithm>
@@ -22,7 +25,7 @@
 
 namespace stwo_hip {
 
-constexpr uint32_t kEvalBlockLog = 11;  // 2048 coefficients per workgroup (8 KiB of LDS + 16 KiB of QM31)
+constexpr uint32_t kEvalBlockLog = 11;  // 2048 coefficients per workgroup and block
 
 struct EvalCol {
   const uint32_t *coeffs;
@@ -37,56 +40,85 @@ __device__ __forceinline__ qm31 ld_factor(const EvalFactors &f, uint32_t k) {
   return qm31_make(f.g[k][0], f.g[k][1], f.g[k][2], f.g[k][3]);
 }
 
-// grid.x = total blocks over all columns; block `b` belongs to the column
-// whose [block_begin, next block_begin) holds it.
-__global__ void __launch_bounds__(256) eval_point_blocks_kernel(const EvalCol *cols, uint32_t n_cols,
-                                                                EvalFactors f, uint32_t *partials) {
-  __shared__ uint32_t sc[1u << kEvalBlockLog];
-  __shared__ qm31 lv[1u << (kEvalBlockLog - 1)];
-  // the last column whose first block is <= blockIdx.x (every column has >= 1 block)
-  uint32_t lo = 0, hi = n_cols - 1;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi + 1) >> 1;
-    if (cols[mid].block_begin <= blockIdx.x) lo = mid;
-    else hi = mid - 1;
-  }
-  const uint32_t ci = lo;
-  const EvalCol col = cols[ci];
-  const uint32_t blk = blockIdx.x - col.block_begin;
-  const uint32_t B = col.log_size < kEvalBlockLog ? col.log_size : kEvalBlockLog;
+// Basis values of the low kEvalBlockLog bits: basis[t] = prod of g_k over the
+// set bits k of t (k < kEvalBlockLog), 4 words each.  The basis of index
+// j = (b << B) + t is basis[t] times the product of g_k over the set bits of
+// b at positions k >= B, so one table serves every column and block.
+__global__ void __launch_bounds__(256) eval_basis_kernel(EvalFactors f, uint32_t *basis, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  qm31 r = qm31_make(1, 0, 0, 0);
+  for (uint32_t k = 0; (t >> k) != 0; k++)
+    if ((t >> k) & 1) r = qm31_mul(r, ld_factor(f, k));
+  for (int w = 0; w < 4; w++) basis[4 * (size_t)t + w] = r.v[w];
+}
+
+// One workgroup: block `block` (2^B coefficients, B = min(log, kEvalBlockLog))
+// of `n` <= kDotCols consecutive columns of the same log size.  Lane t holds
+// the basis values of coefficients 8t .. 8t + 7 in registers and, per
+// column, accumulates sum c_j * basis_j limb by limb in 64 bits
+// (v_mad_u64_u32; each product is below 2^62, so three fit before a fold
+// (x & P) + (x >> 31), which keeps the sum below 2^34).  The 256 lanes' sums
+// are reduced through LDS, multiplied by the block's factor and written as
+// the column's partial for the block.
+constexpr uint32_t kDotCols = 8;
+struct EvalWork {
+  uint32_t col0, n, block, log_size;
+};
+
+
+__global__ void __launch_bounds__(256) eval_point_dot_kernel(const EvalCol *cols, const EvalWork *work, EvalFactors f,
+                                                             const uint32_t *basis, uint32_t *partials) {
+  __shared__ uint32_t red[kDotCols * 4][257];
+  __shared__ uint32_t sums[kDotCols * 4];
+  const EvalWork w = work[blockIdx.x];
+  const uint32_t B = w.log_size < kEvalBlockLog ? w.log_size : kEvalBlockLog;
   const uint32_t nb = 1u << B;
-  for (uint32_t t = threadIdx.x; t < nb; t += blockDim.x) sc[t] = as_global(col.coeffs)[((size_t)blk << B) + t];
-  __syncthreads();
-  qm31 res;
-  if (B == 0) {
-    res = qm31_make(sc[0], 0, 0, 0);
-  } else {
-    // bottom level: c[2t] + c[2t + 1] * g_0
-    const qm31 g0 = ld_factor(f, 0);
-    for (uint32_t t = threadIdx.x; t < (nb >> 1); t += blockDim.x) {
-      qm31 v = qm31_mul_m31(g0, sc[2 * t + 1]);
-      v.v[0] = m31_add(v.v[0], sc[2 * t]);
-      lv[t] = v;
-    }
-    for (uint32_t k = 1; k < B; k++) {
-      __syncthreads();
-      const uint32_t n = nb >> (k + 1);
-      const qm31 g = ld_factor(f, k);
-      qm31 tmp[4];
-      uint32_t cnt = 0;
-      for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) tmp[cnt++] = qm31_add(lv[2 * t], qm31_mul(lv[2 * t + 1], g));
-      __syncthreads();
-      cnt = 0;
-      for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) lv[t] = tmp[cnt++];
-    }
-    __syncthreads();
-    res = lv[0];
+  const uint32_t t0 = threadIdx.x * 8;
+  qm31 b[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const uint32_t t = t0 + i;
+    b[i] = t < nb ? qm31_make(basis[4 * t], basis[4 * t + 1], basis[4 * t + 2], basis[4 * t + 3])
+                  : qm31_make(0, 0, 0, 0);
   }
-  if (threadIdx.x == 0) {
-    // top levels: the block index's set bits select g_B, g_{B+1}, ...
-    for (uint32_t k = B; k < col.log_size; k++)
-      if (blk >> (k - B) & 1) res = qm31_mul(res, ld_factor(f, k));
-    for (int w = 0; w < 4; w++) partials[4 * (size_t)blockIdx.x + w] = res.v[w];
+  for (uint32_t j = 0; j < w.n; j++) {
+    const uint32_t *c = cols[w.col0 + j].coeffs + ((size_t)w.block << B);
+    uint64_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint32_t v = (t0 + i < nb) ? as_global(c)[t0 + i] : 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[k] += (uint64_t)v * b[i].v[k];
+      if (i == 2 || i == 5)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) red[4 * j + k][threadIdx.x] = m31_reduce64(acc[k]);
+  }
+  __syncthreads();
+  // Row r (column r / 4, limb r % 4): 8 lanes sum 32 entries each, then
+  // combine across the 8 lanes (consecutive lanes of one wave).
+  const uint32_t r = threadIdx.x >> 3, seg = threadIdx.x & 7;
+  if (r < 4 * w.n) {
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 32; i++) s += red[r][seg * 32 + i];
+    uint32_t v = m31_reduce64(s);
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) v = m31_add(v, __shfl_xor(v, o, 8));
+    if (seg == 0) sums[r] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < w.n) {
+    const uint32_t j = threadIdx.x;
+    qm31 res = qm31_make(sums[4 * j], sums[4 * j + 1], sums[4 * j + 2], sums[4 * j + 3]);
+    // the block index's set bits select g_B, g_{B+1}, ...
+    for (uint32_t k = B; k < w.log_size; k++)
+      if ((w.block >> (k - B)) & 1) res = qm31_mul(res, ld_factor(f, k));
+    const size_t o = 4 * ((size_t)cols[w.col0 + j].block_begin + w.block);
+    for (int q = 0; q < 4; q++) partials[o + q] = res.v[q];
   }
 }
 
@@ -146,15 +178,33 @@ int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, con
     qm31 sq = qm31_mul(x, x);
     x = qm31_sub(qm31_add(sq, sq), qm31_make(1, 0, 0, 0));
   }
-  void *d_tab = nullptr, *d_part = nullptr, *d_out = nullptr;
+  // Work items: runs of up to kDotCols consecutive columns of one log size, per block.
+  std::vector<EvalWork> work;
+  for (size_t c = 0; c < n_cols;) {
+    size_t e = c + 1;
+    while (e < n_cols && e - c < kDotCols && log_sizes[e] == log_sizes[c]) e++;
+    const uint32_t lg = log_sizes[c];
+    const uint32_t nblk = 1u << (lg > kEvalBlockLog ? lg - kEvalBlockLog : 0);
+    for (uint32_t k = 0; k < nblk; k++) work.push_back({(uint32_t)c, (uint32_t)(e - c), k, lg});
+    c = e;
+  }
+  const uint32_t n_basis = 1u << std::min(max_log, kEvalBlockLog);
+  void *d_tab = nullptr, *d_work = nullptr, *d_basis = nullptr, *d_part = nullptr, *d_out = nullptr;
   int rc = upload_table(ctx, tab.data(), tab.size() * sizeof(EvalCol), &d_tab);
+  if (rc) return rc;
+  rc = upload_table(ctx, work.data(), work.size() * sizeof(EvalWork), &d_work);
+  if (rc) return rc;
+  rc = get_scratch(ctx, "eval_point_basis", (size_t)n_basis * 16, &d_basis);
   if (rc) return rc;
   rc = get_scratch(ctx, "eval_point_partials", total * 16, &d_part);
   if (rc) return rc;
   rc = get_scratch(ctx, "eval_point_out", n_cols * 16, &d_out);
   if (rc) return rc;
-  rc = launch(ctx, "eval_at_point_blocks", eval_point_blocks_kernel, dim3((uint32_t)total), dim3(256), 0,
-              (const EvalCol *)d_tab, (uint32_t)n_cols, f, (uint32_t *)d_part);
+  rc = launch(ctx, "eval_at_point_basis", eval_basis_kernel, dim3((n_basis + 255) / 256), dim3(256), 0, f,
+              (uint32_t *)d_basis, n_basis);
+  if (rc) return rc;
+  rc = launch(ctx, "eval_at_point_dot", eval_point_dot_kernel, dim3((uint32_t)work.size()), dim3(256), 0,
+              (const EvalCol *)d_tab, (const EvalWork *)d_work, f, (const uint32_t *)d_basis, (uint32_t *)d_part);
   if (rc) return rc;
   rc = launch(ctx, "eval_at_point_sum", eval_point_sum_kernel, dim3((uint32_t)n_cols), dim3(256), 0,
               (const EvalCol *)d_tab, (uint32_t)n_cols, (uint32_t)total, (const uint32_t *)d_part, (uint32_t *)d_out);

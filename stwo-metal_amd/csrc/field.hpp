@@ -51,6 +51,16 @@ __host__ __device__ __forceinline__ uint32_t m31_mul_dbl(uint32_t a, uint32_t t2
   uint32_t s = (uint32_t)(p >> 32) + ((uint32_t)p >> 1);
   return umin32(s, s - P);
 }
+// Lazy accumulation of products of canonical values (each below 2^62):
+// m31_fold64 keeps x mod P and brings any 64-bit x below 2^33.6, so three
+// more products fit before the next fold; m31_reduce64 gives the canonical
+// value of any 64-bit x (two folds leave less than P + 8).
+__host__ __device__ __forceinline__ uint64_t m31_fold64(uint64_t x) { return (x & P) + (x >> 31); }
+__host__ __device__ __forceinline__ uint32_t m31_reduce64(uint64_t x) {
+  x = m31_fold64(m31_fold64(x));
+  uint32_t r = (uint32_t)x;
+  return umin32(r, r - P);
+}
 __host__ __device__ inline uint32_t m31_pow(uint32_t a, uint64_t e) {
   uint32_t r = 1;
   while (e) {

@@ -123,9 +123,7 @@ __global__ void alpha_pow2k_kernel(const uint32_t *in, uint32_t *out, uint32_t k
 // the first 4.  chan = [digest 8 words, n_draws, error flag].  The root is
 // also copied to root_out.  The retry loop is bounded (each try fails with
 // probability about 2^-28); running out sets the error flag.
-__global__ void fri_channel_step_kernel(uint32_t *chan, const uint32_t *root, uint32_t *root_out,
-                                        uint32_t *alpha_out) {
-  if (threadIdx.x != 0) return;
+__device__ void channel_step(uint32_t *chan, const uint32_t *root, uint32_t *root_out, uint32_t *alpha_out) {
   uint32_t m[16];
   for (int k = 0; k < 8; k++) {
     m[k] = chan[k];
@@ -153,6 +151,122 @@ __global__ void fri_channel_step_kernel(uint32_t *chan, const uint32_t *root, ui
   chan[8] = n_draws;
   if (!ok) chan[9] = 1;
   for (int k = 0; k < 4; k++) alpha_out[k] = b2_reduce_word(s.h[k]);
+}
+
+__global__ void fri_channel_step_kernel(uint32_t *chan, const uint32_t *root, uint32_t *root_out,
+                                        uint32_t *alpha_out) {
+  if (threadIdx.x != 0) return;
+  channel_step(chan, root, root_out, alpha_out);
+}
+
+// ---- the small layers in one launch ---------------------------------------
+//
+// Below kTailLog the per-layer work (a few thousand hashes) is far shorter
+// than the launches it takes (leaves, node layers, channel step, fold), so
+// one workgroup runs every remaining inner layer: lifted leaves of the 4
+// coordinate columns, the node layers, the channel step, then the line fold
+// into the next layer's evaluation, with a workgroup barrier between phases
+// (all of a workgroup's waves share one CU's L1, so its global writes are
+// visible to the workgroup after __syncthreads).  It writes the same trees,
+// roots, channel state and folded evaluations as the per-layer launches.
+constexpr uint32_t kTailLog = 10;
+constexpr uint32_t kTailMax = 16;
+constexpr uint32_t kTailThreads = 1024;
+
+struct FriTailLayer {
+  const uint32_t *src[4];  // the layer's committed evaluation (log lg)
+  uint32_t *tree;          // its packed lifted tree
+  uint32_t *dst[4];        // the fold's output (log lg - step)
+  uint32_t lg, step, root_slot, pad;
+};
+struct FriTail {
+  FriTailLayer l[kTailMax];
+  uint32_t n;
+  const uint32_t *itw_end;
+  uint32_t *chan, *roots, *alpha;
+};
+
+// fold_line_kernel's per-output fold with a run-time step (1..4).
+template <int STEP>
+__device__ __forceinline__ void tail_fold(const FriTailLayer &L, const uint32_t *itw_end, qm31 alpha, uint32_t i) {
+  constexpr int W = 1 << STEP;
+  qm31 v[W];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int j = 0; j < W; j++) v[j].v[k] = L.src[k][(size_t)i * W + j];
+  qm31 a = alpha;
+#pragma unroll
+  for (int layer = 0; layer < STEP; layer++) {
+    const int half = W >> (layer + 1);
+    const uint32_t *lvl = itw_end - ((int64_t)1 << (L.lg - layer));
+#pragma unroll
+    for (int j = 0; j < half; j++) {
+      const uint32_t xi = lvl[(size_t)i * half + j];
+      qm31 f0, f1;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        f0.v[k] = m31_add(v[2 * j].v[k], v[2 * j + 1].v[k]);
+        f1.v[k] = m31_mul(m31_sub(v[2 * j].v[k], v[2 * j + 1].v[k]), xi);
+      }
+      v[j] = qm31_add(f0, qm31_mul(a, f1));
+    }
+    a = qm31_mul(a, a);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) L.dst[k][i] = v[0].v[k];
+}
+
+__global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t li = 0; li < t.n; li++) {
+    const FriTailLayer &L = t.l[li];
+    // lifted leaves: H(le32 of the 4 coordinates), one 16-byte block
+    const uint32_t n = 1u << L.lg;
+    uint32_t *leaves = L.tree + tree_layer_off(L.lg);
+    for (uint32_t i = tid; i < n; i += kTailThreads) {
+      uint32_t m[16];
+#pragma unroll
+      for (int k = 0; k < 4; k++) m[k] = L.src[k][i];
+#pragma unroll
+      for (int k = 4; k < 16; k++) m[k] = 0;
+      b2_state s;
+      b2_init(s);
+      b2_compress(s, m, 16, true);
+#pragma unroll
+      for (int k = 0; k < 8; k++) leaves[8 * (size_t)i + k] = s.h[k];
+    }
+    // node layers lg - 1 .. 0: H(left || right)
+    for (uint32_t l = L.lg; l-- > 0;) {
+      __syncthreads();
+      const uint32_t *child = L.tree + tree_layer_off(l + 1);
+      uint32_t *out = L.tree + tree_layer_off(l);
+      for (uint32_t i = tid; i < (1u << l); i += kTailThreads) {
+        uint32_t m[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) m[k] = child[16 * (size_t)i + k];
+        b2_state s;
+        b2_init(s);
+        b2_compress(s, m, 64, true);
+#pragma unroll
+        for (int k = 0; k < 8; k++) out[8 * (size_t)i + k] = s.h[k];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) channel_step(t.chan, L.tree, t.roots + 8 * L.root_slot, t.alpha);
+    __syncthreads();
+    const qm31 alpha = qm31_make(t.alpha[0], t.alpha[1], t.alpha[2], t.alpha[3]);
+    const uint32_t n_out = 1u << (L.lg - L.step);
+    for (uint32_t i = tid; i < n_out; i += kTailThreads) {
+      switch (L.step) {
+        case 1: tail_fold<1>(L, t.itw_end, alpha, i); break;
+        case 2: tail_fold<2>(L, t.itw_end, alpha, i); break;
+        case 3: tail_fold<3>(L, t.itw_end, alpha, i); break;
+        default: tail_fold<4>(L, t.itw_end, alpha, i); break;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
@@ -416,6 +530,38 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
     for (;;) {
       bool last = !(lg > last_log + cfg.line_fold_step);
       uint32_t step = last ? lg - last_log : cfg.line_fold_step;
+      if (lg <= kTailLog && cfg.line_fold_step <= 4) {
+        // Every remaining inner layer (at most kTailLog of them) in one launch.
+        FriTail t{};
+        t.itw_end = tw->d_itw + tw->len;
+        t.chan = d_chan;
+        t.roots = d_roots;
+        t.alpha = d_alpha;
+        for (;;) {
+          last = !(lg > last_log + cfg.line_fold_step);
+          step = last ? lg - last_log : cfg.line_fold_step;
+          if (t.n >= kTailMax || (keep && nr >= keep->layers.size()))
+            return set_error(ctx, STWO_HIP_EINVAL, "internal: FRI tail overflow");
+          FriTailLayer &L = t.l[t.n++];
+          L.tree = keep ? keep->layers[nr].tree : scratch_tree;
+          for (int k = 0; k < 4; k++) {
+            L.src[k] = cur[k];
+            if (keep) keep->layers[nr].cols[k] = cur[k];
+            L.dst[k] = layer_buf(n_folds, lg - step, k);
+            cur[k] = L.dst[k];
+          }
+          L.lg = lg;
+          L.step = step;
+          L.root_slot = nr++;
+          n_folds++;
+          line_init = (uint32_t)(((uint64_t)line_init << step) & IDX_MASK);
+          lg -= step;
+          if (last) break;
+        }
+        rc = launch(ctx, "fri_tail", fri_tail_kernel, dim3(1), dim3(kTailThreads), 0, t);
+        if (rc) return rc;
+        break;
+      }
       rc = commit_root(cur, lg);
       if (rc) return rc;
       uint32_t *nxt[4];
@@ -429,9 +575,19 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
       if (last) break;
     }
   }
+  // One synchronization for the channel state, the roots and the last layer.
   std::vector<uint32_t> st(kRootsOff + 8 * nr);
-  rc = stwo_hip_download(ctx, st.data(), state, st.size());
-  if (rc) return rc;
+  size_t n = (size_t)1 << lg;
+  std::vector<uint32_t> h(4 * n);
+  {
+    const DownloadReq reqs[5] = {{st.data(), state, st.size()},
+                                 {h.data(), cur[0], n},
+                                 {h.data() + n, cur[1], n},
+                                 {h.data() + 2 * n, cur[2], n},
+                                 {h.data() + 3 * n, cur[3], n}};
+    rc = download_many(ctx, reqs, 5);
+    if (rc) return rc;
+  }
   if (st[9]) return set_error(ctx, STWO_HIP_EDEVICE, "fri_commit: channel draw did not terminate");
   if (nr != sched.size()) return set_error(ctx, STWO_HIP_EDEVICE, "internal: FRI schedule mismatch");
   memcpy(roots, st.data() + kRootsOff, 32 * nr);
@@ -441,12 +597,6 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
   memcpy(ch.digest, st.data(), 32);
   ch.n_draws = st[8];
   // Last layer (fri.rs:274-291).
-  size_t n = (size_t)1 << lg;
-  std::vector<uint32_t> h(4 * n);
-  for (int k = 0; k < 4; k++) {
-    rc = stwo_hip_download(ctx, h.data() + k * n, cur[k], n);
-    if (rc) return rc;
-  }
   std::vector<qm31> vals(n);
   for (size_t i = 0; i < n; i++) vals[i] = qm31_make(h[i], h[n + i], h[2 * n + i], h[3 * n + i]);
   host_line_interpolate(vals, line_init, lg);
@@ -494,6 +644,16 @@ int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], ui
                                stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
                                const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots,
                                uint32_t *last_layer_poly, stwo_hip_fri_prover **out) {
+  return fri_prover_commit_mem(ctx, channel_digest, channel_n_draws, cfg, column, log_size, tw, roots, n_roots,
+                               last_layer_poly, nullptr, out);
+}
+
+}  // extern "C"
+
+int stwo_hip::fri_prover_commit_mem(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                                    stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
+                                    const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots,
+                                    uint32_t *last_layer_poly, const char *scratch_key, stwo_hip_fri_prover **out) {
   if (!ctx) return STWO_HIP_EINVAL;
   STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws && column && roots && n_roots && last_layer_poly && out,
                      "fri_prover_commit: NULL argument");
@@ -520,7 +680,14 @@ int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], ui
     eval_off.push_back(words);
     words += (size_t)4 << lg;
   }
-  rc = stwo_hip_alloc(ctx, words, &pr->mem);
+  if (scratch_key) {
+    void *m = nullptr;
+    rc = get_scratch(ctx, scratch_key, words * 4, &m);
+    pr->mem = (uint32_t *)m;
+    pr->owns_mem = false;
+  } else {
+    rc = stwo_hip_alloc(ctx, words, &pr->mem);
+  }
   if (rc) {
     delete pr;
     return rc;
@@ -535,13 +702,15 @@ int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], ui
   rc = fri_commit_impl(ctx, channel_digest, channel_n_draws, cfg, column, log_size, tw, roots, n_roots,
                        last_layer_poly, pr);
   if (rc) {
-    stwo_hip_free(ctx, pr->mem);
+    if (pr->owns_mem) stwo_hip_free(ctx, pr->mem);
     delete pr;
     return rc;
   }
   *out = pr;
   return STWO_HIP_OK;
 }
+
+extern "C" {
 
 // FriProver::decommit (fri.rs:294-347): draw the query positions from the
 // channel (draw_queries, core/queries.rs:11-28; Queries::new sorts and
@@ -670,11 +839,9 @@ int stwo_hip_fri_prover_destroy(stwo_hip_ctx *ctx, stwo_hip_fri_prover *prover) 
   if (!ctx) return STWO_HIP_EINVAL;
   if (!prover) return STWO_HIP_OK;
   int rc = stwo_hip_sync(ctx);
-  int rc2 = stwo_hip_free(ctx, prover->mem);
+  int rc2 = prover->owns_mem ? stwo_hip_free(ctx, prover->mem) : STWO_HIP_OK;
   delete prover;
   return rc ? rc : rc2;
 }
-
-
 
 }  // extern "C"

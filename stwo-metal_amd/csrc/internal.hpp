@@ -120,6 +120,7 @@ struct stwo_hip_fri_prover {
   uint32_t log_size = 0;
   uint32_t *mem = nullptr;  // trees and folded evaluations, one allocation
   size_t mem_words = 0;
+  bool owns_mem = true;     // false: context scratch memory
   std::vector<stwo_hip::FriLayerKeep> layers;  // committed layers, first (circle) layer first
   std::vector<uint32_t *> evals;               // output of fold f (4 columns of 2^log each)
   std::vector<uint32_t> last_layer_poly;       // 4 words per coefficient
@@ -132,6 +133,14 @@ struct stwo_hip_fri_prover {
 };
 
 namespace stwo_hip {
+
+// stwo_hip_fri_prover_commit whose memory is the context's scratch buffer
+// `scratch_key` (kept by the context, reused by the next call with the same
+// key) when scratch_key is not NULL.  One such prover per key at a time.
+int fri_prover_commit_mem(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
+                          stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
+                          const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly,
+                          const char *scratch_key, stwo_hip_fri_prover **out);
 
 // Merkle column reference (pointer + lifting shift).
 struct ColRef {
@@ -146,6 +155,6 @@ int launch_merkle_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const ColRef *d
 // Node layers [stop_log, top_log) of a packed tree whose layer top_log is filled.
 int hash_node_layers(stwo_hip_ctx *ctx, uint32_t *layers, uint32_t top_log, uint32_t stop_log, int mode);
 
-inline size_t tree_layer_off(uint32_t l) { return (((size_t)1 << l) - 1) * 8; }
+__host__ __device__ inline size_t tree_layer_off(uint32_t l) { return (((size_t)1 << l) - 1) * 8; }
 
 }  // namespace stwo_hip

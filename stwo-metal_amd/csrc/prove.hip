@@ -202,14 +202,14 @@ static double ms_since(Clock::time_point t0) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
-// Everything the prove allocates on the device, freed on every exit path.
+// The FRI prover, destroyed on every exit path.  Its memory and the prove's
+// buffers are context scratch (kept for the next prove, so a prove makes no
+// device allocation once the context has seen one of its size).
 struct ProveMem {
   stwo_hip_ctx *ctx = nullptr;
-  uint32_t *mem = nullptr;
   stwo_hip_fri_prover *fri = nullptr;
   ~ProveMem() {
     if (fri) (void)stwo_hip_fri_prover_destroy(ctx, fri);
-    if (mem) (void)stwo_hip_free(ctx, mem);
   }
 };
 
@@ -288,9 +288,10 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
 
   ProveMem pm;
   pm.ctx = ctx;
-  int rc = stwo_hip_alloc(ctx, off, &pm.mem);
+  void *mem_p = nullptr;
+  int rc = get_scratch(ctx, "wide_fib_prove", off * 4, &mem_p);
   if (rc) return rc;
-  uint32_t *mem = pm.mem;
+  uint32_t *mem = (uint32_t *)mem_p;
   std::vector<uint32_t *> coef(n_cols), eval(n_cols), ext(n_cols);
   for (uint32_t k = 0; k < n_cols; k++) {
     coef[k] = mem + o_coef + k * n;
@@ -435,8 +436,8 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
   std::vector<uint32_t> roots(8 * (E + 1));
   uint32_t n_roots = 0;
   std::vector<uint32_t> last(4u << config.fri.log_last_layer_degree_bound);
-  rc = stwo_hip_fri_prover_commit(ctx, ch.digest, &ch.n_draws, config.fri, quot, E, tw, roots.data(), &n_roots,
-                                  last.data(), &pm.fri);
+  rc = fri_prover_commit_mem(ctx, ch.digest, &ch.n_draws, config.fri, quot, E, tw, roots.data(), &n_roots,
+                             last.data(), "wide_fib_prove_fri", &pm.fri);
   if (rc) return rc;
   proof->last_layer_poly = last;
   proof->last_layer_log_size = config.fri.log_last_layer_degree_bound;

@@ -8,6 +8,8 @@
 //   sum over sample points z of
 //     ( sum over the entries (column i, sample at z) of (c_i * f~_i(p) - b_i)
 //       - (sum of a_i) * p.y ) * 1 / ((Re z.x - p.x) Im z.y - (Re z.y - p.y) Im z.x)
+// evaluated as (sum of c_i f~_i) - (sum of b_i) - (sum of a_i) p.y, the first
+// sum accumulated lazily in 64 bits and the other two precomputed per point;
 // with f~_i(p) column i read at its lifted index lift(row, L - log_i), and
 // (a_i, b_i, c_i) = alpha^k_i times complex_conjugate_line_coeffs
 // (STWO/core/constraints.rs:119-134).  The reference materialises one
@@ -32,11 +34,12 @@ struct QuotEntry {
   const uint32_t *col;
   uint32_t shift;  // L - log size of the column
   uint32_t pad;
-  uint32_t c[4], b[4];
+  uint32_t c[4];
 };
 struct QuotPoint {
   uint32_t prx[2], pix[2], pry[2], piy[2];  // Re/Im (CM31) of z.x and z.y
   uint32_t first[4];                        // sum of a_i over the point's entries
+  uint32_t bsum[4];                         // sum of b_i over the point's entries
   uint32_t e_begin, e_end;
 };
 struct GenTable {
@@ -74,14 +77,29 @@ __global__ void __launch_bounds__(256) quotient_kernel(const QuotPoint *pts, uin
     // 1 / (d0 + d1 i) = (d0 - d1 i) / (d0^2 + d1^2)
     const uint32_t ninv = m31_inv(m31_add(m31_mul(d0, d0), m31_mul(d1, d1)));
     const uint32_t i0 = m31_mul(d0, ninv), i1 = m31_mul(m31_neg(d1), ninv);
-    qm31 num = qm31_make(0, 0, 0, 0);
-    for (uint32_t e = z.e_begin; e < z.e_end; e++) {
-      const QuotEntry en = ents[e];
-      const uint32_t li = ((row >> (en.shift + 1)) << 1) + (row & 1);
-      const uint32_t v = as_global(en.col)[li];
-      num = qm31_add(num, qm31_sub(qm31_mul_m31(qm31_make(en.c[0], en.c[1], en.c[2], en.c[3]), v),
-                                   qm31_make(en.b[0], en.b[1], en.b[2], en.b[3])));
+    // sum (c_i f~_i - b_i) = sum c_i f~_i - sum b_i: the products go into
+    // 64-bit accumulators, folded after every third entry.
+    uint64_t acc[4] = {0, 0, 0, 0};
+    uint32_t e = z.e_begin;
+    for (; e + 3 <= z.e_end; e += 3) {
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+        const QuotEntry en = ents[e + u];
+        const uint32_t v = as_global(en.col)[((row >> (en.shift + 1)) << 1) + (row & 1)];
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += (uint64_t)v * en.c[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k]);
     }
+    for (; e < z.e_end; e++) {
+      const QuotEntry en = ents[e];
+      const uint32_t v = as_global(en.col)[((row >> (en.shift + 1)) << 1) + (row & 1)];
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[k] += (uint64_t)v * en.c[k];
+    }
+    qm31 num = qm31_make(m31_reduce64(acc[0]), m31_reduce64(acc[1]), m31_reduce64(acc[2]), m31_reduce64(acc[3]));
+    num = qm31_sub(num, qm31_make(z.bsum[0], z.bsum[1], z.bsum[2], z.bsum[3]));
     num = qm31_sub(num, qm31_mul_m31(qm31_make(z.first[0], z.first[1], z.first[2], z.first[3]), p.y));
     // QM31 x CM31: each CM31 half times (i0 + i1 i)
     uint32_t r0, r1, r2, r3;
@@ -160,7 +178,7 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
   // Group by sample point; line coefficients (constraints.rs:119-134).
   std::vector<QuotPoint> pts;
   std::vector<std::vector<QuotEntry>> per;
-  std::vector<qm31> firsts;
+  std::vector<qm31> firsts, bsums;
   std::vector<std::pair<qm31, qm31>> keys;
   for (const Ent &e : all) {
     STWO_HIP_CHECK_ARG(ctx, !eq(e.py, conj(e.py)), "Cannot evaluate a line with a single point.");
@@ -170,17 +188,18 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
       keys.push_back({e.px, e.py});
       per.emplace_back();
       firsts.push_back(qm31_make(0, 0, 0, 0));
+      bsums.push_back(qm31_make(0, 0, 0, 0));
     }
     const qm31 a = qm31_sub(conj(e.value), e.value);
     const qm31 cc = qm31_sub(conj(e.py), e.py);
     const qm31 bb = qm31_sub(qm31_mul(e.value, cc), qm31_mul(a, e.py));
     const qm31 ra = qm31_mul(e.rand, a), rb = qm31_mul(e.rand, bb), rc = qm31_mul(e.rand, cc);
     firsts[s] = qm31_add(firsts[s], ra);
+    bsums[s] = qm31_add(bsums[s], rb);
     QuotEntry q{};
     q.col = cols[e.col];
     q.shift = L - log_sizes[e.col];
     memcpy(q.c, rc.v, 16);
-    memcpy(q.b, rb.v, 16);
     per[s].push_back(q);
   }
   std::vector<QuotEntry> ents;
@@ -195,6 +214,7 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
     z.piy[0] = keys[s].second.v[2];
     z.piy[1] = keys[s].second.v[3];
     memcpy(z.first, firsts[s].v, 16);
+    memcpy(z.bsum, bsums[s].v, 16);
     z.e_begin = (uint32_t)ents.size();
     ents.insert(ents.end(), per[s].begin(), per[s].end());
     z.e_end = (uint32_t)ents.size();

@@ -61,12 +61,41 @@ void prof_end(stwo_hip_ctx *ctx, const char *name, hipEvent_t start) {
 }
 
 static const size_t kArenaBytes = 8u << 20;
+static const size_t kStageBytes = 1u << 20;
 
 static int arena_init(stwo_hip_ctx *ctx) {
   STWO_HIP_TRY(ctx, hipHostMalloc((void **)&ctx->h_arena, kArenaBytes, hipHostMallocDefault));
   STWO_HIP_TRY(ctx, hipMalloc((void **)&ctx->d_arena, kArenaBytes));
   ctx->arena_cap = kArenaBytes;
   ctx->arena_head = 0;
+  STWO_HIP_TRY(ctx, hipHostMalloc((void **)&ctx->h_stage, kStageBytes, hipHostMallocDefault));
+  ctx->stage_cap = kStageBytes;
+  return STWO_HIP_OK;
+}
+
+int download_many(stwo_hip_ctx *ctx, const DownloadReq *reqs, size_t n_reqs) {
+  size_t total = 0;
+  for (size_t i = 0; i < n_reqs; i++) total += reqs[i].n_words * 4;
+  if (total > ctx->stage_cap) {
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n_reqs; i++)
+      if (reqs[i].n_words)
+        STWO_HIP_TRY(ctx, hipMemcpy(reqs[i].dst, reqs[i].src, reqs[i].n_words * 4, hipMemcpyDeviceToHost));
+    return STWO_HIP_OK;
+  }
+  size_t off = 0;
+  for (size_t i = 0; i < n_reqs; i++) {
+    if (reqs[i].n_words)
+      STWO_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_stage + off, reqs[i].src, reqs[i].n_words * 4, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+    off += reqs[i].n_words * 4;
+  }
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  off = 0;
+  for (size_t i = 0; i < n_reqs; i++) {
+    memcpy(reqs[i].dst, ctx->h_stage + off, reqs[i].n_words * 4);
+    off += reqs[i].n_words * 4;
+  }
   return STWO_HIP_OK;
 }
 
@@ -193,6 +222,7 @@ int stwo_hip_ctx_destroy(stwo_hip_ctx *ctx) {
   for (auto e : ctx->event_pool) hipEventDestroy(e);
   for (auto &kv : ctx->scratch) hipFree(kv.second.first);
   if (ctx->h_arena) hipHostFree(ctx->h_arena);
+  if (ctx->h_stage) hipHostFree(ctx->h_stage);
   if (ctx->d_arena) hipFree(ctx->d_arena);
   hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -239,15 +269,20 @@ int stwo_hip_free(stwo_hip_ctx *ctx, uint32_t *ptr) {
 int stwo_hip_upload(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words) {
   if (!ctx || (!dst && n_words) || (!src && n_words)) return STWO_HIP_EINVAL;
   STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (n_words * 4 <= ctx->stage_cap) {
+    memcpy(ctx->h_stage, src, n_words * 4);
+    STWO_HIP_TRY(ctx, hipMemcpyAsync(dst, ctx->h_stage, n_words * 4, hipMemcpyHostToDevice, ctx->stream));
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return STWO_HIP_OK;
+  }
   STWO_HIP_TRY(ctx, hipMemcpy(dst, src, n_words * 4, hipMemcpyHostToDevice));
   return STWO_HIP_OK;
 }
 
 int stwo_hip_download(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words) {
   if (!ctx || (!dst && n_words) || (!src && n_words)) return STWO_HIP_EINVAL;
-  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  STWO_HIP_TRY(ctx, hipMemcpy(dst, src, n_words * 4, hipMemcpyDeviceToHost));
-  return STWO_HIP_OK;
+  const DownloadReq r = {dst, src, n_words};
+  return download_many(ctx, &r, 1);
 }
 
 int stwo_hip_copy(stwo_hip_ctx *ctx, uint32_t *dst, const uint32_t *src, size_t n_words) {

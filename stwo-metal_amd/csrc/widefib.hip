@@ -41,23 +41,27 @@ __global__ void __launch_bounds__(256) wide_fib_trace_kernel(uint32_t *const *co
   }
 }
 
+// The row's combination regrouped by column: with weights w_j = alpha^(N-3-j),
+//   sum_j w_j (x_{j+2} - x_j^2 - x_{j+1}^2) = sum_i (x_i W1_i + x_i^2 W2_i),
+// W1_i = w_{i-2} (i >= 2) and W2_i = -(w_i [i <= N-3] + w_{i-1} [1 <= i <= N-2]),
+// so each column costs one square and 8 multiply-adds into 64-bit
+// accumulators (folded after each column: two products per limb).
 __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t *const *cols, uint32_t n_cols,
                                                                  uint32_t eval_log, uint32_t log_n,
-                                                                 const uint32_t *powers, const uint32_t *denom_inv,
+                                                                 const uint32_t *weights, const uint32_t *denom_inv,
                                                                  uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
                                                                  uint32_t *acc3) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >> eval_log) return;
-  uint32_t a = as_global(cols[0])[r], b = as_global(cols[1])[r];
-  qm31 res = qm31_make(0, 0, 0, 0);
-  for (uint32_t k = 2; k < n_cols; k++) {
-    const uint32_t c = as_global(cols[k])[r];
-    const uint32_t con = m31_sub(c, m31_add(m31_mul(a, a), m31_mul(b, b)));
-    const uint32_t *pw = powers + 4 * (k - 2);  // alpha^(N-3-(k-2)) for constraint k-2
-    res = qm31_add(res, qm31_mul_m31(qm31_make(pw[0], pw[1], pw[2], pw[3]), con));
-    a = b;
-    b = c;
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (uint32_t i = 0; i < n_cols; i++) {
+    const uint32_t x = as_global(cols[i])[r];
+    const uint32_t sq = m31_mul(x, x);
+    const uint32_t *w = weights + 8 * i;  // W1 (4 words), W2 (4 words)
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k] + (uint64_t)x * w[k] + (uint64_t)sq * w[4 + k]);
   }
+  qm31 res = qm31_make(m31_reduce64(acc[0]), m31_reduce64(acc[1]), m31_reduce64(acc[2]), m31_reduce64(acc[3]));
   res = qm31_mul_m31(res, denom_inv[r >> log_n]);
   acc0[r] = m31_add(acc0[r], res.v[0]);
   acc1[r] = m31_add(acc1[r], res.v[1]);
@@ -96,15 +100,27 @@ int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *co
   for (int k = 0; k < 4; k++) STWO_HIP_CHECK_ARG(ctx, accum[k] != nullptr, "wide_fib_constraint_quotients: NULL accumulator");
   const uint32_t eval_log = log_n + 1;
   const uint32_t n_cons = n_cols - 2;
-  // Powers alpha^0 .. alpha^(n_cons - 1), reversed: constraint j uses alpha^(n_cons - 1 - j).
-  std::vector<uint32_t> powers(4 * n_cons);
+  // Powers alpha^0 .. alpha^(n_cons - 1), reversed: constraint j uses
+  // w_j = alpha^(n_cons - 1 - j); per column the weights W1, W2 of the kernel.
+  std::vector<qm31> pw(n_cons);
   {
     const qm31 alpha = qm31_make(random_coeff[0], random_coeff[1], random_coeff[2], random_coeff[3]);
     qm31 p = qm31_make(1, 0, 0, 0);
     for (uint32_t e = 0; e < n_cons; e++) {
-      memcpy(&powers[4 * (n_cons - 1 - e)], p.v, 16);
+      pw[n_cons - 1 - e] = p;
       p = qm31_mul(p, alpha);
     }
+  }
+  std::vector<uint32_t> powers(8 * (size_t)n_cols);
+  for (uint32_t i = 0; i < n_cols; i++) {
+    const qm31 zero = qm31_make(0, 0, 0, 0);
+    const qm31 w1 = i >= 2 ? pw[i - 2] : zero;
+    qm31 w2 = zero;
+    if (i < n_cons) w2 = qm31_add(w2, pw[i]);
+    if (i >= 1 && i - 1 < n_cons) w2 = qm31_add(w2, pw[i - 1]);
+    w2 = qm31_sub(zero, w2);
+    memcpy(&powers[8 * i], w1.v, 16);
+    memcpy(&powers[8 * i + 4], w2.v, 16);
   }
   // Vanishing-polynomial inverses at the first 2^(eval_log - log_n) = 2 points of
   // CanonicCoset(eval_log).circle_domain(), bit-reversed.  The trace coset is

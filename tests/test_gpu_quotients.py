@@ -118,6 +118,25 @@ def test_eval_at_point_vs_oracle(ctx):
         x.free()
 
 
+def test_eval_at_point_grouped_and_extreme_values(ctx):
+    """Runs of same-size columns (grouped 8 to a workgroup, with a remainder)
+    and coefficients all P - 1, which drive the 64-bit accumulation to its
+    largest sums; bit-exact against the oracle."""
+    rng = np.random.default_rng(9)
+    logs = [12] * 19 + [3] * 9 + [11] * 2 + [14]
+    cols = [rng.integers(0, P, 1 << lg, dtype=np.uint64).astype(U) for lg in logs]
+    cols[3][:] = P - 1
+    cols[20][:] = P - 1
+    cols[-1][:] = P - 1
+    d = [ctx.to_device(c) for c in cols]
+    pt = secure_point(0x5EED)
+    got = ctx.eval_at_point(d, logs, pt)
+    for c, g in zip(cols, got):
+        assert tuple(int(v) for v in g) == O.eval_at_point(c, pt)
+    for x in d:
+        x.free()
+
+
 @pytest.mark.parametrize("log_n,n_cols", [(5, 8), (12, 100), (16, 256)])
 def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols):
     """Trace generation and the constraint quotients, bit-exact against the
