@@ -123,39 +123,47 @@ __global__ void alpha_pow2k_kernel(const uint32_t *in, uint32_t *out, uint32_t k
 // the first 4.  chan = [digest 8 words, n_draws, error flag].  The root is
 // also copied to root_out.  The retry loop is bounded (each try fails with
 // probability about 2^-28); running out sets the error flag.
+// Run by the 4 lanes of one quad (lanes 0-3 of a wave, all active), each
+// compression shared by the quad (b2_compress_quad).
 __device__ void channel_step(uint32_t *chan, const uint32_t *root, uint32_t *root_out, uint32_t *alpha_out) {
+  const uint32_t j = __lane_id() & 3;
+  const uint32_t init_lo = b2_iv_lo(j), init_hi = b2_iv_hi(j);
   uint32_t m[16];
   for (int k = 0; k < 8; k++) {
     m[k] = chan[k];
     m[8 + k] = root[k];
-    root_out[k] = root[k];
   }
-  b2_state s;
-  b2_init(s);
-  b2_compress(s, m, 64, true);
+  root_out[j] = root[j];
+  root_out[4 + j] = root[4 + j];
+  uint32_t lo = init_lo, hi = init_hi;
+  b2_compress_quad(lo, hi, m, 64, true);
   uint32_t digest[8];
-  for (int k = 0; k < 8; k++) digest[k] = s.h[k];
+  b2_quad_gather(lo, hi, digest);
   uint32_t n_draws = 0;
   bool ok = false;
   for (int tries = 0; tries < 64 && !ok; tries++) {
     for (int k = 0; k < 8; k++) m[k] = digest[k];
     m[8] = n_draws;
     for (int k = 9; k < 16; k++) m[k] = 0;  // the 0x00 separator byte and zero padding
-    b2_init(s);
-    b2_compress(s, m, 37, true);
+    lo = init_lo;
+    hi = init_hi;
+    b2_compress_quad(lo, hi, m, 37, true);
     n_draws++;
-    ok = true;
-    for (int k = 0; k < 8; k++) ok = ok && s.h[k] < 2u * P;
+    const uint64_t bal = __ballot(lo < 2u * P && hi < 2u * P);
+    ok = ((bal >> (__lane_id() & ~3u)) & 0xF) == 0xF;
   }
-  for (int k = 0; k < 8; k++) chan[k] = digest[k];
-  chan[8] = n_draws;
-  if (!ok) chan[9] = 1;
-  for (int k = 0; k < 4; k++) alpha_out[k] = b2_reduce_word(s.h[k]);
+  chan[j] = digest[j];
+  chan[4 + j] = digest[4 + j];
+  if (j == 0) {
+    chan[8] = n_draws;
+    if (!ok) chan[9] = 1;
+  }
+  alpha_out[j] = b2_reduce_word(lo);
 }
 
 __global__ void fri_channel_step_kernel(uint32_t *chan, const uint32_t *root, uint32_t *root_out,
                                         uint32_t *alpha_out) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x >= 4) return;
   channel_step(chan, root, root_out, alpha_out);
 }
 
@@ -222,25 +230,54 @@ __global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
   for (uint32_t li = 0; li < t.n; li++) {
     const FriTailLayer &L = t.l[li];
     // lifted leaves: H(le32 of the 4 coordinates), one 16-byte block
+    // Levels of at most kTailThreads / 4 hashes use a quad per hash (their
+    // latency is the layer's critical path), larger ones a lane per hash.
     const uint32_t n = 1u << L.lg;
+    const uint32_t q = tid >> 2, j = tid & 3;
     uint32_t *leaves = L.tree + tree_layer_off(L.lg);
-    for (uint32_t i = tid; i < n; i += kTailThreads) {
-      uint32_t m[16];
+    if (n <= kTailThreads / 4) {
+      if (q < n) {
+        uint32_t m[16];
 #pragma unroll
-      for (int k = 0; k < 4; k++) m[k] = L.src[k][i];
+        for (int k = 0; k < 4; k++) m[k] = L.src[k][q];
 #pragma unroll
-      for (int k = 4; k < 16; k++) m[k] = 0;
-      b2_state s;
-      b2_init(s);
-      b2_compress(s, m, 16, true);
+        for (int k = 4; k < 16; k++) m[k] = 0;
+        uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+        b2_compress_quad(lo, hi, m, 16, true);
+        leaves[8 * (size_t)q + j] = lo;
+        leaves[8 * (size_t)q + 4 + j] = hi;
+      }
+    } else {
+      for (uint32_t i = tid; i < n; i += kTailThreads) {
+        uint32_t m[16];
 #pragma unroll
-      for (int k = 0; k < 8; k++) leaves[8 * (size_t)i + k] = s.h[k];
+        for (int k = 0; k < 4; k++) m[k] = L.src[k][i];
+#pragma unroll
+        for (int k = 4; k < 16; k++) m[k] = 0;
+        b2_state s;
+        b2_init(s);
+        b2_compress(s, m, 16, true);
+#pragma unroll
+        for (int k = 0; k < 8; k++) leaves[8 * (size_t)i + k] = s.h[k];
+      }
     }
     // node layers lg - 1 .. 0: H(left || right)
     for (uint32_t l = L.lg; l-- > 0;) {
       __syncthreads();
       const uint32_t *child = L.tree + tree_layer_off(l + 1);
       uint32_t *out = L.tree + tree_layer_off(l);
+      if ((1u << l) <= kTailThreads / 4) {
+        if (q < (1u << l)) {
+          uint32_t m[16];
+#pragma unroll
+          for (int k = 0; k < 16; k++) m[k] = child[16 * (size_t)q + k];
+          uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+          b2_compress_quad(lo, hi, m, 64, true);
+          out[8 * (size_t)q + j] = lo;
+          out[8 * (size_t)q + 4 + j] = hi;
+        }
+        continue;
+      }
       for (uint32_t i = tid; i < (1u << l); i += kTailThreads) {
         uint32_t m[16];
 #pragma unroll
@@ -253,7 +290,7 @@ __global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
       }
     }
     __syncthreads();
-    if (tid == 0) channel_step(t.chan, L.tree, t.roots + 8 * L.root_slot, t.alpha);
+    if (tid < 4) channel_step(t.chan, L.tree, t.roots + 8 * L.root_slot, t.alpha);
     __syncthreads();
     const qm31 alpha = qm31_make(t.alpha[0], t.alpha[1], t.alpha[2], t.alpha[3]);
     const uint32_t n_out = 1u << (L.lg - L.step);

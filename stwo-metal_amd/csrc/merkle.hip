@@ -119,6 +119,34 @@ __global__ void __launch_bounds__(256) merkle_nodes_kernel(const uint32_t *prev,
     __syncthreads();
     active >>= 1;
     uint32_t lg = out_log - layer;
+    if (active <= 64) {
+      // Few hashes left: a quad per hash (b2_compress_quad) shortens the
+      // critical path of the remaining layers.
+      const uint32_t q = threadIdx.x >> 2, j = threadIdx.x & 3;
+      const bool work = q < active;
+      uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+      if (work) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          m[k] = hs[2 * q][k];
+          m[8 + k] = hs[2 * q + 1][k];
+        }
+        b2_compress_quad(lo, hi, m, 64, true);
+        if (m31) {
+          lo = b2_reduce_word(lo);
+          hi = b2_reduce_word(hi);
+        }
+        g_u32 *o = as_global(packed + (((size_t)1 << lg) - 1) * 8 + 8 * ((size_t)blockIdx.x * (256u >> layer) + q));
+        o[j] = lo;
+        o[4 + j] = hi;
+      }
+      __syncthreads();
+      if (work) {
+        hs[q][j] = lo;
+        hs[q][4 + j] = hi;
+      }
+      continue;
+    }
     uint32_t gi = blockIdx.x * (256u >> layer) + threadIdx.x;
     b2_state s2;
     bool work = threadIdx.x < active;
