@@ -115,6 +115,24 @@ __global__ void alpha_pow2k_kernel(const uint32_t *in, uint32_t *out, uint32_t k
   for (int j = 0; j < 4; j++) out[j] = a.v[j];
 }
 
+// Lifted Merkle leaves of one FRI layer: leaf i = H(le32 of the 4 coordinate
+// values at i), one 16-byte block (CPUB/merkle_lifted.rs:44-78 with four
+// columns of the tree's size).
+__global__ void __launch_bounds__(256) fri_leaves_kernel(Qm31ColsConst src, uint32_t lg, uint32_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >> lg) return;
+  uint32_t m[16];
+#pragma unroll
+  for (int k = 0; k < 4; k++) m[k] = as_global(src.c[k])[i];
+#pragma unroll
+  for (int k = 4; k < 16; k++) m[k] = 0;
+  b2_state s;
+  b2_init(s);
+  b2_compress(s, m, 16, true);
+#pragma unroll
+  for (int k = 0; k < 8; k++) as_global(out)[8 * (size_t)i + k] = s.h[k];
+}
+
 // One Fiat-Shamir step of the FRI layer loop on the device, for the
 // Blake2sChannel (STWO/core/channel/blake2s.rs): digest = H(digest || root)
 // (MerkleChannel::mix_root, STWO/core/vcs_lifted/blake2_merkle.rs:49-55),
@@ -537,7 +555,15 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
     uint32_t *tree = keep ? keep->layers[nr].tree : scratch_tree;
     if (keep)
       for (int k = 0; k < 4; k++) keep->layers[nr].cols[k] = cols[k];
-    int r = stwo_hip_merkle_lifted_commit(ctx, cols, logs, 4, lg, tree, STWO_HIP_HASH_BLAKE2S);
+    (void)logs;
+    // Lifted leaves of the 4 equal-size coordinate columns (pointers by
+    // value: no table upload), then the node layers.
+    Qm31ColsConst c4;
+    for (int k = 0; k < 4; k++) c4.c[k] = cols[k];
+    int r = launch(ctx, "merkle_lifted_leaves", fri_leaves_kernel, dim3((uint32_t)(((1ull << lg) + 255) / 256)),
+                   dim3(256), 0, c4, lg, tree + tree_layer_off(lg));
+    if (r) return r;
+    r = hash_node_layers(ctx, tree, lg, 0, STWO_HIP_HASH_BLAKE2S);
     if (r) return r;
     r = launch(ctx, "fri_channel_step", fri_channel_step_kernel, dim3(1), dim3(64), 0, d_chan, (const uint32_t *)tree,
                d_roots + 8 * nr, d_alpha);
