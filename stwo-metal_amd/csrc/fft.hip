@@ -476,7 +476,11 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     in = p.cols[col_idx].in;
     out = p.cols[col_idx].out;
   }
-  if (tpt_log >= 6) {  // a tile spans whole waves: the column is wave-uniform
+  // A tile spans whole waves: the column is wave-uniform.  The fixed
+  // schedules only run tiles of 2^12 or more, so for them this is known at
+  // compile time and the pointers stay in SGPRs (loads and stores then take
+  // an SGPR base and a 32-bit lane offset instead of 64-bit lane addresses).
+  if (NS > 0 || tpt_log >= 6) {
     in = (const uint32_t *)readfirstlane_u64((uint64_t)in);
     out = (uint32_t *)readfirstlane_u64((uint64_t)out);
   }
