@@ -477,10 +477,12 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     out = p.cols[col_idx].out;
   }
   // A tile spans whole waves: the column is wave-uniform.  The fixed
-  // schedules only run tiles of 2^12 or more, so for them this is known at
-  // compile time and the pointers stay in SGPRs (loads and stores then take
-  // an SGPR base and a 32-bit lane offset instead of 64-bit lane addresses).
-  if (NS > 0 || tpt_log >= 6) {
+  // schedules (NS > 0) only run tiles of 2^12 or more and NS = -1 marks a
+  // run-time schedule launched only for tiles of 2^10 or more, so for them
+  // this is known at compile time and the pointers stay in SGPRs (loads and
+  // stores then take an SGPR base and a 32-bit lane offset instead of 64-bit
+  // lane addresses, which also frees half the kernel's VGPRs).
+  if (NS != 0 || tpt_log >= 6) {
     in = (const uint32_t *)readfirstlane_u64((uint64_t)in);
     out = (uint32_t *)readfirstlane_u64((uint64_t)out);
   }
@@ -766,9 +768,18 @@ int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t
     return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, 2, 5, 9>, dim3(wgs), dim3(512), 0, p)
                    : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, 2, 9, 5>, dim3(wgs), dim3(512), 0, p);
   // Profiling names follow the instantiation: *_wg9 = fft_pass_kernel<INV, 9, ...>.
-  if (wg_log == 9)
+  // Tiles of 2^10 or more span whole waves (NS = -1: wave-uniform columns).
+  const bool uni = p.T >= 10;
+  if (wg_log == 9) {
+    if (uni)
+      return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, -1>, dim3(wgs), dim3(512), 0, p)
+                     : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, -1>, dim3(wgs), dim3(512), 0, p);
     return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9>, dim3(wgs), dim3(512), 0, p)
                    : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9>, dim3(wgs), dim3(512), 0, p);
+  }
+  if (uni)
+    return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, -1>, dim3(wgs), dim3(256), 0, p)
+                   : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, -1>, dim3(wgs), dim3(256), 0, p);
   return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8>, dim3(wgs), dim3(256), 0, p)
                  : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8>, dim3(wgs), dim3(256), 0, p);
 }
