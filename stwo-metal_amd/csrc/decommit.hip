@@ -50,21 +50,29 @@ int Gatherer::run(stwo_hip_ctx *ctx) {
   std::vector<GatherOp> ops(ops_.size());
   for (size_t i = 0; i < ops_.size(); i++)
     ops[i] = {(const uint32_t *)ops_[i].src, ops_[i].n, ops_[i].dst};
+  // The op table goes through the pinned upload ring when it fits (stream
+  // order keeps it valid for the kernel), the result through the pinned
+  // staging buffer: one synchronization per gather.
   void *d_ops = nullptr, *d_out = nullptr;
-  int rc = get_scratch(ctx, "gather_ops", ops.size() * sizeof(GatherOp), &d_ops);
-  if (rc) return rc;
+  int rc;
+  const size_t ops_bytes = ops.size() * sizeof(GatherOp);
+  if (ops_bytes <= ctx->arena_cap / 2) {
+    rc = upload_table(ctx, ops.data(), ops_bytes, &d_ops);
+    if (rc) return rc;
+  } else {
+    rc = get_scratch(ctx, "gather_ops", ops_bytes, &d_ops);
+    if (rc) return rc;
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    STWO_HIP_TRY(ctx, hipMemcpy(d_ops, ops.data(), ops_bytes, hipMemcpyHostToDevice));
+  }
   rc = get_scratch(ctx, "gather_out", words * 4, &d_out);
   if (rc) return rc;
-  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  STWO_HIP_TRY(ctx, hipMemcpyAsync(d_ops, ops.data(), ops.size() * sizeof(GatherOp), hipMemcpyHostToDevice,
-                                   ctx->stream));
   const uint32_t n_ops = (uint32_t)ops.size();
   rc = launch(ctx, "decommit_gather", gather_kernel, dim3((n_ops + 255) / 256), dim3(256), 0,
               (const GatherOp *)d_ops, n_ops, (uint32_t *)d_out);
   if (rc) return rc;
-  STWO_HIP_TRY(ctx, hipMemcpyAsync(out.data(), d_out, words * 4, hipMemcpyDeviceToHost, ctx->stream));
-  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  return STWO_HIP_OK;
+  const DownloadReq r = {out.data(), (const uint32_t *)d_out, words};
+  return download_many(ctx, &r, 1);
 }
 
 // The hash-witness traversal of MerkleProverLifted::decommit

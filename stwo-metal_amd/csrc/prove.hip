@@ -224,21 +224,34 @@ static int fetch_layer(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *fri, uint32
                                          out.hash_witness.data(), nh, &nh, nullptr, nullptr, 0, &npos);
 }
 
-// Lifted-tree decommitment of `cols` (all of log size L) at the query positions.
-static int decommit_tree(stwo_hip_ctx *ctx, const uint32_t *tree, uint32_t L, const std::vector<const uint32_t *> &cols,
-                         const std::vector<uint32_t> &queries, std::vector<std::vector<uint32_t>> &values,
-                         std::vector<uint32_t> &witness) {
-  const size_t nq = queries.size(), nc = cols.size();
-  std::vector<uint32_t> logs(nc, L), vals(nc * nq);
-  const size_t cap = nq * (L + 1) + 1;
-  witness.assign(8 * cap, 0);
-  size_t nw = 0, naux = 0;
-  int rc = stwo_hip_merkle_lifted_decommit(ctx, tree, L, cols.data(), logs.data(), nc, queries.data(), nq,
-                                           vals.data(), witness.data(), cap, &nw, nullptr, 0, &naux);
+// MerkleProverLifted::decommit of several trees (all columns of log size L)
+// at the same query positions, with one device gather for all of them.
+struct TreeDecommit {
+  const uint32_t *tree;
+  std::vector<const uint32_t *> cols;
+  std::vector<std::vector<uint32_t>> *values;  // [column][query]
+  std::vector<uint32_t> *witness;              // 8 words per hash
+};
+static int decommit_trees(stwo_hip_ctx *ctx, uint32_t L, const std::vector<uint32_t> &queries,
+                          std::vector<TreeDecommit> &trees) {
+  Gatherer g;
+  std::vector<std::vector<size_t>> val_off(trees.size());
+  std::vector<MerkleDecommitPlan> plans(trees.size());
+  for (size_t t = 0; t < trees.size(); t++) {
+    for (const uint32_t *c : trees[t].cols)
+      for (uint32_t pos : queries) val_off[t].push_back(g.add(c + pos, 1));  // log size L: no lifting
+    plan_merkle_decommit(g, trees[t].tree, L, queries, plans[t]);
+  }
+  int rc = g.run(ctx);
   if (rc) return rc;
-  witness.resize(8 * nw);
-  values.assign(nc, {});
-  for (size_t c = 0; c < nc; c++) values[c].assign(vals.begin() + c * nq, vals.begin() + (c + 1) * nq);
+  const size_t nq = queries.size();
+  for (size_t t = 0; t < trees.size(); t++) {
+    const size_t nc = trees[t].cols.size();
+    trees[t].values->assign(nc, std::vector<uint32_t>(nq));
+    for (size_t c = 0; c < nc; c++)
+      for (size_t q = 0; q < nq; q++) (*trees[t].values)[c][q] = g.out[val_off[t][c * nq + q]];
+    finish_merkle_decommit(g, plans[t], *trees[t].witness, nullptr);
+  }
   return STWO_HIP_OK;
 }
 
@@ -468,9 +481,12 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
   proof->decommitments.resize(3);
   proof->queried_values.resize(3);
   {
-    std::vector<const uint32_t *> c1(eval.begin(), eval.end()), c2(ceval.begin(), ceval.end());
-    rc = decommit_tree(ctx, tree1, E, c1, qpos, proof->queried_values[1], proof->decommitments[1]);
-    if (!rc) rc = decommit_tree(ctx, tree2, E, c2, qpos, proof->queried_values[2], proof->decommitments[2]);
+    std::vector<TreeDecommit> trees(2);
+    trees[0] = {tree1, std::vector<const uint32_t *>(eval.begin(), eval.end()), &proof->queried_values[1],
+                &proof->decommitments[1]};
+    trees[1] = {tree2, std::vector<const uint32_t *>(ceval.begin(), ceval.end()), &proof->queried_values[2],
+                &proof->decommitments[2]};
+    rc = decommit_trees(ctx, E, qpos, trees);
     if (rc) return rc;
   }
   proof->phase_ms[6] = ms_since(t0);
