@@ -299,17 +299,26 @@ def prove_wide_fib(ctx, runs=3, cpu=True):
     return res
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
-    reading for a copy of known size), or None when none covers it."""
+# Integer-VALU issue ceiling of the MI355X: 256 CUs x 4 SIMDs, one wave64
+# VALU instruction per SIMD every 4 cycles at 2.4 GHz.
+VALU_PEAK_WAVE_INSTR_S = 256 * 4 * 2.4e9 / 4
+
+
+def measured_counters(kernel):
+    """The committed rocprofv3 --pmc summary's entry for `kernel` (HBM bytes
+    per launch from FETCH_SIZE and WRITE_SIZE, each scaled by the same
+    counter's reading for a copy of known size; VALU wave-instructions per
+    launch from SQ_INSTS_VALU), or {} when none covers it."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
-    k = t.get("kernels", {}).get(kernel)
-    return None if k is None else k.get("hbm_bytes_per_launch")
+        return {}
+    return t.get("kernels", {}).get(kernel) or {}
+
+
+def measured_traffic(kernel):
+    return measured_counters(kernel).get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(sample_cols=N_COLS, runs=3):
@@ -413,6 +422,16 @@ def main():
     dom_bytes = alg[dom_name] / dom_launches_per_step
     achieved = dom_bytes / (dom_launch_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom_name)
+    # The same kernel against the integer-VALU issue ceiling, which is what
+    # bounds it: its VALU wave-instructions per launch (committed counter run)
+    # over the live launch time.
+    valu_instr = measured_counters(dom_name).get("valu_wave_instr_per_launch")
+    valu_roofline = None
+    if valu_instr:
+        va = valu_instr / (dom_launch_ms * 1e-3)
+        valu_roofline = {"kernel": dom_name, "wave_instr_per_launch": valu_instr, "achieved": round(va / 1e9, 1),
+                         "peak": round(VALU_PEAK_WAVE_INSTR_S / 1e9, 1), "unit": "G wave-instr/s",
+                         "frac": round(va / VALU_PEAK_WAVE_INSTR_S, 4)}
     log("kernels: %s" % json.dumps(kernels))
 
     serve = None
@@ -478,6 +497,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
         },
+        "valu_roofline": valu_roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "batched_uneven": batched,
