@@ -443,6 +443,8 @@ __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16]
 // 256 x 2^20 -> 2^22: 9.06 ms per step against 9.48 ms with the run-time
 // schedule at 5 waves (the 8-layer strided RFFT pass 1.95 ms against 2.28).
 //
+constexpr uint32_t T_FIXED12 = 12;  // tile bits of the fixed three-stage passes
+
 // NS > 0: the pass's stages are the compile-time schedule B0A, B0B, B0C
 // (first NS of them), each applying all 4 layers of its group; the host only
 // picks such a kernel for passes whose plan is exactly that schedule.
@@ -523,12 +525,24 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
       // --- load in the top layout
       // (gb | j << sh) & in_mask = (gb & in_mask) | ((j << sh) & in_mask):
       // a lane offset plus a wave-uniform one.
+      // One SGPR base and a lane byte offset per element: (j << sh) & in_mask
+      // = (j & (in_mask >> sh)) << sh, since in_mask is a low-bit mask.
+      // (Sixteen 64-bit element bases in SGPRs overflowed the SGPR file and
+      // spilled into VGPR lanes.)
+      // The fixed 12-layer passes (NS == 3) have w = lo = 0 and an input of
+      // at least one tile (both checked at launch), so there the shift is the
+      // compile-time top_b0 = 8 and the element offsets are constants.
       const uint32_t gb = gidx(local), sh = top_b0 - p.w + p.lo;
       if (rep > 0) __syncthreads();
       if (active) {
-        const uint32_t vo = (gb & in_mask) << 2;
+        const uint32_t vo = (gb & in_mask) << 2, jmask = in_mask >> sh;
+        if (NS == 3) {
 #pragma unroll
-        for (int j = 0; j < 16; j++) x[j] = gld(in + (((uint32_t)j << sh) & in_mask), vo);
+          for (int j = 0; j < 16; j++) x[j] = gld(in, vo + ((uint32_t)j << (T_FIXED12 - 4 + 2)));
+        } else {
+#pragma unroll
+          for (int j = 0; j < 16; j++) x[j] = gld(in, vo + ((((uint32_t)j) & jmask) << (sh + 2)));
+        }
       }
     }
     if constexpr (NS > 0) {
@@ -579,8 +593,15 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
     if (active) {
       const uint32_t gb = gidx(group_base(cur)), sh = cur - p.w + p.lo;
       const uint32_t vo = gb << 2;
+      if (NS == 3) {
+        // fixed 12-layer passes end in the top layout (B0C is 0 or 8; 0 was
+        // moved there above) with w = lo = 0: sh = 8
 #pragma unroll
-      for (int j = 0; j < 16; j++) gst(out + ((uint32_t)j << sh), vo, x[j]);
+        for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (T_FIXED12 - 4 + 2)), x[j]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (sh + 2)), x[j]);
+      }
     }
   }
 }
@@ -761,7 +782,8 @@ static bool full_schedule(const FftPass2 &p, const uint32_t *b0s, uint32_t ns) {
 
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log) {
   static const uint32_t kFwd12[3] = {8, 4, 0}, kInv12[3] = {0, 4, 8}, kFwd13[2] = {9, 5}, kInv13[2] = {5, 9};
-  if (wg_log == 8 && p.T == 12 && full_schedule(p, inverse ? kInv12 : kFwd12, 3))
+  if (wg_log == 8 && p.T == T_FIXED12 && p.w == 0 && p.lo == 0 && p.in_log >= T_FIXED12 &&
+      full_schedule(p, inverse ? kInv12 : kFwd12, 3))
     return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 3, 0, 4, 8>, dim3(wgs), dim3(256), 0, p)
                    : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 3, 8, 4, 0>, dim3(wgs), dim3(256), 0, p);
   if (wg_log == 9 && p.T == 13 && full_schedule(p, inverse ? kInv13 : kFwd13, 2))
