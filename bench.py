@@ -433,6 +433,24 @@ def main():
                          "peak": round(VALU_PEAK_WAVE_INSTR_S / 1e9, 1), "unit": "G wave-instr/s",
                          "frac": round(va / VALU_PEAK_WAVE_INSTR_S, 4)}
     log("kernels: %s" % json.dumps(kernels))
+    # Side rates: evaluated (post-blowup) elements per second; the FFT
+    # kernels' algorithmic bytes over their device time; Blake2s
+    # compressions (2^L * ceil(4 C / 64) leaf blocks + 2^L - 1 nodes) per
+    # second of step time and of Merkle kernel time.
+    fft_keys = [k for k in kernels if "fft_pass" in k]
+    fft_ms = sum(kernels[k]["ms_total"] for k in fft_keys) / args.steps
+    merkle_ms = sum(kernels[k]["ms_total"] for k in ("merkle_commit_on_layer", "merkle_nodes") if k in kernels) / args.steps
+    n_compress = n_out * -(-4 * N_COLS // 64) + (n_out - 1)
+    side = {
+        "evaluated_melem_s": round(N_COLS * n_out / (total / args.steps) / 1e6, 1),
+        "fft_ms_per_step": round(fft_ms, 3),
+        "fft_algorithmic_GBps": round(sum(alg[k] for k in fft_keys) / (fft_ms * 1e-3) / 1e9, 1) if fft_ms else None,
+        "merkle_ms_per_step": round(merkle_ms, 3),
+        "compressions_per_step": n_compress,
+        "compressions_per_s_step": round(n_compress / (total / args.steps), 1),
+        "compressions_per_s_merkle_kernels": round(n_compress / (merkle_ms * 1e-3), 1) if merkle_ms else None,
+    }
+    log("side rates: %s" % json.dumps(side))
 
     serve = None
     batched = None
@@ -498,6 +516,7 @@ def main():
             "traffic": traffic,
         },
         "valu_roofline": valu_roofline,
+        "side_rates": side,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "batched_uneven": batched,

@@ -21,6 +21,7 @@
 #include "common.hpp"
 #include "field.hpp"
 #include "internal.hpp"
+#include "merkle_dev.hpp"
 
 namespace stwo_hip {
 
@@ -41,18 +42,12 @@ __device__ __forceinline__ uint32_t circle_itwiddle(const uint32_t *itw_end, uin
 
 __device__ __forceinline__ qm31 load_qm31(const uint32_t *p) { return qm31_make(p[0], p[1], p[2], p[3]); }
 
-// `alpha_dev`, when not NULL, holds alpha in device memory (written by the
-// device channel) and replaces the by-value alpha.
-__global__ void __launch_bounds__(256) fold_circle_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
-                                                          uint32_t N, qm31 alpha, qm31 alpha_sq, uint32_t accumulate,
-                                                          const uint32_t *alpha_dev) {
-  const uint32_t n_out = 1u << (N - 1);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_out) return;
-  if (alpha_dev) {
-    alpha = load_qm31(alpha_dev);
-    alpha_sq = qm31_mul(alpha, alpha);
-  }
+// Per-output folds (shared by the standalone fold kernels, the fused FRI
+// layer kernel and the small-layer tail).  Circle: output i of the fold of a
+// circle evaluation of log size N (f0 = f[2i] + f[2i+1], f1 = (f[2i] -
+// f[2i+1]) / p.y, f0 + alpha f1).
+__device__ __forceinline__ qm31 fold_circle_at(const Qm31ColsConst &src, const uint32_t *itw_end, uint32_t N,
+                                               const qm31 &alpha, uint32_t i) {
   const uint32_t iy = circle_itwiddle(itw_end, N, i);
   qm31 f0, f1;
 #pragma unroll
@@ -61,23 +56,15 @@ __global__ void __launch_bounds__(256) fold_circle_kernel(Qm31Cols dst, Qm31Cols
     f0.v[k] = m31_add(v.x, v.y);
     f1.v[k] = m31_mul(m31_sub(v.x, v.y), iy);
   }
-  qm31 r = qm31_add(f0, qm31_mul(alpha, f1));
-  if (accumulate) {
-    qm31 d = qm31_make(dst.c[0][i], dst.c[1][i], dst.c[2][i], dst.c[3][i]);
-    r = qm31_add(qm31_mul(d, alpha_sq), r);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; k++) dst.c[k][i] = r.v[k];
+  return qm31_add(f0, qm31_mul(alpha, f1));
 }
 
+// Line: output i of STEP successive folds of a line evaluation of log size n
+// (the 2^STEP inputs of the output in VGPRs; alpha, alpha^2, ... per fold).
 template <int STEP>
-__global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
-                                                        uint32_t n, qm31 alpha, const uint32_t *alpha_dev) {
+__device__ __forceinline__ qm31 fold_line_at(const Qm31ColsConst &src, const uint32_t *itw_end, uint32_t n,
+                                             qm31 a, uint32_t i) {
   constexpr int W = 1 << STEP;
-  const uint32_t n_out = 1u << (n - STEP);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_out) return;
-  if (alpha_dev) alpha = load_qm31(alpha_dev);
   qm31 v[W];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -85,7 +72,6 @@ __global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsCo
 #pragma unroll
     for (int j = 0; j < W; j++) v[j].v[k] = s[j];
   }
-  qm31 a = alpha;
 #pragma unroll
   for (int layer = 0; layer < STEP; layer++) {
     const int half = W >> (layer + 1);
@@ -103,8 +89,40 @@ __global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsCo
     }
     a = qm31_mul(a, a);
   }
+  return v[0];
+}
+
+// `alpha_dev`, when not NULL, holds alpha in device memory (written by the
+// device channel) and replaces the by-value alpha.
+__global__ void __launch_bounds__(256) fold_circle_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
+                                                          uint32_t N, qm31 alpha, qm31 alpha_sq, uint32_t accumulate,
+                                                          const uint32_t *alpha_dev) {
+  const uint32_t n_out = 1u << (N - 1);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  if (alpha_dev) {
+    alpha = load_qm31(alpha_dev);
+    alpha_sq = qm31_mul(alpha, alpha);
+  }
+  qm31 r = fold_circle_at(src, itw_end, N, alpha, i);
+  if (accumulate) {
+    qm31 d = qm31_make(dst.c[0][i], dst.c[1][i], dst.c[2][i], dst.c[3][i]);
+    r = qm31_add(qm31_mul(d, alpha_sq), r);
+  }
 #pragma unroll
-  for (int k = 0; k < 4; k++) dst.c[k][i] = v[0].v[k];
+  for (int k = 0; k < 4; k++) dst.c[k][i] = r.v[k];
+}
+
+template <int STEP>
+__global__ void __launch_bounds__(256) fold_line_kernel(Qm31Cols dst, Qm31ColsConst src, const uint32_t *itw_end,
+                                                        uint32_t n, qm31 alpha, const uint32_t *alpha_dev) {
+  const uint32_t n_out = 1u << (n - STEP);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  if (alpha_dev) alpha = load_qm31(alpha_dev);
+  const qm31 r = fold_line_at<STEP>(src, itw_end, n, alpha, i);
+#pragma unroll
+  for (int k = 0; k < 4; k++) dst.c[k][i] = r.v[k];
 }
 
 // out = in^(2^k) (alpha for the folds after the first 4 of a longer fold step).
@@ -113,24 +131,6 @@ __global__ void alpha_pow2k_kernel(const uint32_t *in, uint32_t *out, uint32_t k
   qm31 a = load_qm31(in);
   for (uint32_t i = 0; i < k; i++) a = qm31_mul(a, a);
   for (int j = 0; j < 4; j++) out[j] = a.v[j];
-}
-
-// Lifted Merkle leaves of one FRI layer: leaf i = H(le32 of the 4 coordinate
-// values at i), one 16-byte block (CPUB/merkle_lifted.rs:44-78 with four
-// columns of the tree's size).
-__global__ void __launch_bounds__(256) fri_leaves_kernel(Qm31ColsConst src, uint32_t lg, uint32_t *out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >> lg) return;
-  uint32_t m[16];
-#pragma unroll
-  for (int k = 0; k < 4; k++) m[k] = as_global(src.c[k])[i];
-#pragma unroll
-  for (int k = 4; k < 16; k++) m[k] = 0;
-  b2_state s;
-  b2_init(s);
-  b2_compress(s, m, 16, true);
-#pragma unroll
-  for (int k = 0; k < 8; k++) as_global(out)[8 * (size_t)i + k] = s.h[k];
 }
 
 // One Fiat-Shamir step of the FRI layer loop on the device, for the
@@ -179,12 +179,6 @@ __device__ void channel_step(uint32_t *chan, const uint32_t *root, uint32_t *roo
   alpha_out[j] = b2_reduce_word(lo);
 }
 
-__global__ void fri_channel_step_kernel(uint32_t *chan, const uint32_t *root, uint32_t *root_out,
-                                        uint32_t *alpha_out) {
-  if (threadIdx.x >= 4) return;
-  channel_step(chan, root, root_out, alpha_out);
-}
-
 // ---- the small layers in one launch ---------------------------------------
 //
 // Below kTailLog the per-layer work (a few thousand hashes) is far shorter
@@ -212,38 +206,75 @@ struct FriTail {
   uint32_t *chan, *roots, *alpha;
 };
 
-// fold_line_kernel's per-output fold with a run-time step (1..4).
+// Output i of the layer's fold (a run-time step of 1..4 at the call site).
 template <int STEP>
 __device__ __forceinline__ void tail_fold(const FriTailLayer &L, const uint32_t *itw_end, qm31 alpha, uint32_t i) {
-  constexpr int W = 1 << STEP;
-  qm31 v[W];
+  Qm31ColsConst src;
 #pragma unroll
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < 4; k++) src.c[k] = L.src[k];
+  const qm31 r = fold_line_at<STEP>(src, itw_end, L.lg, alpha, i);
 #pragma unroll
-    for (int j = 0; j < W; j++) v[j].v[k] = L.src[k][(size_t)i * W + j];
-  qm31 a = alpha;
+  for (int k = 0; k < 4; k++) L.dst[k][i] = r.v[k];
+}
+
+// Node levels top_log - 1 .. 0 of a packed tree, by one workgroup of
+// kTailThreads lanes: H(left || right), a quad per hash on levels of at most
+// kTailThreads / 4 hashes (their latency is the critical path), a lane per
+// hash above.  Every level is written to the tree and kept in LDS (lv[0] and
+// lv[1] in turn, at most kTailThreads digests each), and the next level reads
+// it from there: the global stores are never waited for.  Level top_log is
+// read from lv[0] when `in_lds` (written there by the caller), else from the
+// tree.  Starts with a barrier; returns the root (LDS, or the tree when
+// top_log = 0 and not in_lds) after a closing barrier.
+__device__ __forceinline__ const uint32_t *tree_levels_to_root(uint32_t *tree, uint32_t top_log,
+                                                               uint32_t (&lv)[2][kTailThreads * 8], bool in_lds) {
+  const uint32_t tid = threadIdx.x, q = tid >> 2, j = tid & 3;
+  uint32_t c = 0;  // lv[c] holds the level below the one being hashed
+  auto level = [&](const uint32_t *child, uint32_t l, uint32_t *keep) {
+    uint32_t *out = tree + tree_layer_off(l);
+    if ((1u << l) <= kTailThreads / 4) {
+      if (q < (1u << l)) {
+        uint32_t m[16];
 #pragma unroll
-  for (int layer = 0; layer < STEP; layer++) {
-    const int half = W >> (layer + 1);
-    const uint32_t *lvl = itw_end - ((int64_t)1 << (L.lg - layer));
-#pragma unroll
-    for (int j = 0; j < half; j++) {
-      const uint32_t xi = lvl[(size_t)i * half + j];
-      qm31 f0, f1;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        f0.v[k] = m31_add(v[2 * j].v[k], v[2 * j + 1].v[k]);
-        f1.v[k] = m31_mul(m31_sub(v[2 * j].v[k], v[2 * j + 1].v[k]), xi);
+        for (int k = 0; k < 16; k++) m[k] = child[16 * (size_t)q + k];
+        uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+        b2_compress_quad(lo, hi, m, 64, true);
+        out[8 * (size_t)q + j] = lo;
+        out[8 * (size_t)q + 4 + j] = hi;
+        keep[8 * q + j] = lo;
+        keep[8 * q + 4 + j] = hi;
       }
-      v[j] = qm31_add(f0, qm31_mul(a, f1));
+      return;
     }
-    a = qm31_mul(a, a);
-  }
+    for (uint32_t i = tid; i < (1u << l); i += kTailThreads) {
+      uint32_t m[16];
 #pragma unroll
-  for (int k = 0; k < 4; k++) L.dst[k][i] = v[0].v[k];
+      for (int k = 0; k < 16; k++) m[k] = child[16 * (size_t)i + k];
+      b2_state s;
+      b2_init(s);
+      b2_compress(s, m, 64, true);
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        out[8 * (size_t)i + k] = s.h[k];
+        keep[8 * i + k] = s.h[k];
+      }
+    }
+  };
+  for (uint32_t l = top_log; l-- > 0;) {
+    __syncthreads();
+    if (!in_lds && l + 1 == top_log) {
+      level(tree + tree_layer_off(l + 1), l, lv[0]);
+      continue;  // the level is now in lv[0] = lv[c]
+    }
+    level(lv[c], l, lv[c ^ 1]);
+    c ^= 1;
+  }
+  __syncthreads();
+  return (top_log == 0 && !in_lds) ? tree : lv[c];
 }
 
 __global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
+  __shared__ uint32_t lv[2][kTailThreads * 8];
   const uint32_t tid = threadIdx.x;
   for (uint32_t li = 0; li < t.n; li++) {
     const FriTailLayer &L = t.l[li];
@@ -264,6 +295,8 @@ __global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
         b2_compress_quad(lo, hi, m, 16, true);
         leaves[8 * (size_t)q + j] = lo;
         leaves[8 * (size_t)q + 4 + j] = hi;
+        lv[0][8 * q + j] = lo;
+        lv[0][8 * q + 4 + j] = hi;
       }
     } else {
       for (uint32_t i = tid; i < n; i += kTailThreads) {
@@ -276,39 +309,15 @@ __global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
         b2_init(s);
         b2_compress(s, m, 16, true);
 #pragma unroll
-        for (int k = 0; k < 8; k++) leaves[8 * (size_t)i + k] = s.h[k];
-      }
-    }
-    // node layers lg - 1 .. 0: H(left || right)
-    for (uint32_t l = L.lg; l-- > 0;) {
-      __syncthreads();
-      const uint32_t *child = L.tree + tree_layer_off(l + 1);
-      uint32_t *out = L.tree + tree_layer_off(l);
-      if ((1u << l) <= kTailThreads / 4) {
-        if (q < (1u << l)) {
-          uint32_t m[16];
-#pragma unroll
-          for (int k = 0; k < 16; k++) m[k] = child[16 * (size_t)q + k];
-          uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
-          b2_compress_quad(lo, hi, m, 64, true);
-          out[8 * (size_t)q + j] = lo;
-          out[8 * (size_t)q + 4 + j] = hi;
+        for (int k = 0; k < 8; k++) {
+          leaves[8 * (size_t)i + k] = s.h[k];
+          lv[0][8 * i + k] = s.h[k];
         }
-        continue;
-      }
-      for (uint32_t i = tid; i < (1u << l); i += kTailThreads) {
-        uint32_t m[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) m[k] = child[16 * (size_t)i + k];
-        b2_state s;
-        b2_init(s);
-        b2_compress(s, m, 64, true);
-#pragma unroll
-        for (int k = 0; k < 8; k++) out[8 * (size_t)i + k] = s.h[k];
       }
     }
-    __syncthreads();
-    if (tid < 4) channel_step(t.chan, L.tree, t.roots + 8 * L.root_slot, t.alpha);
+    // node layers lg - 1 .. 0
+    const uint32_t *root = tree_levels_to_root(L.tree, L.lg, lv, true);
+    if (tid < 4) channel_step(t.chan, root, t.roots + 8 * L.root_slot, t.alpha);
     __syncthreads();
     const qm31 alpha = qm31_make(t.alpha[0], t.alpha[1], t.alpha[2], t.alpha[3]);
     const uint32_t n_out = 1u << (L.lg - L.step);
@@ -322,6 +331,67 @@ __global__ void __launch_bounds__(kTailThreads) fri_tail_kernel(FriTail t) {
     }
     __syncthreads();
   }
+}
+
+// ---- one FRI layer in two launches --------------------------------------
+//
+// Above kTailLog a layer is committed by fri_layer_kernel, which also runs
+// the fold that produces the layer (from the previous layer, with alpha from
+// the device channel), then hashes its lifted leaves and the first node
+// levels of every 256-leaf subtree; the remaining levels and the channel
+// step on the root run in fri_top_kernel (after merkle_nodes launches for
+// levels wider than 2^kTopLog).  The trees, roots, channel state and folded
+// evaluations written are the ones the separate fold, leaf, node and channel
+// launches write.
+constexpr int kFoldCircle = 5;    // fri_layer_kernel's FOLD for the circle fold
+constexpr uint32_t kTopLog = 10;  // fri_top_kernel starts at a level of at most 2^kTopLog nodes
+
+// FOLD = 0: the layer's values are src at i.  FOLD = 1..4: line folds of
+// that many steps from a layer of log lg + FOLD.  kFoldCircle: the circle
+// fold from log lg + 1.  A fold writes the layer to dst.  n_sub <= min(8, lg)
+// node levels follow the leaves in each workgroup.
+template <int FOLD>
+__global__ void __launch_bounds__(256) fri_layer_kernel(Qm31ColsConst src, Qm31Cols dst, const uint32_t *itw_end,
+                                                        const uint32_t *alpha_dev, uint32_t lg, uint32_t *tree,
+                                                        uint32_t n_sub) {
+  __shared__ uint32_t hs[256][9];
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < (1u << lg)) {
+    qm31 v;
+    if constexpr (FOLD == 0) {
+      v = qm31_make(as_global(src.c[0])[i], as_global(src.c[1])[i], as_global(src.c[2])[i], as_global(src.c[3])[i]);
+    } else {
+      const qm31 alpha = load_qm31(alpha_dev);
+      if constexpr (FOLD == kFoldCircle)
+        v = fold_circle_at(src, itw_end, lg + 1, alpha, i);
+      else
+        v = fold_line_at<FOLD>(src, itw_end, lg + FOLD, alpha, i);
+#pragma unroll
+      for (int k = 0; k < 4; k++) as_global(dst.c[k])[i] = v.v[k];
+    }
+    // lifted leaf: H(le32 of the 4 coordinates), one 16-byte block
+    uint32_t m[16];
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[k] = v.v[k];
+#pragma unroll
+    for (int k = 4; k < 16; k++) m[k] = 0;
+    b2_state s;
+    b2_init(s);
+    b2_compress(s, m, 16, true);
+    store_hash(tree + tree_layer_off(lg) + 8 * (size_t)i, s, 0);
+#pragma unroll
+    for (int k = 0; k < 8; k++) hs[threadIdx.x][k] = s.h[k];
+  }
+  subtree_levels(hs, tree, lg, n_sub, 0);
+}
+
+// Levels top_log - 1 .. 0 (level top_log - 1 of at most 2^kTopLog nodes),
+// then the channel step on the root.
+__global__ void __launch_bounds__(kTailThreads) fri_top_kernel(uint32_t *tree, uint32_t top_log, uint32_t *chan,
+                                                               uint32_t *root_out, uint32_t *alpha_out) {
+  __shared__ uint32_t lv[2][kTailThreads * 8];
+  const uint32_t *root = tree_levels_to_root(tree, top_log, lv, false);
+  if (threadIdx.x < 4) channel_step(chan, root, root_out, alpha_out);
 }
 
 static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
@@ -516,9 +586,9 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
   if (keep && keep->layers.size() != sched.size())
     return set_error(ctx, STWO_HIP_EINVAL, "internal: FRI prover storage does not match the schedule");
   // The layer loop runs on the stream without host round trips: after each
-  // layer's Merkle commit, fri_channel_step_kernel mixes the root into the
-  // device copy of the channel and draws the next alpha, which the fold
-  // kernels read from device memory.  The host reads the roots, the channel
+  // layer's Merkle commit, the kernel that hashes its root (fri_top_kernel,
+  // or the tail) mixes the root into the device copy of the channel and
+  // draws the next alpha, which the folds read from device memory.  The host reads the roots, the channel
   // and the last layer once, at the end.
   constexpr uint32_t kMaxRoots = 32;
   // chan[0..10) | alpha[10..14) | alpha_tmp[14..18) | pad | roots[32 ..)
@@ -549,31 +619,74 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
   }
   uint32_t nr = 0;
 
-  auto commit_root = [&](const uint32_t *const cols[4], uint32_t lg) -> int {
-    uint32_t logs[4] = {lg, lg, lg, lg};
+  // A fold not launched yet: the circle fold after the first layer or a line
+  // fold of `step` steps.  The next layer's commit runs it in its own launch
+  // (fri_layer_kernel) when the step is at most 4; otherwise, and before the
+  // tail or the end, it runs on its own.
+  struct PendingFold {
+    bool on, circle;
+    uint32_t step, src_log;
+    const uint32_t *src[4];
+    uint32_t *dst[4];
+  } pend{};
+  const qm31 unused = qm31_make(0, 0, 0, 0);
+  auto flush = [&]() -> int {
+    if (!pend.on) return STWO_HIP_OK;
+    pend.on = false;
+    if (pend.circle) return fold_circle(ctx, pend.dst, pend.src, pend.src_log, unused, tw, false, d_alpha);
+    return fold_line_steps(ctx, pend.src, pend.src_log, unused, pend.step, tw, pend.dst, d_alpha, d_alpha_tmp);
+  };
+
+  // Commits the layer `cols` of log lg (running the pending fold that writes
+  // it, if it fuses), then mixes its root into the channel and draws alpha.
+  auto commit_layer = [&](const uint32_t *const cols[4], uint32_t lg) -> int {
     if (keep && nr >= keep->layers.size()) return set_error(ctx, STWO_HIP_EINVAL, "internal: FRI layer overflow");
     uint32_t *tree = keep ? keep->layers[nr].tree : scratch_tree;
     if (keep)
       for (int k = 0; k < 4; k++) keep->layers[nr].cols[k] = cols[k];
-    (void)logs;
-    // Lifted leaves of the 4 equal-size coordinate columns (pointers by
-    // value: no table upload), then the node layers.
-    Qm31ColsConst c4;
-    for (int k = 0; k < 4; k++) c4.c[k] = cols[k];
-    int r = launch(ctx, "merkle_lifted_leaves", fri_leaves_kernel, dim3((uint32_t)(((1ull << lg) + 255) / 256)),
-                   dim3(256), 0, c4, lg, tree + tree_layer_off(lg));
+    Qm31ColsConst src;
+    Qm31Cols dst{};
+    int fold = 0;
+    if (pend.on && (pend.circle || pend.step <= 4)) {
+      fold = pend.circle ? kFoldCircle : (int)pend.step;
+      for (int k = 0; k < 4; k++) {
+        src.c[k] = pend.src[k];
+        dst.c[k] = pend.dst[k];
+      }
+      pend.on = false;
+    } else {
+      int r = flush();
+      if (r) return r;
+      for (int k = 0; k < 4; k++) src.c[k] = cols[k];
+    }
+    const uint32_t n_sub = std::min<uint32_t>(8, lg);
+    const dim3 grid((uint32_t)(((1ull << lg) + 255) / 256)), block(256);
+    const uint32_t *itw_end = tw->d_itw + tw->len;
+    int r;
+    switch (fold) {
+      case 0: r = launch(ctx, "fri_layer", fri_layer_kernel<0>, grid, block, 0, src, dst, itw_end, (const uint32_t *)d_alpha, lg, tree, n_sub); break;
+      case 1: r = launch(ctx, "fri_layer", fri_layer_kernel<1>, grid, block, 0, src, dst, itw_end, (const uint32_t *)d_alpha, lg, tree, n_sub); break;
+      case 2: r = launch(ctx, "fri_layer", fri_layer_kernel<2>, grid, block, 0, src, dst, itw_end, (const uint32_t *)d_alpha, lg, tree, n_sub); break;
+      case 3: r = launch(ctx, "fri_layer", fri_layer_kernel<3>, grid, block, 0, src, dst, itw_end, (const uint32_t *)d_alpha, lg, tree, n_sub); break;
+      case 4: r = launch(ctx, "fri_layer", fri_layer_kernel<4>, grid, block, 0, src, dst, itw_end, (const uint32_t *)d_alpha, lg, tree, n_sub); break;
+      default: r = launch(ctx, "fri_layer", fri_layer_kernel<kFoldCircle>, grid, block, 0, src, dst, itw_end, (const uint32_t *)d_alpha, lg, tree, n_sub); break;
+    }
     if (r) return r;
-    r = hash_node_layers(ctx, tree, lg, 0, STWO_HIP_HASH_BLAKE2S);
-    if (r) return r;
-    r = launch(ctx, "fri_channel_step", fri_channel_step_kernel, dim3(1), dim3(64), 0, d_chan, (const uint32_t *)tree,
-               d_roots + 8 * nr, d_alpha);
+    uint32_t top = lg - n_sub;
+    if (top > kTopLog + 1) {
+      r = hash_node_layers(ctx, tree, top, kTopLog + 1, STWO_HIP_HASH_BLAKE2S);
+      if (r) return r;
+      top = kTopLog + 1;
+    }
+    r = launch(ctx, "fri_top", fri_top_kernel, dim3(1), dim3(kTailThreads), 0, tree, top, d_chan, d_roots + 8 * nr,
+               d_alpha);
     if (r) return r;
     nr++;
     return STWO_HIP_OK;
   };
 
   // First layer (fri.rs:176-183, 360-369).
-  rc = commit_root(column, log_size);
+  rc = commit_layer(column, log_size);
   if (rc) return rc;
   // Inner layers (fri.rs:188-262).
   uint32_t lg = log_size - 1;
@@ -586,14 +699,21 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
   };
   for (int k = 0; k < 4; k++) cur[k] = layer_buf(0, lg, k);
   int n_folds = 1;
-  const qm31 unused = qm31_make(0, 0, 0, 0);
-  rc = fold_circle(ctx, cur, column, log_size, unused, tw, false, d_alpha);
-  if (rc) return rc;
+  pend.on = true;
+  pend.circle = true;
+  pend.step = 1;
+  pend.src_log = log_size;
+  for (int k = 0; k < 4; k++) {
+    pend.src[k] = column[k];
+    pend.dst[k] = cur[k];
+  }
   if (lg != last_log) {
     for (;;) {
       bool last = !(lg > last_log + cfg.line_fold_step);
       uint32_t step = last ? lg - last_log : cfg.line_fold_step;
       if (lg <= kTailLog && cfg.line_fold_step <= 4) {
+        rc = flush();
+        if (rc) return rc;
         // Every remaining inner layer (at most kTailLog of them) in one launch.
         FriTail t{};
         t.itw_end = tw->d_itw + tw->len;
@@ -625,19 +745,26 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
         if (rc) return rc;
         break;
       }
-      rc = commit_root(cur, lg);
+      rc = commit_layer(cur, lg);
       if (rc) return rc;
-      uint32_t *nxt[4];
-      for (int k = 0; k < 4; k++) nxt[k] = layer_buf(n_folds, lg - step, k);
-      rc = fold_line_steps(ctx, cur, lg, unused, step, tw, nxt, d_alpha, d_alpha_tmp);
-      if (rc) return rc;
+      pend.on = true;
+      pend.circle = false;
+      pend.step = step;
+      pend.src_log = lg;
+      for (int k = 0; k < 4; k++) {
+        pend.src[k] = cur[k];
+        pend.dst[k] = layer_buf(n_folds, lg - step, k);
+        cur[k] = pend.dst[k];
+      }
       n_folds++;
-      for (int k = 0; k < 4; k++) cur[k] = nxt[k];
       line_init = (uint32_t)(((uint64_t)line_init << step) & IDX_MASK);
       lg -= step;
       if (last) break;
     }
   }
+  // The fold into the last layer when no tail ran (or the circle fold alone).
+  rc = flush();
+  if (rc) return rc;
   // One synchronization for the channel state, the roots and the last layer.
   std::vector<uint32_t> st(kRootsOff + 8 * nr);
   size_t n = (size_t)1 << lg;

@@ -25,22 +25,10 @@
 #include "common.hpp"
 #include "field.hpp"
 #include "internal.hpp"
+#include "merkle_dev.hpp"
 
 namespace stwo_hip {
 
-
-__device__ __forceinline__ void store_hash(uint32_t *dst, const b2_state &s, uint32_t m31) {
-  uint4 a, b;
-  if (m31) {
-    a = make_uint4(b2_reduce_word(s.h[0]), b2_reduce_word(s.h[1]), b2_reduce_word(s.h[2]), b2_reduce_word(s.h[3]));
-    b = make_uint4(b2_reduce_word(s.h[4]), b2_reduce_word(s.h[5]), b2_reduce_word(s.h[6]), b2_reduce_word(s.h[7]));
-  } else {
-    a = make_uint4(s.h[0], s.h[1], s.h[2], s.h[3]);
-    b = make_uint4(s.h[4], s.h[5], s.h[6], s.h[7]);
-  }
-  reinterpret_cast<uint4 *>(dst)[0] = a;
-  reinterpret_cast<uint4 *>(dst)[1] = b;
-}
 
 __device__ __forceinline__ void load_block64(uint32_t m[16], const uint32_t *src) {
   const uint4 *s = reinterpret_cast<const uint4 *>(src);
@@ -114,60 +102,7 @@ __global__ void __launch_bounds__(256) merkle_nodes_kernel(const uint32_t *prev,
 #pragma unroll
     for (int k = 0; k < 8; k++) hs[threadIdx.x][k] = s.h[k];
   }
-  uint32_t active = min(256u, n);
-  for (uint32_t layer = 1; layer < n_layers; layer++) {
-    __syncthreads();
-    active >>= 1;
-    uint32_t lg = out_log - layer;
-    if (active <= 64) {
-      // Few hashes left: a quad per hash (b2_compress_quad) shortens the
-      // critical path of the remaining layers.
-      const uint32_t q = threadIdx.x >> 2, j = threadIdx.x & 3;
-      const bool work = q < active;
-      uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
-      if (work) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          m[k] = hs[2 * q][k];
-          m[8 + k] = hs[2 * q + 1][k];
-        }
-        b2_compress_quad(lo, hi, m, 64, true);
-        if (m31) {
-          lo = b2_reduce_word(lo);
-          hi = b2_reduce_word(hi);
-        }
-        g_u32 *o = as_global(packed + (((size_t)1 << lg) - 1) * 8 + 8 * ((size_t)blockIdx.x * (256u >> layer) + q));
-        o[j] = lo;
-        o[4 + j] = hi;
-      }
-      __syncthreads();
-      if (work) {
-        hs[q][j] = lo;
-        hs[q][4 + j] = hi;
-      }
-      continue;
-    }
-    uint32_t gi = blockIdx.x * (256u >> layer) + threadIdx.x;
-    b2_state s2;
-    bool work = threadIdx.x < active;
-    if (work) {
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        m[k] = hs[2 * threadIdx.x][k];
-        m[8 + k] = hs[2 * threadIdx.x + 1][k];
-      }
-      b2_init(s2);
-      b2_compress(s2, m, 64, true);
-      if (m31)
-#pragma unroll
-        for (int k = 0; k < 8; k++) s2.h[k] = b2_reduce_word(s2.h[k]);
-      store_hash(packed + (((size_t)1 << lg) - 1) * 8 + 8 * (size_t)gi, s2, 0);
-    }
-    __syncthreads();
-    if (work)
-#pragma unroll
-      for (int k = 0; k < 8; k++) hs[threadIdx.x][k] = s2.h[k];
-  }
+  subtree_levels(hs, packed, out_log, n_layers - 1, m31);
 }
 
 static inline size_t layer_off(uint32_t l) { return tree_layer_off(l); }
