@@ -82,14 +82,39 @@ __global__ void __launch_bounds__(256) eval_point_dot_kernel(const EvalCol *cols
     b[i] = t < nb ? qm31_make(basis[4 * t], basis[4 * t + 1], basis[4 * t + 2], basis[4 * t + 3])
                   : qm31_make(0, 0, 0, 0);
   }
-  for (uint32_t j = 0; j < w.n; j++) {
-    const uint32_t *c = cols[w.col0 + j].coeffs + ((size_t)w.block << B);
+  // Every coefficient load of the workgroup is issued before the first
+  // product (one memory latency per workgroup rather than one per column):
+  // 8 words per lane and column, as two 16-byte loads when the block is
+  // whole and the column pointer 32-byte aligned (a wave-uniform test).
+  uint32_t cv[kDotCols][8];
+#pragma unroll
+  for (uint32_t j = 0; j < kDotCols; j++) {
+    if (j < w.n) {
+      const uint32_t *c = cols[w.col0 + j].coeffs + ((size_t)w.block << B);
+      if (nb == (1u << kEvalBlockLog) && ((uintptr_t)c & 31) == 0) {
+        const u32x4 lo = *(gconst_u4 *)(as_global(c) + t0), hi = *(gconst_u4 *)(as_global(c) + t0 + 4);
+        cv[j][0] = lo.x;
+        cv[j][1] = lo.y;
+        cv[j][2] = lo.z;
+        cv[j][3] = lo.w;
+        cv[j][4] = hi.x;
+        cv[j][5] = hi.y;
+        cv[j][6] = hi.z;
+        cv[j][7] = hi.w;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) cv[j][i] = (t0 + i < nb) ? as_global(c)[t0 + i] : 0;
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kDotCols; j++) {
+    if (j >= w.n) break;
     uint64_t acc[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const uint32_t v = (t0 + i < nb) ? as_global(c)[t0 + i] : 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) acc[k] += (uint64_t)v * b[i].v[k];
+      for (int k = 0; k < 4; k++) acc[k] += (uint64_t)cv[j][i] * b[i].v[k];
       if (i == 2 || i == 5)
 #pragma unroll
         for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k]);

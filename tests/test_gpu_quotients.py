@@ -137,6 +137,29 @@ def test_eval_at_point_grouped_and_extreme_values(ctx):
         x.free()
 
 
+def test_eval_at_point_unaligned_columns(ctx):
+    """Column pointers at word offsets 1, 3, 4 and 8 of a buffer (not 32-byte
+    aligned: whole 2^11 blocks then take the per-word loads) beside aligned
+    ones in the same workgroups; bit-exact against the oracle."""
+    rng = np.random.default_rng(13)
+    logs = [13, 13, 13, 13, 13, 12, 11]
+    offs = [1, 0, 3, 4, 8, 0, 5]
+    cols = [rng.integers(0, P, 1 << lg, dtype=np.uint64).astype(U) for lg in logs]
+    bufs = []
+    ptrs = []
+    for c, o in zip(cols, offs):
+        b = ctx.alloc(len(c) + 8)
+        b.upload(c, offset=o)
+        bufs.append(b)
+        ptrs.append(b.at(o))
+    pt = secure_point(0xA11)
+    got = ctx.eval_at_point(ptrs, logs, pt)
+    for c, g in zip(cols, got):
+        assert tuple(int(v) for v in g) == O.eval_at_point(c, pt)
+    for b in bufs:
+        b.free()
+
+
 @pytest.mark.parametrize("log_n,n_cols", [(5, 8), (12, 100), (16, 256)])
 def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols):
     """Trace generation and the constraint quotients, bit-exact against the
