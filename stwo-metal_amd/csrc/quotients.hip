@@ -78,9 +78,26 @@ __global__ void __launch_bounds__(256) quotient_kernel(const QuotPoint *pts, uin
     const uint32_t ninv = m31_inv(m31_add(m31_mul(d0, d0), m31_mul(d1, d1)));
     const uint32_t i0 = m31_mul(d0, ninv), i1 = m31_mul(m31_neg(d1), ninv);
     // sum (c_i f~_i - b_i) = sum c_i f~_i - sum b_i: the products go into
-    // 64-bit accumulators, folded after every third entry.
+    // 64-bit accumulators, folded after every third entry.  Six entries'
+    // loads are issued together.
     uint64_t acc[4] = {0, 0, 0, 0};
     uint32_t e = z.e_begin;
+    for (; e + 6 <= z.e_end; e += 6) {
+      uint32_t v[6];
+#pragma unroll
+      for (int u = 0; u < 6; u++) {
+        const QuotEntry &en = ents[e + u];
+        v[u] = as_global(en.col)[((row >> (en.shift + 1)) << 1) + (row & 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < 6; u++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[k] += (uint64_t)v[u] * ents[e + u].c[k];
+        if (u == 2 || u == 5)
+#pragma unroll
+          for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k]);
+      }
+    }
     for (; e + 3 <= z.e_end; e += 3) {
 #pragma unroll
       for (int u = 0; u < 3; u++) {

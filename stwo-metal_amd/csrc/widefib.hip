@@ -45,7 +45,25 @@ __global__ void __launch_bounds__(256) wide_fib_trace_kernel(uint32_t *const *co
 //   sum_j w_j (x_{j+2} - x_j^2 - x_{j+1}^2) = sum_i (x_i W1_i + x_i^2 W2_i),
 // W1_i = w_{i-2} (i >= 2) and W2_i = -(w_i [i <= N-3] + w_{i-1} [1 <= i <= N-2]),
 // so each column costs one square and 8 multiply-adds into 64-bit
-// accumulators (folded after each column: two products per limb).
+// accumulators: sum x W1 and sum x^2 W2 apart, one product per column each,
+// folded after every third column (three products below 2^62 fit above a
+// folded value below 2^34).  Columns go 8 at a time, their loads issued
+// together.
+__device__ __forceinline__ void wf_column(uint64_t (&a1)[4], uint64_t (&a2)[4], uint32_t x, const uint32_t *w) {
+  const uint32_t sq = m31_mul(x, x);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    a1[k] += (uint64_t)x * w[k];  // W1 (words 0..3)
+    a2[k] += (uint64_t)sq * w[4 + k];  // W2 (words 4..7)
+  }
+}
+__device__ __forceinline__ void wf_fold(uint64_t (&a1)[4], uint64_t (&a2)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    a1[k] = m31_fold64(a1[k]);
+    a2[k] = m31_fold64(a2[k]);
+  }
+}
 __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t *const *cols, uint32_t n_cols,
                                                                  uint32_t eval_log, uint32_t log_n,
                                                                  const uint32_t *weights, const uint32_t *denom_inv,
@@ -53,15 +71,25 @@ __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t 
                                                                  uint32_t *acc3) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >> eval_log) return;
-  uint64_t acc[4] = {0, 0, 0, 0};
-  for (uint32_t i = 0; i < n_cols; i++) {
-    const uint32_t x = as_global(cols[i])[r];
-    const uint32_t sq = m31_mul(x, x);
-    const uint32_t *w = weights + 8 * i;  // W1 (4 words), W2 (4 words)
+  uint64_t a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0};
+  uint32_t i = 0;
+  for (; i + 8 <= n_cols; i += 8) {
+    uint32_t x[8];
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k] + (uint64_t)x * w[k] + (uint64_t)sq * w[4 + k]);
+    for (int u = 0; u < 8; u++) x[u] = as_global(cols[i + u])[r];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      wf_column(a1, a2, x[u], weights + 8 * (i + u));
+      if (u == 2 || u == 5 || u == 7) wf_fold(a1, a2);
+    }
   }
-  qm31 res = qm31_make(m31_reduce64(acc[0]), m31_reduce64(acc[1]), m31_reduce64(acc[2]), m31_reduce64(acc[3]));
+  for (; i < n_cols; i++) {
+    wf_column(a1, a2, as_global(cols[i])[r], weights + 8 * i);
+    wf_fold(a1, a2);
+  }
+  qm31 res;
+#pragma unroll
+  for (int k = 0; k < 4; k++) res.v[k] = m31_add(m31_reduce64(a1[k]), m31_reduce64(a2[k]));
   res = qm31_mul_m31(res, denom_inv[r >> log_n]);
   acc0[r] = m31_add(acc0[r], res.v[0]);
   acc1[r] = m31_add(acc1[r], res.v[1]);
