@@ -51,6 +51,7 @@ struct Staging {
 
 struct LevelLaunch {
   bool inverse;
+  bool uniform;  // every group's tile is 2^10 or more
   size_t groups_off, wgs_off;  // offsets into the device buffer
   uint32_t n_groups, total_wgs, wg_log;
 };
@@ -179,10 +180,12 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
       std::vector<FftPass2> descs;
       std::vector<uint32_t> starts;
       uint64_t acc = 0;
+      bool uniform = true;
       for (auto &pl : v) {
         descs.push_back(pl.p);
         starts.push_back((uint32_t)acc);
         acc += pl.wgs;
+        if (pl.p.T < 10) uniform = false;
       }
       if (acc >= (1ull << 31)) {
         (void)hipFree(dmem);
@@ -195,6 +198,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
       ll.n_groups = (uint32_t)descs.size();
       ll.total_wgs = (uint32_t)acc;
       ll.wg_log = key.second;
+      ll.uniform = uniform;
       levels.push_back(ll);
     }
   }
@@ -219,7 +223,8 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   b->issue = [levels, lt, dbase, layers, max_log, mode, kind](stwo_hip_ctx *c) -> int {
     for (const LevelLaunch &ll : levels) {
       int rc = launch_fft_multi(c, ll.inverse, (const FftPass2 *)(dbase + ll.groups_off),
-                                (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs, ll.wg_log);
+                                (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs, ll.wg_log,
+                                ll.uniform);
       if (rc) return rc;
     }
     if (kind == STWO_HIP_TREE_LIFTED) {

@@ -614,15 +614,16 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
 
 // Several passes of different column groups (sizes) in one launch: group g
 // owns workgroups [wg_start[g], wg_start[g + 1]).  Used by the batched path,
-// where every column size contributes its pass of the same level.
-template <bool INV, int WG_LOG>
+// where every column size contributes its pass of the same level.  NS = -1
+// (wave-uniform columns) when every group's tile is 2^10 or more.
+template <bool INV, int WG_LOG, int NS>
 __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(5))) fft_pass_multi_kernel(const FftPass2 *groups,
                                                                     const uint32_t *wg_start, uint32_t n_groups) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
   uint32_t g = 0;
   while (g + 1 < n_groups && wg_start[g + 1] <= blockIdx.x) g++;
   const FftPass2 p = groups[g];
-  fft_pass_body<INV, WG_LOG, 0, 0, 0, 0>(p, blockIdx.x - wg_start[g], lds);
+  fft_pass_body<INV, WG_LOG, NS, 0, 0, 0>(p, blockIdx.x - wg_start[g], lds);
 }
 
 // Stages covering layer tile bits [A, B) of a tile of T bits, forward order.
@@ -807,16 +808,20 @@ int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t
 }
 
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
-                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log) {
-  if (wg_log == 9)
-    return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 9>, dim3(total_wgs), dim3(512), 0,
-                            d_groups, d_wg_start, n_groups)
-                   : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 9>, dim3(total_wgs), dim3(512), 0,
-                            d_groups, d_wg_start, n_groups);
-  return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 8>, dim3(total_wgs), dim3(256), 0,
-                          d_groups, d_wg_start, n_groups)
-                 : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 8>, dim3(total_wgs), dim3(256), 0,
-                          d_groups, d_wg_start, n_groups);
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, bool uniform) {
+  const dim3 grid(total_wgs), b9(512), b8(256);
+  if (wg_log == 9) {
+    if (uniform)
+      return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 9, -1>, grid, b9, 0, d_groups, d_wg_start, n_groups)
+                     : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 9, -1>, grid, b9, 0, d_groups, d_wg_start, n_groups);
+    return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 9, 0>, grid, b9, 0, d_groups, d_wg_start, n_groups)
+                   : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 9, 0>, grid, b9, 0, d_groups, d_wg_start, n_groups);
+  }
+  if (uniform)
+    return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 8, -1>, grid, b8, 0, d_groups, d_wg_start, n_groups)
+                   : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 8, -1>, grid, b8, 0, d_groups, d_wg_start, n_groups);
+  return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 8, 0>, grid, b8, 0, d_groups, d_wg_start, n_groups)
+                 : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 8, 0>, grid, b8, 0, d_groups, d_wg_start, n_groups);
 }
 
 // Radix-16 path for N >= 4: one launch per pass over all columns of the group.

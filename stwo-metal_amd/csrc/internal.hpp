@@ -51,8 +51,9 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
                       const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
                       std::vector<PassLaunch> &out);
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log);
+// `uniform`: every group's tile is 2^10 or more (the wave-uniform-column kernel).
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
-                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log);
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, bool uniform);
 
 // ---- decommitment: batched device gather -----------------------------------
 

@@ -44,16 +44,37 @@ __device__ __forceinline__ void load_block64(uint32_t m[16], const uint32_t *src
 
 // Mixed-degree layer: optional previous layer, then the layer's columns.
 // `lifted` selects the lifted leaf index for column words.
+// AHEAD: each block's column words are loaded one compression ahead, so
+// their latency overlaps the previous compression.  Used for layers of few
+// nodes, where a node's chain of blocks is the launch's critical path; on
+// large layers the other waves hide the latency and the extra registers and
+// copies cost more than they save.
+template <bool AHEAD>
 __global__ void __launch_bounds__(256) merkle_layer_kernel(const uint32_t *prev, const ColRef *cols, uint32_t n_cols,
                                                            uint32_t log_size, uint32_t *out, uint32_t m31,
                                                            uint32_t lifted) {
   const uint32_t n = 1u << log_size;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // The words of columns c0 .. c0 + 15 at this row (zeros past n_cols).
+  auto load_cols = [&](uint32_t c0, uint32_t (&dst)[16]) {
+    const uint32_t cnt = min(16u, n_cols - c0);
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+      uint32_t v = 0;
+      if (k < cnt) {
+        const ColRef cr = cols[c0 + k];
+        uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
+        v = as_global(cr.p)[idx];
+      }
+      dst[k] = v;
+    }
+  };
   b2_state s;
   b2_init(s);
-  uint32_t m[16];
+  uint32_t m[16], nx[16];
   uint32_t t = 0;
+  if (AHEAD && n_cols) load_cols(0, nx);
   if (prev) {
     load_block64(m, prev + 16 * (size_t)i);
     t = 64;
@@ -65,15 +86,12 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(const uint32_t *prev,
   }
   for (uint32_t c0 = 0; c0 < n_cols; c0 += 16) {
     const uint32_t cnt = min(16u, n_cols - c0);
+    if (AHEAD) {
 #pragma unroll
-    for (uint32_t k = 0; k < 16; k++) {
-      uint32_t v = 0;
-      if (k < cnt) {
-        const ColRef cr = cols[c0 + k];
-        uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
-        v = as_global(cr.p)[idx];
-      }
-      m[k] = v;
+      for (int k = 0; k < 16; k++) m[k] = nx[k];
+      if (c0 + 16 < n_cols) load_cols(c0 + 16, nx);
+    } else {
+      load_cols(c0, m);
     }
     t += 4 * cnt;
     b2_compress(s, m, t, c0 + 16 >= n_cols);
@@ -110,9 +128,12 @@ static inline size_t layer_off(uint32_t l) { return tree_layer_off(l); }
 int launch_merkle_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const ColRef *d_cols, uint32_t n_cols,
                         uint32_t log_size, uint32_t *out, int mode, bool lifted) {
   uint32_t n = 1u << log_size;
-  return launch(ctx, lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer", merkle_layer_kernel,
-                dim3((n + 255) / 256), dim3(256), 0, prev, d_cols, n_cols, log_size, out,
-                (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31), (uint32_t)lifted);
+  // Below 2^16 nodes (256 workgroups) a layer with more than one block of
+  // columns is latency-bound: load ahead.
+  const bool ahead = log_size < 16 && n_cols > 16;
+  return launch(ctx, lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer",
+                ahead ? merkle_layer_kernel<true> : merkle_layer_kernel<false>, dim3((n + 255) / 256), dim3(256), 0,
+                prev, d_cols, n_cols, log_size, out, (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31), (uint32_t)lifted);
 }
 
 static int launch_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const std::vector<ColRef> &cols, uint32_t log_size,

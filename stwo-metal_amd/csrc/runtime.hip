@@ -49,14 +49,14 @@ void prof_begin(stwo_hip_ctx *ctx, const char *name, hipEvent_t *start) {
   *start = nullptr;
   if (!ctx->prof) return;
   *start = take_event(ctx);
-  if (*start) hipEventRecord(*start, ctx->stream);
+  if (*start) (void)hipEventRecord(*start, ctx->stream);
 }
 
 void prof_end(stwo_hip_ctx *ctx, const char *name, hipEvent_t start) {
   if (!ctx->prof || !start) return;
   hipEvent_t stop = take_event(ctx);
   if (!stop) return;
-  hipEventRecord(stop, ctx->stream);
+  (void)hipEventRecord(stop, ctx->stream);
   ctx->prof_pending.push_back({name, start, stop});
 }
 
@@ -149,7 +149,7 @@ int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr) {
   }
   if (it != ctx->scratch.end()) {
     STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    hipFree(it->second.first);
+    (void)hipFree(it->second.first);
     ctx->scratch.erase(it);
   }
   void *p = nullptr;
@@ -202,7 +202,7 @@ int stwo_hip_ctx_create(int device, stwo_hip_ctx **out) {
     return STWO_HIP_EDEVICE;
   }
   if (arena_init(ctx) != STWO_HIP_OK) {
-    hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return STWO_HIP_ENOMEM;
   }
@@ -212,19 +212,19 @@ int stwo_hip_ctx_create(int device, stwo_hip_ctx **out) {
 
 int stwo_hip_ctx_destroy(stwo_hip_ctx *ctx) {
   if (!ctx) return STWO_HIP_EINVAL;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  for (auto &f : ctx->inflight) hipEventDestroy(f.ev);
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &f : ctx->inflight) (void)hipEventDestroy(f.ev);
   for (auto &p : ctx->prof_pending) {
-    hipEventDestroy(p.start);
-    hipEventDestroy(p.stop);
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
   }
-  for (auto e : ctx->event_pool) hipEventDestroy(e);
-  for (auto &kv : ctx->scratch) hipFree(kv.second.first);
-  if (ctx->h_arena) hipHostFree(ctx->h_arena);
-  if (ctx->h_stage) hipHostFree(ctx->h_stage);
-  if (ctx->d_arena) hipFree(ctx->d_arena);
-  hipStreamDestroy(ctx->stream);
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  for (auto &kv : ctx->scratch) (void)hipFree(kv.second.first);
+  if (ctx->h_arena) (void)hipHostFree(ctx->h_arena);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+  if (ctx->d_arena) (void)hipFree(ctx->d_arena);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return STWO_HIP_OK;
 }
@@ -251,7 +251,7 @@ int stwo_hip_alloc(stwo_hip_ctx *ctx, size_t n_words, uint32_t **out) {
   if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, STWO_HIP_EDEVICE, "hipSetDevice failed");
   hipError_t e = hipMalloc((void **)out, (n_words ? n_words : 1) * sizeof(uint32_t));
   if (e != hipSuccess) {
-    hipGetLastError();
+    (void)hipGetLastError();
     *out = nullptr;
     return set_error(ctx, STWO_HIP_ENOMEM, "hipMalloc(%zu words): %s", n_words, hipGetErrorString(e));
   }
@@ -315,7 +315,7 @@ static int prof_collect(stwo_hip_ctx *ctx) {
   STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (auto &p : ctx->prof_pending) {
     float ms = 0;
-    hipEventElapsedTime(&ms, p.start, p.stop);
+    (void)hipEventElapsedTime(&ms, p.start, p.stop);
     auto &t = ctx->prof_totals[p.name];
     t.first += 1;
     t.second += ms;
