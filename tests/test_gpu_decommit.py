@@ -101,7 +101,7 @@ def low_degree_column(log_size, log_blowup, seed):
 
 
 FRI_CASES = [(8, (1, 0, 5, 1)), (10, (2, 1, 8, 2)), (9, (1, 2, 12, 3)), (7, (1, 1, 3, 1)), (6, (1, 4, 4, 1)),
-             (14, (1, 3, 20, 1)), (16, (2, 2, 30, 4))]
+             (14, (1, 3, 20, 1)), (16, (2, 2, 30, 4)), (13, (1, 1, 6, 5)), (20, (1, 0, 3, 1))]
 
 
 @pytest.mark.parametrize("log_size,cfgt", FRI_CASES)
