@@ -99,6 +99,60 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(const uint32_t *prev,
   store_hash(out + 8 * (size_t)i, s, m31);
 }
 
+// Layers of few nodes, where a node's chain of blocks is the launch's
+// critical path: one node per quad (b2_compress_quad, about 410
+// instructions per lane instead of 960 on one lane), each block's column
+// words loaded one compression ahead.  Same inputs and outputs as
+// merkle_layer_kernel; the grid has 4 lanes per node.
+__global__ void __launch_bounds__(256) merkle_layer_quad_kernel(const uint32_t *prev, const ColRef *cols,
+                                                                uint32_t n_cols, uint32_t log_size, uint32_t *out,
+                                                                uint32_t m31, uint32_t lifted) {
+  const uint32_t n = 1u << log_size;
+  const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 2, j = threadIdx.x & 3;
+  if (i >= n) return;  // a quad's four lanes share i and leave together
+  auto load_cols = [&](uint32_t c0, uint32_t (&dst)[16]) {
+    const uint32_t cnt = min(16u, n_cols - c0);
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+      uint32_t v = 0;
+      if (k < cnt) {
+        const ColRef cr = cols[c0 + k];
+        uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
+        v = as_global(cr.p)[idx];
+      }
+      dst[k] = v;
+    }
+  };
+  uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+  uint32_t m[16], nx[16];
+  uint32_t t = 0;
+  if (n_cols) load_cols(0, nx);
+  if (prev) {
+    load_block64(m, prev + 16 * (size_t)i);
+    t = 64;
+    b2_compress_quad(lo, hi, m, t, n_cols == 0);
+  } else if (n_cols == 0) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = 0;
+    b2_compress_quad(lo, hi, m, 0, true);
+  }
+  for (uint32_t c0 = 0; c0 < n_cols; c0 += 16) {
+    const uint32_t cnt = min(16u, n_cols - c0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = nx[k];
+    if (c0 + 16 < n_cols) load_cols(c0 + 16, nx);
+    t += 4 * cnt;
+    b2_compress_quad(lo, hi, m, t, c0 + 16 >= n_cols);
+  }
+  if (m31) {
+    lo = b2_reduce_word(lo);
+    hi = b2_reduce_word(hi);
+  }
+  g_u32 *o = as_global(out + 8 * (size_t)i);
+  o[j] = lo;
+  o[4 + j] = hi;
+}
+
 // Hashes `n_layers` (<= 8) consecutive node layers below `prev` (of log size
 // out_log + 1).  Layer k (k = 0 first) has log size out_log - k and is
 // written at out_base + offset(out_log - k) in the packed layout.
@@ -128,12 +182,17 @@ static inline size_t layer_off(uint32_t l) { return tree_layer_off(l); }
 int launch_merkle_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const ColRef *d_cols, uint32_t n_cols,
                         uint32_t log_size, uint32_t *out, int mode, bool lifted) {
   uint32_t n = 1u << log_size;
-  // Below 2^16 nodes (256 workgroups) a layer with more than one block of
-  // columns is latency-bound: load ahead.
+  const char *name = lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer";
+  const uint32_t m31 = (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31);
+  // Up to 2^14 nodes with more than one block per node, the layer is
+  // latency-bound even at four lanes per node: a quad per node.  Below 2^16
+  // nodes, a lane per node with the loads one block ahead.
+  if (log_size <= 14 && n_cols > 16)
+    return launch(ctx, name, merkle_layer_quad_kernel, dim3((4 * n + 255) / 256), dim3(256), 0, prev, d_cols, n_cols,
+                  log_size, out, m31, (uint32_t)lifted);
   const bool ahead = log_size < 16 && n_cols > 16;
-  return launch(ctx, lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer",
-                ahead ? merkle_layer_kernel<true> : merkle_layer_kernel<false>, dim3((n + 255) / 256), dim3(256), 0,
-                prev, d_cols, n_cols, log_size, out, (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31), (uint32_t)lifted);
+  return launch(ctx, name, ahead ? merkle_layer_kernel<true> : merkle_layer_kernel<false>, dim3((n + 255) / 256),
+                dim3(256), 0, prev, d_cols, n_cols, log_size, out, m31, (uint32_t)lifted);
 }
 
 static int launch_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const std::vector<ColRef> &cols, uint32_t log_size,

@@ -273,8 +273,11 @@ def test_merkle_golden(ctx, golden):
 
 @pytest.mark.parametrize("mode", [sh.HASH_BLAKE2S, sh.HASH_BLAKE2S_M31])
 def test_commit_on_layer_vs_oracle(ctx, mode):
+    # > 16 columns up to 2^14 nodes: a quad per node; at 2^15: a lane per node
+    # with the loads one block ahead; otherwise a lane per node.
     for log_size, n_cols, with_prev in [(10, 0, True), (10, 1, False), (11, 16, False), (9, 17, True),
-                                        (12, 33, True), (3, 5, True), (0, 2, True)]:
+                                        (12, 33, True), (3, 5, True), (0, 2, True), (14, 40, False),
+                                        (1, 18, False), (15, 20, True), (16, 17, False)]:
         cols = [rand(1 << log_size, 13, 100 * log_size + c) for c in range(n_cols)]
         prev = rand(16 << log_size, 17, log_size) if with_prev else None
         dcols = [ctx.to_device(c) for c in cols]
