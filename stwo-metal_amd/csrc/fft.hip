@@ -620,8 +620,13 @@ template <bool INV, int WG_LOG, int NS>
 __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(5))) fft_pass_multi_kernel(const FftPass2 *groups,
                                                                     const uint32_t *wg_start, uint32_t n_groups) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
-  uint32_t g = 0;
-  while (g + 1 < n_groups && wg_start[g + 1] <= blockIdx.x) g++;
+  // The group whose range holds this workgroup: the last g with
+  // wg_start[g] <= blockIdx.x (wg_start[0] = 0, increasing), by bisection.
+  uint32_t g = 0, hi = n_groups;
+  while (hi - g > 1) {
+    const uint32_t mid = (g + hi) >> 1;
+    if (wg_start[mid] <= blockIdx.x) g = mid; else hi = mid;
+  }
   const FftPass2 p = groups[g];
   fft_pass_body<INV, WG_LOG, NS, 0, 0, 0>(p, blockIdx.x - wg_start[g], lds);
 }
