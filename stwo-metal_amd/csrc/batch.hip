@@ -51,7 +51,7 @@ struct Staging {
 
 struct LevelLaunch {
   bool inverse;
-  bool uniform;  // every group's tile is 2^10 or more
+  int kind;  // launch_fft_multi's schedule kind for every group of the launch
   size_t groups_off, wgs_off;  // offsets into the device buffer
   uint32_t n_groups, total_wgs, wg_log;
 };
@@ -144,8 +144,10 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   // device addresses of the tables, known once the buffer exists.
   size_t tables_bytes = (st.host.size() + 255) & ~(size_t)255;
   void *dmem = nullptr;
-  // Upper bound on descriptors: 2 directions x 3 levels x 2 workgroup sizes, plus alignment.
-  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 512 + 4096;
+  // Upper bound on descriptors: one per group and pass (2 directions, at most
+  // 4 passes), plus the 256-byte alignment of two tables per launch (2
+  // directions x 4 levels x 2 workgroup sizes x 2 schedule kinds).
+  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 2 * 2 * 256 + 4096;
   if (hipMalloc(&dmem, tables_bytes + desc_bytes) != hipSuccess) {
     (void)hipGetLastError();
     return set_error(ctx, STWO_HIP_ENOMEM, "batch_create: out of device memory for tables");
@@ -177,29 +179,35 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     if (!inverse) std::reverse(order.begin(), order.end());  // forward: highest level first
     for (const auto &key : order) {
       auto &v = by_level[key];
-      std::vector<FftPass2> descs;
-      std::vector<uint32_t> starts;
-      uint64_t acc = 0;
-      bool uniform = true;
-      for (auto &pl : v) {
-        descs.push_back(pl.p);
-        starts.push_back((uint32_t)acc);
-        acc += pl.wgs;
-        if (pl.p.T < 10) uniform = false;
+      // Groups whose pass matches a compile-time schedule get a launch of
+      // their own with that schedule's kernel; the rest share one.
+      for (int part = 0; part < 2; part++) {
+        std::vector<FftPass2> descs;
+        std::vector<uint32_t> starts;
+        uint64_t acc = 0;
+        bool uniform = true;
+        for (auto &pl : v) {
+          if ((pl.p.fixed != 0) != (part == 0)) continue;
+          descs.push_back(pl.p);
+          starts.push_back((uint32_t)acc);
+          acc += pl.wgs;
+          if (pl.p.T < 10) uniform = false;
+        }
+        if (descs.empty()) continue;
+        if (acc >= (1ull << 31)) {
+          (void)hipFree(dmem);
+          return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
+        }
+        LevelLaunch ll;
+        ll.inverse = inverse;
+        ll.kind = part == 0 ? (key.second == 9 ? 2 : 1) : (uniform ? -1 : 0);
+        ll.groups_off = st.put(descs.data(), descs.size() * sizeof(FftPass2));
+        ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
+        ll.n_groups = (uint32_t)descs.size();
+        ll.total_wgs = (uint32_t)acc;
+        ll.wg_log = key.second;
+        levels.push_back(ll);
       }
-      if (acc >= (1ull << 31)) {
-        (void)hipFree(dmem);
-        return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
-      }
-      LevelLaunch ll;
-      ll.inverse = inverse;
-      ll.groups_off = st.put(descs.data(), descs.size() * sizeof(FftPass2));
-      ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
-      ll.n_groups = (uint32_t)descs.size();
-      ll.total_wgs = (uint32_t)acc;
-      ll.wg_log = key.second;
-      ll.uniform = uniform;
-      levels.push_back(ll);
     }
   }
   if (st.host.size() > tables_bytes + desc_bytes) {
@@ -224,7 +232,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     for (const LevelLaunch &ll : levels) {
       int rc = launch_fft_multi(c, ll.inverse, (const FftPass2 *)(dbase + ll.groups_off),
                                 (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs, ll.wg_log,
-                                ll.uniform);
+                                ll.kind);
       if (rc) return rc;
     }
     if (kind == STWO_HIP_TREE_LIFTED) {

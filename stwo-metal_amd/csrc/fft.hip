@@ -614,10 +614,12 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
 
 // Several passes of different column groups (sizes) in one launch: group g
 // owns workgroups [wg_start[g], wg_start[g + 1]).  Used by the batched path,
-// where every column size contributes its pass of the same level.  NS = -1
-// (wave-uniform columns) when every group's tile is 2^10 or more.
-template <bool INV, int WG_LOG, int NS>
-__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(5))) fft_pass_multi_kernel(const FftPass2 *groups,
+// where every column size contributes its pass of the same level.  The
+// schedule parameters are those of fft_pass_kernel: every group of a launch
+// matches them (NS = 3 or 2: the compile-time schedules; NS = -1: tiles of
+// 2^10 or more; NS = 0: any).
+template <bool INV, int WG_LOG, int NS, int B0A = 0, int B0B = 0, int B0C = 0>
+__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(NS > 0 ? 4 : 5))) fft_pass_multi_kernel(const FftPass2 *groups,
                                                                     const uint32_t *wg_start, uint32_t n_groups) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
   // The group whose range holds this workgroup: the last g with
@@ -628,7 +630,7 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
     if (wg_start[mid] <= blockIdx.x) g = mid; else hi = mid;
   }
   const FftPass2 p = groups[g];
-  fft_pass_body<INV, WG_LOG, NS, 0, 0, 0>(p, blockIdx.x - wg_start[g], lds);
+  fft_pass_body<INV, WG_LOG, NS, B0A, B0B, B0C>(p, blockIdx.x - wg_start[g], lds);
 }
 
 // Stages covering layer tile bits [A, B) of a tile of T bits, forward order.
@@ -720,6 +722,26 @@ static int check_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_tw
   return STWO_HIP_OK;
 }
 
+// The compile-time schedules: every stage applies its group's 4 layers, and
+// the groups are T-4, T-8, ... (forward) or the reverse (inverse).  These are
+// the passes of columns of log size >= 16 split 12 + 8 + ..., the low pass
+// (T = 12) and the 8-layer strided passes (T = 13).
+static bool full_schedule(const FftPass2 &p, const uint32_t *b0s, uint32_t ns) {
+  if (p.n_stages != ns) return false;
+  for (uint32_t i = 0; i < ns; i++)
+    if (p.st[i].b0 != b0s[i] || p.st[i].r_lo != 0 || p.st[i].r_hi != 4) return false;
+  return true;
+}
+
+static uint32_t fixed_schedule(const FftPass2 &p, bool inverse, uint32_t wg_log) {
+  static const uint32_t kFwd12[3] = {8, 4, 0}, kInv12[3] = {0, 4, 8}, kFwd13[2] = {9, 5}, kInv13[2] = {5, 9};
+  if (wg_log == 8 && p.T == T_FIXED12 && p.w == 0 && p.lo == 0 && p.in_log >= T_FIXED12 &&
+      full_schedule(p, inverse ? kInv12 : kFwd12, 3))
+    return 1;
+  if (wg_log == 9 && p.T == 13 && full_schedule(p, inverse ? kInv13 : kFwd13, 2)) return 2;
+  return 0;
+}
+
 // Pass parameters (in execution order) of one group of same-shape columns on
 // the radix-16 path (N >= 4).  `d_tab` is the group's FftCol table for the
 // first pass; later forward passes run in place on the outputs and use
@@ -770,29 +792,17 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
       if (p.st[0].b0 == 0 && p.in_log >= 4) p.vec_io |= 1;
       if (p.st[p.n_stages - 1].b0 == 0) p.vec_io |= 2;
     }
+    p.fixed = fixed_schedule(p, inverse, wg_log);
     out.push_back({p, (uint32_t)wgs, wg_log});
   }
   return STWO_HIP_OK;
 }
 
-// The compile-time schedules: every stage applies its group's 4 layers, and
-// the groups are T-4, T-8, ... (forward) or the reverse (inverse).  These are
-// the passes of columns of log size >= 16 split 12 + 8 + ..., the low pass
-// (T = 12) and the 8-layer strided passes (T = 13).
-static bool full_schedule(const FftPass2 &p, const uint32_t *b0s, uint32_t ns) {
-  if (p.n_stages != ns) return false;
-  for (uint32_t i = 0; i < ns; i++)
-    if (p.st[i].b0 != b0s[i] || p.st[i].r_lo != 0 || p.st[i].r_hi != 4) return false;
-  return true;
-}
-
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log) {
-  static const uint32_t kFwd12[3] = {8, 4, 0}, kInv12[3] = {0, 4, 8}, kFwd13[2] = {9, 5}, kInv13[2] = {5, 9};
-  if (wg_log == 8 && p.T == T_FIXED12 && p.w == 0 && p.lo == 0 && p.in_log >= T_FIXED12 &&
-      full_schedule(p, inverse ? kInv12 : kFwd12, 3))
+  if (wg_log == 8 && p.fixed == 1)
     return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 3, 0, 4, 8>, dim3(wgs), dim3(256), 0, p)
                    : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 3, 8, 4, 0>, dim3(wgs), dim3(256), 0, p);
-  if (wg_log == 9 && p.T == 13 && full_schedule(p, inverse ? kInv13 : kFwd13, 2))
+  if (wg_log == 9 && p.fixed == 2)
     return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, 2, 5, 9>, dim3(wgs), dim3(512), 0, p)
                    : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, 2, 9, 5>, dim3(wgs), dim3(512), 0, p);
   // Profiling names follow the instantiation: *_wg9 = fft_pass_kernel<INV, 9, ...>.
@@ -813,20 +823,25 @@ int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t
 }
 
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
-                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, bool uniform) {
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind) {
   const dim3 grid(total_wgs), b9(512), b8(256);
+  const char *name = inverse ? "ifft_pass_batched" : "rfft_pass_batched";
+#define STWO_MULTI(WG, ...)                                                                                    \
+  (inverse ? launch(ctx, name, fft_pass_multi_kernel<true, WG, __VA_ARGS__>, grid, WG == 9 ? b9 : b8, 0, d_groups, \
+                    d_wg_start, n_groups)                                                                      \
+           : launch(ctx, name, fft_pass_multi_kernel<false, WG, __VA_ARGS__>, grid, WG == 9 ? b9 : b8, 0,       \
+                    d_groups, d_wg_start, n_groups))
   if (wg_log == 9) {
-    if (uniform)
-      return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 9, -1>, grid, b9, 0, d_groups, d_wg_start, n_groups)
-                     : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 9, -1>, grid, b9, 0, d_groups, d_wg_start, n_groups);
-    return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 9, 0>, grid, b9, 0, d_groups, d_wg_start, n_groups)
-                   : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 9, 0>, grid, b9, 0, d_groups, d_wg_start, n_groups);
+    if (kind == 2) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 9, 2, 5, 9>, grid, b9, 0, d_groups, d_wg_start, n_groups)
+                                  : launch(ctx, name, fft_pass_multi_kernel<false, 9, 2, 9, 5>, grid, b9, 0, d_groups, d_wg_start, n_groups);
+    if (kind == -1) return STWO_MULTI(9, -1);
+    return STWO_MULTI(9, 0);
   }
-  if (uniform)
-    return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 8, -1>, grid, b8, 0, d_groups, d_wg_start, n_groups)
-                   : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 8, -1>, grid, b8, 0, d_groups, d_wg_start, n_groups);
-  return inverse ? launch(ctx, "ifft_pass_batched", fft_pass_multi_kernel<true, 8, 0>, grid, b8, 0, d_groups, d_wg_start, n_groups)
-                 : launch(ctx, "rfft_pass_batched", fft_pass_multi_kernel<false, 8, 0>, grid, b8, 0, d_groups, d_wg_start, n_groups);
+  if (kind == 1) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 8, 3, 0, 4, 8>, grid, b8, 0, d_groups, d_wg_start, n_groups)
+                                : launch(ctx, name, fft_pass_multi_kernel<false, 8, 3, 8, 4, 0>, grid, b8, 0, d_groups, d_wg_start, n_groups);
+  if (kind == -1) return STWO_MULTI(8, -1);
+  return STWO_MULTI(8, 0);
+#undef STWO_MULTI
 }
 
 // Radix-16 path for N >= 4: one launch per pass over all columns of the group.

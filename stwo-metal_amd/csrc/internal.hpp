@@ -38,6 +38,10 @@ struct FftPass2 {
   // layout (b0 = 0) is stored the same way.  Set only for contiguous tiles
   // (w = 0) of 16-byte aligned columns.
   uint32_t vec_io;
+  // The compile-time schedule the pass matches (set by make_group_passes):
+  // 1 = the 12-layer pass of 3 full stages (wg8), 2 = the 8-layer strided
+  // pass of 2 full stages on a 2^13 tile (wg9), 0 = none (run-time schedule).
+  uint32_t fixed;
 };
 
 struct PassLaunch {
@@ -51,9 +55,11 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
                       const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
                       std::vector<PassLaunch> &out);
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log);
-// `uniform`: every group's tile is 2^10 or more (the wave-uniform-column kernel).
+// `kind`: the schedule every group of the launch matches: 1 or 2 (FftPass2
+// fixed, with wg_log 8 or 9), -1 (tiles of 2^10 or more: wave-uniform
+// columns) or 0 (any).
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
-                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, bool uniform);
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind);
 
 // ---- decommitment: batched device gather -----------------------------------
 
