@@ -233,7 +233,7 @@ ROCPROF_NAMES = {
     "rfft_pass_wg9": "void stwo_hip::fft_pass_kernel<false, 9, 2, 9, 5, 0>(stwo_hip::FftPass2)",
     "ifft_pass_wg8": "void stwo_hip::fft_pass_kernel<true, 8, 3, 0, 4, 8>(stwo_hip::FftPass2)",
     "ifft_pass_wg9": "void stwo_hip::fft_pass_kernel<true, 9, 2, 5, 9, 0>(stwo_hip::FftPass2)",
-    "merkle_commit_on_layer": "stwo_hip::merkle_layer_kernel(...)",
+    "merkle_commit_on_layer": "void stwo_hip::merkle_layer_kernel<false>(...)",
     "merkle_nodes": "stwo_hip::merkle_nodes_kernel(...)",
 }
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_lde_merkle_256x2e20_b4.json")
