@@ -264,6 +264,13 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   };
   b->n_launches = levels.size();
   if (use_graph) {
+    // Tables of earlier calls still pending are copied outside the graph.
+    int frc = flush_uploads(ctx);
+    if (frc) {
+      (void)hipFree(dmem);
+      delete b;
+      return frc;
+    }
     bool prof = ctx->prof;
     ctx->prof = false;  // no event records inside the captured sequence
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);

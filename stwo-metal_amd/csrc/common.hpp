@@ -42,13 +42,16 @@ struct stwo_hip_ctx {
   // Ring arena for small per-call tables (column pointer lists, per-column
   // parameters).  Host side is pinned so the H2D copy is a true async DMA;
   // the device side is read by kernels on the same stream, so stream order
-  // alone protects it.  `inflight` keeps, per upload, its host byte range and
-  // an event after its copy; host bytes are only rewritten once the copy that
-  // read them has completed.
+  // alone protects it.  `inflight` keeps, per copy, its host byte range and
+  // an event after it; host bytes are only rewritten once the copy that read
+  // them has completed.  Tables written back to back form one pending range
+  // [pend_begin, pend_end), copied by a single hipMemcpyAsync when the next
+  // kernel is launched (flush_uploads).
   uint8_t *h_arena = nullptr;
   uint8_t *d_arena = nullptr;
   size_t arena_cap = 0;
   size_t arena_head = 0;
+  size_t pend_begin = 0, pend_end = 0;
   struct Inflight {
     size_t begin, end;
     hipEvent_t ev;
@@ -88,8 +91,13 @@ int set_error(stwo_hip_ctx *ctx, int code, const char *fmt, ...);
   } while (0)
 
 // Copies `bytes` of host data into the arena and returns the device address
-// (16-byte aligned).  The copy is enqueued on the context stream.
+// (16-byte aligned).  The host-to-device copy is enqueued on the context
+// stream by the next flush_uploads (every launch() makes one), together with
+// the other tables written since the last one.
 int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr);
+
+// Enqueues the copy of the pending tables (none: nothing to do).
+int flush_uploads(stwo_hip_ctx *ctx);
 
 // Returns a device scratch buffer of at least `bytes` for `key` (grown on
 // demand; growing synchronizes the stream before freeing the old buffer).
@@ -114,6 +122,8 @@ template <typename... KArgs, typename... Args>
 int launch(stwo_hip_ctx *ctx, const char *name, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem,
            Args... args) {
   if (grid.x == 0 || grid.y == 0 || grid.z == 0) return STWO_HIP_OK;
+  int frc = flush_uploads(ctx);
+  if (frc) return frc;
   hipEvent_t start = nullptr;
   prof_begin(ctx, name, &start);
   hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, static_cast<KArgs>(args)...);

@@ -122,20 +122,35 @@ static int arena_wait_range(stwo_hip_ctx *ctx, size_t b, size_t e) {
   return STWO_HIP_OK;
 }
 
+int flush_uploads(stwo_hip_ctx *ctx) {
+  if (ctx->pend_end <= ctx->pend_begin) return STWO_HIP_OK;
+  const size_t b = ctx->pend_begin, e = ctx->pend_end;
+  ctx->pend_begin = ctx->pend_end = 0;
+  STWO_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_arena + b, ctx->h_arena + b, e - b, hipMemcpyHostToDevice, ctx->stream));
+  hipEvent_t ev = take_event(ctx);
+  if (!ev) return set_error(ctx, STWO_HIP_EDEVICE, "hipEventCreate failed");
+  STWO_HIP_TRY(ctx, hipEventRecord(ev, ctx->stream));
+  ctx->inflight.push_back({b, e, ev});
+  return STWO_HIP_OK;
+}
+
 int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr) {
   size_t sz = (bytes + 255) & ~(size_t)255;
   if (sz == 0) sz = 256;
   if (sz > ctx->arena_cap) return set_error(ctx, STWO_HIP_EINVAL, "table of %zu bytes exceeds the arena", bytes);
   size_t off = ctx->arena_head;
   if (off + sz > ctx->arena_cap) off = 0;
-  int rc = arena_wait_range(ctx, off, off + sz);
+  // A table that does not extend the pending range (the ring wrapped) starts
+  // a new one: the pending bytes are copied first, so that the wait below
+  // also covers them.
+  int rc = STWO_HIP_OK;
+  if (ctx->pend_end > ctx->pend_begin && off != ctx->pend_end) rc = flush_uploads(ctx);
+  if (rc) return rc;
+  rc = arena_wait_range(ctx, off, off + sz);
   if (rc) return rc;
   memcpy(ctx->h_arena + off, src, bytes);
-  STWO_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_arena + off, ctx->h_arena + off, bytes, hipMemcpyHostToDevice, ctx->stream));
-  hipEvent_t ev = take_event(ctx);
-  if (!ev) return set_error(ctx, STWO_HIP_EDEVICE, "hipEventCreate failed");
-  STWO_HIP_TRY(ctx, hipEventRecord(ev, ctx->stream));
-  ctx->inflight.push_back({off, off + sz, ev});
+  if (ctx->pend_end <= ctx->pend_begin) ctx->pend_begin = off;
+  ctx->pend_end = off + sz;
   ctx->arena_head = off + sz;
   *dptr = ctx->d_arena + off;
   return STWO_HIP_OK;
@@ -239,6 +254,8 @@ int stwo_hip_last_error(const stwo_hip_ctx *ctx, char *buf, size_t len) {
 
 int stwo_hip_sync(stwo_hip_ctx *ctx) {
   if (!ctx) return STWO_HIP_EINVAL;
+  int rc = flush_uploads(ctx);
+  if (rc) return rc;
   STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   return STWO_HIP_OK;
 }
