@@ -42,6 +42,13 @@ __device__ __forceinline__ void load_block64(uint32_t m[16], const uint32_t *src
   }
 }
 
+// Word idx of a column.  Layers are at most 2^30 rows, so the byte offset
+// fits 32 bits: the load takes the column pointer as an SGPR base and a
+// 32-bit lane offset (no per-column 64-bit address arithmetic).
+__device__ __forceinline__ uint32_t ld_col_word(const uint32_t *col, uint32_t idx) {
+  return *(gconst_u32 *)((gconst_u8 *)as_global(col) + (idx << 2));
+}
+
 // Mixed-degree layer: optional previous layer, then the layer's columns.
 // `lifted` selects the lifted leaf index for column words.
 // AHEAD: each block's column words are loaded one compression ahead, so
@@ -65,7 +72,7 @@ __global__ void __launch_bounds__(256) merkle_layer_kernel(const uint32_t *prev,
       if (k < cnt) {
         const ColRef cr = cols[c0 + k];
         uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
-        v = as_global(cr.p)[idx];
+        v = ld_col_word(cr.p, idx);
       }
       dst[k] = v;
     }
@@ -118,7 +125,7 @@ __global__ void __launch_bounds__(256) merkle_layer_quad_kernel(const uint32_t *
       if (k < cnt) {
         const ColRef cr = cols[c0 + k];
         uint32_t idx = lifted ? (((i >> (cr.shift + 1)) << 1) + (i & 1)) : i;
-        v = as_global(cr.p)[idx];
+        v = ld_col_word(cr.p, idx);
       }
       dst[k] = v;
     }
