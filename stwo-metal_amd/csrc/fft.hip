@@ -665,9 +665,12 @@ static std::vector<PassPlan> plan_passes2(uint32_t N, uint32_t L) {
     uint32_t rest = L - t0;
     uint32_t npass = (rest + kStridedMaxLayers2 - 1) / kStridedMaxLayers2;
     uint32_t hi = L;
+    // Full 8-layer passes from the top (the compile-time two-stage schedule
+    // of a 2^13 tile, which also serves a blowup's first pass), the
+    // remainder in the last strided pass.
     for (uint32_t k = 0; k < npass; k++) {
-      uint32_t left = hi - t0, passes_left = npass - k;
-      uint32_t m = (left + passes_left - 1) / passes_left;
+      uint32_t left = hi - t0;
+      uint32_t m = k + 1 < npass ? kStridedMaxLayers2 : left;
       uint32_t lo = hi - m;
       // The tile's row bits must hold a whole 4-bit register group: a pass of
       // fewer than 4 layers takes 4 rows, the extra ones (batch bits) being
@@ -696,9 +699,12 @@ static std::vector<PassPlan> plan_passes(uint32_t N, uint32_t L) {
     uint32_t rest = L - t0;
     uint32_t npass = (rest + kStridedMaxLayers - 1) / kStridedMaxLayers;
     uint32_t hi = L;
+    // Full 8-layer passes from the top (the compile-time two-stage schedule
+    // of a 2^13 tile, which also serves a blowup's first pass), the
+    // remainder in the last strided pass.
     for (uint32_t k = 0; k < npass; k++) {
-      uint32_t left = hi - t0, passes_left = npass - k;
-      uint32_t m = (left + passes_left - 1) / passes_left;
+      uint32_t left = hi - t0;
+      uint32_t m = k + 1 < npass ? kStridedMaxLayers2 : left;
       uint32_t lo = hi - m;
       fwd.push_back({lo, m, kTileLog - m, lo, hi});
       hi = lo;
