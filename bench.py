@@ -321,6 +321,55 @@ def measured_traffic(kernel):
     return measured_counters(kernel).get("hbm_bytes_per_launch")
 
 
+def hbm_kernels(ctx, log_n=26, runs=3):
+    """The kernels the HBM bandwidth bounds, at sizes far past the caches:
+    bit_reverse_column of a 2^27-word column (8 B per element: one read and
+    one write), fold_circle_into_line of a 2^26 QM31 evaluation into a 2^25
+    line (read 32 B of pairs and 16 B of the accumulator, write 16 B per
+    output) and one fold_line step 2^25 -> 2^24 (read 32 B, write 16 B per
+    output); twiddle reads are not counted.  Device time from HIP events."""
+    n_br = log_n + 1
+    tw = ctx.precompute_twiddles(sh.half_odds(log_n - 1))
+    col = ctx.alloc(1 << n_br)
+    ctx.fill_random(col, 1 << n_br, 8, 0)
+    src = [ctx.alloc(1 << log_n) for _ in range(4)]
+    acc = [ctx.alloc(1 << (log_n - 1)) for _ in range(4)]
+    line = [ctx.alloc(1 << (log_n - 2)) for _ in range(4)]
+    for k, b in enumerate(src):
+        ctx.fill_random(b, 1 << log_n, 8, 1 + k)
+    for k, b in enumerate(acc):
+        ctx.fill_random(b, 1 << (log_n - 1), 8, 5 + k)
+    alpha = [1, 2, 3, 4]
+
+    def once():
+        ctx.bit_reverse_column(col, n_br)
+        ctx.fold_circle_into_line(acc, src, sh.half_odds(log_n - 1), alpha, tw)
+        ctx.fold_line(acc, sh.half_odds(log_n - 1), alpha, 1, tw, line)
+
+    once()
+    ctx.sync()
+    ctx.prof_enable(True)
+    ctx.prof_reset()
+    for _ in range(runs):
+        once()
+    ctx.sync()
+    prof = ctx.prof_read()
+    ctx.prof_enable(False)
+    out = {}
+    for name, elems, per in (("bit_reverse_tiled", 1 << n_br, 8.0),
+                             ("fold_circle_into_line", 1 << (log_n - 1), 64.0),
+                             ("fold_line", 1 << (log_n - 2), 48.0)):
+        cnt, ms = prof[name]
+        t = ms / cnt
+        gbps = elems * per / (t * 1e-3) / 1e9
+        out[name] = {"outputs": elems, "bytes_per_output": per, "launch_ms": round(t, 4),
+                     "algorithmic_GBps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBS, 4)}
+    for b in [col] + src + acc + line:
+        b.free()
+    tw.destroy()
+    return out
+
+
 def cpu_baseline(sample_cols=N_COLS, runs=3):
     """The oracle (C, OpenMP over columns and nodes) on the same workload, on the host cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -471,6 +520,8 @@ def main():
     for x in pristine + work + outs + [layers]:
         x.free()
     pristine, work, outs, layers = [], [], [], None
+    hbm = hbm_kernels(ctx)
+    log("HBM-bound kernels: %s" % json.dumps(hbm))
     if not args.no_serve:
         serve = serving()
         log("serving: %s" % json.dumps(serve))
@@ -517,6 +568,7 @@ def main():
         },
         "valu_roofline": valu_roofline,
         "side_rates": side,
+        "hbm_kernels": hbm,
         "cpu_baseline": cpu,
         "kernels": kernels,
         "batched_uneven": batched,

@@ -187,7 +187,13 @@ __global__ void __launch_bounds__(256) bit_reverse_small_kernel(uint32_t *v, uin
 // For log_n >= 2 * kBrLog: i = (hi << (log_n - 6)) | (mid << 6) | lo with
 // hi, lo 6 bits; bitrev(i) = (rev6(lo) << (log_n - 6)) | (rev(mid) << 6) | rev6(hi).
 // Block (mid) handles the 64x64 tile {hi, lo} at `mid` and its partner at rev(mid).
-__global__ void __launch_bounds__(256) bit_reverse_tiled_kernel(uint32_t *v, uint32_t log_n) {
+// 512 lanes: the two tiles' 33 KB of LDS allow 4 workgroups per CU, and the
+// kernel is bound by memory latency, so each workgroup carries 8 waves.
+// VEC (16-byte aligned column): each lane moves 4 consecutive words of a row
+// with one 16-byte load or store.
+constexpr uint32_t kBrThreads = 512;
+template <bool VEC>
+__global__ void __launch_bounds__(kBrThreads) bit_reverse_tiled_kernel(uint32_t *v, uint32_t log_n) {
   __shared__ uint32_t ta[64][65];
   __shared__ uint32_t tb[64][65];
   const uint32_t mid_bits = log_n - 2 * kBrLog;
@@ -195,18 +201,53 @@ __global__ void __launch_bounds__(256) bit_reverse_tiled_kernel(uint32_t *v, uin
   const uint32_t rmid = bitrev32(mid, mid_bits);
   if (rmid < mid) return;  // the partner block does the swap
   const uint32_t hs = log_n - kBrLog;
-  for (uint32_t e = threadIdx.x; e < 4096; e += 256) {
-    uint32_t hi = e >> 6, lo = e & 63;
-    ta[hi][lo] = v[(hi << hs) | (mid << kBrLog) | lo];
-    tb[hi][lo] = v[(hi << hs) | (rmid << kBrLog) | lo];
+  if (VEC) {
+    for (uint32_t e = threadIdx.x; e < 1024; e += kBrThreads) {
+      const uint32_t hi = e >> 4, lo = (e & 15) << 2;
+      const u32x4 a = *(gconst_u4 *)as_global(v + ((hi << hs) | (mid << kBrLog) | lo));
+      const u32x4 b = *(gconst_u4 *)as_global(v + ((hi << hs) | (rmid << kBrLog) | lo));
+      ta[hi][lo] = a.x;
+      ta[hi][lo + 1] = a.y;
+      ta[hi][lo + 2] = a.z;
+      ta[hi][lo + 3] = a.w;
+      tb[hi][lo] = b.x;
+      tb[hi][lo + 1] = b.y;
+      tb[hi][lo + 2] = b.z;
+      tb[hi][lo + 3] = b.w;
+    }
+  } else {
+    for (uint32_t e = threadIdx.x; e < 4096; e += kBrThreads) {
+      uint32_t hi = e >> 6, lo = e & 63;
+      ta[hi][lo] = v[(hi << hs) | (mid << kBrLog) | lo];
+      tb[hi][lo] = v[(hi << hs) | (rmid << kBrLog) | lo];
+    }
   }
   __syncthreads();
   // new v[(hi, mid, lo)] = old v[bitrev] = old v[(rev(lo), rmid, rev(hi))] = tb[rev lo][rev hi]
-  for (uint32_t e = threadIdx.x; e < 4096; e += 256) {
-    uint32_t hi = e >> 6, lo = e & 63;
-    uint32_t rh = bitrev32(hi, kBrLog), rl = bitrev32(lo, kBrLog);
-    v[(hi << hs) | (mid << kBrLog) | lo] = tb[rl][rh];
-    if (rmid != mid) v[(hi << hs) | (rmid << kBrLog) | lo] = ta[rl][rh];
+  if (VEC) {
+    for (uint32_t e = threadIdx.x; e < 1024; e += kBrThreads) {
+      const uint32_t hi = e >> 4, lo = (e & 15) << 2, rh = bitrev32(hi, kBrLog);
+      u32x4 a, b;
+      a.x = tb[bitrev32(lo, kBrLog)][rh];
+      a.y = tb[bitrev32(lo + 1, kBrLog)][rh];
+      a.z = tb[bitrev32(lo + 2, kBrLog)][rh];
+      a.w = tb[bitrev32(lo + 3, kBrLog)][rh];
+      *(g_u4 *)as_global(v + ((hi << hs) | (mid << kBrLog) | lo)) = a;
+      if (rmid != mid) {
+        b.x = ta[bitrev32(lo, kBrLog)][rh];
+        b.y = ta[bitrev32(lo + 1, kBrLog)][rh];
+        b.z = ta[bitrev32(lo + 2, kBrLog)][rh];
+        b.w = ta[bitrev32(lo + 3, kBrLog)][rh];
+        *(g_u4 *)as_global(v + ((hi << hs) | (rmid << kBrLog) | lo)) = b;
+      }
+    }
+  } else {
+    for (uint32_t e = threadIdx.x; e < 4096; e += kBrThreads) {
+      uint32_t hi = e >> 6, lo = e & 63;
+      uint32_t rh = bitrev32(hi, kBrLog), rl = bitrev32(lo, kBrLog);
+      v[(hi << hs) | (mid << kBrLog) | lo] = tb[rl][rh];
+      if (rmid != mid) v[(hi << hs) | (rmid << kBrLog) | lo] = ta[rl][rh];
+    }
   }
 }
 
@@ -946,7 +987,9 @@ static int run_fft_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, uin
 static int bit_reverse_one(stwo_hip_ctx *ctx, uint32_t *col, uint32_t log_size) {
   if (log_size >= 2 * kBrLog) {
     uint32_t mids = 1u << (log_size - 2 * kBrLog);
-    return launch(ctx, "bit_reverse_tiled", bit_reverse_tiled_kernel, dim3(mids), dim3(256), 0, col, log_size);
+    return launch(ctx, "bit_reverse_tiled",
+                  ((uintptr_t)col & 15) ? bit_reverse_tiled_kernel<false> : bit_reverse_tiled_kernel<true>,
+                  dim3(mids), dim3(kBrThreads), 0, col, log_size);
   }
   uint32_t n = 1u << log_size;
   return launch(ctx, "bit_reverse_small", bit_reverse_small_kernel, dim3((n + 255) / 256), dim3(256), 0, col,

@@ -80,6 +80,23 @@ def test_bit_reverse_column(ctx, log_size):
     d.free()
 
 
+@pytest.mark.parametrize("log_size,offset", [(12, 1), (17, 3), (20, 2)])
+def test_bit_reverse_column_unaligned(ctx, log_size, offset):
+    """A column that starts `offset` words into its buffer (not 16-byte
+    aligned: the tiled kernel's one-word path); the words around it unchanged."""
+    n = 1 << log_size
+    v = rand(n, 7, log_size)
+    buf = ctx.alloc(n + 8)
+    guard = np.full(n + 8, 0x5A5A5A5A, dtype=U)
+    buf.upload(guard)
+    buf.upload(v, offset=offset)
+    ctx.bit_reverse_column(buf.at(offset), log_size)
+    got = buf.download()
+    assert np.array_equal(got[offset:offset + n], oracle.bit_reverse(v))
+    assert np.all(got[:offset] == 0x5A5A5A5A) and np.all(got[offset + n:] == 0x5A5A5A5A)
+    buf.free()
+
+
 def test_bit_reverse_secure_column(ctx):
     """metal_secure_field_bit_reverse_matches_cpu_backend (tests/metal_secure_field_bit_reverse.rs:8)."""
     log_size = 10
