@@ -426,7 +426,8 @@ def test_fri_first_layer_fold_reference_shape(ctx):
     tw.destroy()
 
 
-@pytest.mark.parametrize("n,step", [(9, 2), (12, 1), (16, 3), (14, 4), (13, 6), (20, 1)])
+# (5, 5): the last of the two launches (4 steps, then 1) folds 2 values into 1.
+@pytest.mark.parametrize("n,step", [(9, 2), (12, 1), (16, 3), (14, 4), (13, 6), (20, 1), (5, 5), (2, 1)])
 def test_fold_line_vs_oracle(ctx, n, step):
     """Includes metal_line_fold_matches_cpu_backend's shape (tests/metal_line_fold.rs:12): log 9,
     fold_step 2, alpha (2, 4, 6, 8), values 8i + k."""
