@@ -146,8 +146,8 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   void *dmem = nullptr;
   // Upper bound on descriptors: one per group and pass (2 directions, at most
   // 4 passes), plus the 256-byte alignment of two tables per launch (2
-  // directions x 4 levels x 2 workgroup sizes x 2 schedule kinds).
-  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 2 * 2 * 256 + 4096;
+  // directions x 4 levels x 2 workgroup sizes x at most 4 schedule kinds).
+  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 4 * 2 * 256 + 4096;
   if (hipMalloc(&dmem, tables_bytes + desc_bytes) != hipSuccess) {
     (void)hipGetLastError();
     return set_error(ctx, STWO_HIP_ENOMEM, "batch_create: out of device memory for tables");
@@ -179,15 +179,18 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     if (!inverse) std::reverse(order.begin(), order.end());  // forward: highest level first
     for (const auto &key : order) {
       auto &v = by_level[key];
-      // Groups whose pass matches a compile-time schedule get a launch of
-      // their own with that schedule's kernel; the rest share one.
-      for (int part = 0; part < 2; part++) {
+      // Groups whose pass matches a compile-time schedule get a launch per
+      // schedule with that schedule's kernel; the rest share one.
+      std::vector<uint32_t> kinds;
+      for (auto &pl : v)
+        if (std::find(kinds.begin(), kinds.end(), pl.p.fixed) == kinds.end()) kinds.push_back(pl.p.fixed);
+      for (uint32_t kd : kinds) {
         std::vector<FftPass2> descs;
         std::vector<uint32_t> starts;
         uint64_t acc = 0;
         bool uniform = true;
         for (auto &pl : v) {
-          if ((pl.p.fixed != 0) != (part == 0)) continue;
+          if (pl.p.fixed != kd) continue;
           descs.push_back(pl.p);
           starts.push_back((uint32_t)acc);
           acc += pl.wgs;
@@ -200,7 +203,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
         }
         LevelLaunch ll;
         ll.inverse = inverse;
-        ll.kind = part == 0 ? (key.second == 9 ? 2 : 1) : (uniform ? -1 : 0);
+        ll.kind = kd != 0 ? (int)kd : (uniform ? -1 : 0);
         ll.groups_off = st.put(descs.data(), descs.size() * sizeof(FftPass2));
         ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
         ll.n_groups = (uint32_t)descs.size();

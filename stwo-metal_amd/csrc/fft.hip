@@ -445,9 +445,10 @@ __device__ __forceinline__ void lds_get(const uint32_t *L, uint32_t b0, uint32_t
 // of every one of the 2^rep_log output blocks: the coefficients are read
 // once from HBM (repeats hit L2) and the blocks differ only in twiddles.
 // One stage of a compile-time schedule: the register group at tile bit B0,
-// all 4 layers of it.  PREV is the previous stage's group, or -1 when it is
-// only known at run time (the first stage follows the load layout).
-template <bool INV, int B0, int PREV, typename GB, typename GI>
+// its register bits [0, RHI) (all 4 layers, or the low RHI of them).  PREV is
+// the previous stage's group, or -1 when it is only known at run time (the
+// first stage follows the load layout).
+template <bool INV, int B0, int PREV, int RHI = 4, typename GB, typename GI>
 __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16], uint32_t *L, uint32_t &cur,
                                             uint32_t local, GB group_base, GI gidx) {
   // Keep the scheduler from hoisting later stages' twiddle loads above this
@@ -458,7 +459,7 @@ __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16]
   const uint32_t gb = gidx(base);
   const uint32_t sh = B0 - p.w + p.lo;
   StageTw t;
-  load_stage_twiddles(t, p.tw_end, p.N, sh, gb, 0, 4);
+  load_stage_twiddles(t, p.tw_end, p.N, sh, gb, 0, RHI);
   if constexpr (PREV >= 0) {
     if constexpr (PREV != B0) {
       const uint32_t cb = ((local >> PREV) << (PREV + 4)) | (local & ((1u << PREV) - 1));
@@ -476,7 +477,7 @@ __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16]
     }
   }
   cur = B0;
-  stage_layers<INV, 0, 4>(x, t);
+  stage_layers<INV, 0, RHI>(x, t);
 }
 
 // The fixed schedules need about 125 VGPRs (4 waves per SIMD); asking for 5
@@ -586,7 +587,12 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         }
       }
     }
-    if constexpr (NS > 0) {
+    if constexpr (NS == 2 && B0C != 0) {
+      // A strided pass of 4 + B0C layers: the group at tile bit 6 (= w)
+      // applies only its low B0C register bits (first stage when inverse).
+      fixed_stage<INV, B0A, -1, INV ? B0C : 4>(p, x, L, cur, local, group_base, gidx);
+      fixed_stage<INV, B0B, B0A, INV ? 4 : B0C>(p, x, L, cur, local, group_base, gidx);
+    } else if constexpr (NS > 0) {
       fixed_stage<INV, B0A, -1>(p, x, L, cur, local, group_base, gidx);
       if constexpr (NS > 1) fixed_stage<INV, B0B, B0A>(p, x, L, cur, local, group_base, gidx);
       if constexpr (NS > 2) fixed_stage<INV, B0C, B0B>(p, x, L, cur, local, group_base, gidx);
@@ -780,12 +786,29 @@ static bool full_schedule(const FftPass2 &p, const uint32_t *b0s, uint32_t ns) {
   return true;
 }
 
+// A strided pass of m = 5, 6 or 7 layers on a tile of 2^(m + 6) (w = 6): the
+// full group at tile bit T - 4 and the low m - 4 register bits of the group
+// at tile bit 6 (forward: full first; inverse: partial first).
+static bool partial_schedule(const FftPass2 &p, bool inverse, uint32_t m) {
+  if (p.n_stages != 2 || p.w != 6 || p.T != m + 6) return false;
+  const FftStage full = {p.T - 4, 0, 4}, part = {6, 0, m - 4};
+  const FftStage &a = p.st[inverse ? 1 : 0], &b = p.st[inverse ? 0 : 1];
+  return a.b0 == full.b0 && a.r_lo == full.r_lo && a.r_hi == full.r_hi && b.b0 == part.b0 &&
+         b.r_lo == part.r_lo && b.r_hi == part.r_hi;
+}
+
+// FftPass2::fixed for a pass: 1 = 12 layers in 3 full stages (wg8), 2 = 8
+// strided layers in 2 full stages on a 2^13 tile (wg9), 3 / 4 / 5 = 5 / 6 / 7
+// strided layers (partial_schedule; wg8, wg8, wg9), 0 = run-time schedule.
 static uint32_t fixed_schedule(const FftPass2 &p, bool inverse, uint32_t wg_log) {
   static const uint32_t kFwd12[3] = {8, 4, 0}, kInv12[3] = {0, 4, 8}, kFwd13[2] = {9, 5}, kInv13[2] = {5, 9};
   if (wg_log == 8 && p.T == T_FIXED12 && p.w == 0 && p.lo == 0 && p.in_log >= T_FIXED12 &&
       full_schedule(p, inverse ? kInv12 : kFwd12, 3))
     return 1;
   if (wg_log == 9 && p.T == 13 && full_schedule(p, inverse ? kInv13 : kFwd13, 2)) return 2;
+  if (wg_log == 8 && partial_schedule(p, inverse, 5)) return 3;
+  if (wg_log == 8 && partial_schedule(p, inverse, 6)) return 4;
+  if (wg_log == 9 && partial_schedule(p, inverse, 7)) return 5;
   return 0;
 }
 
@@ -849,6 +872,15 @@ int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t
   if (wg_log == 8 && p.fixed == 1)
     return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 3, 0, 4, 8>, dim3(wgs), dim3(256), 0, p)
                    : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 3, 8, 4, 0>, dim3(wgs), dim3(256), 0, p);
+  if (p.fixed == 3)
+    return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 2, 6, 7, 1>, dim3(wgs), dim3(256), 0, p)
+                   : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 2, 7, 6, 1>, dim3(wgs), dim3(256), 0, p);
+  if (p.fixed == 4)
+    return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 2, 6, 8, 2>, dim3(wgs), dim3(256), 0, p)
+                   : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 2, 8, 6, 2>, dim3(wgs), dim3(256), 0, p);
+  if (p.fixed == 5)
+    return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, 2, 6, 9, 3>, dim3(wgs), dim3(512), 0, p)
+                   : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, 2, 9, 6, 3>, dim3(wgs), dim3(512), 0, p);
   if (wg_log == 9 && p.fixed == 2)
     return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, 2, 5, 9>, dim3(wgs), dim3(512), 0, p)
                    : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, 2, 9, 5>, dim3(wgs), dim3(512), 0, p);
@@ -881,11 +913,17 @@ int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, 
   if (wg_log == 9) {
     if (kind == 2) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 9, 2, 5, 9>, grid, b9, 0, d_groups, d_wg_start, n_groups)
                                   : launch(ctx, name, fft_pass_multi_kernel<false, 9, 2, 9, 5>, grid, b9, 0, d_groups, d_wg_start, n_groups);
+    if (kind == 5) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 9, 2, 6, 9, 3>, grid, b9, 0, d_groups, d_wg_start, n_groups)
+                                  : launch(ctx, name, fft_pass_multi_kernel<false, 9, 2, 9, 6, 3>, grid, b9, 0, d_groups, d_wg_start, n_groups);
     if (kind == -1) return STWO_MULTI(9, -1);
     return STWO_MULTI(9, 0);
   }
   if (kind == 1) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 8, 3, 0, 4, 8>, grid, b8, 0, d_groups, d_wg_start, n_groups)
                                 : launch(ctx, name, fft_pass_multi_kernel<false, 8, 3, 8, 4, 0>, grid, b8, 0, d_groups, d_wg_start, n_groups);
+  if (kind == 3) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 8, 2, 6, 7, 1>, grid, b8, 0, d_groups, d_wg_start, n_groups)
+                                : launch(ctx, name, fft_pass_multi_kernel<false, 8, 2, 7, 6, 1>, grid, b8, 0, d_groups, d_wg_start, n_groups);
+  if (kind == 4) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 8, 2, 6, 8, 2>, grid, b8, 0, d_groups, d_wg_start, n_groups)
+                                : launch(ctx, name, fft_pass_multi_kernel<false, 8, 2, 8, 6, 2>, grid, b8, 0, d_groups, d_wg_start, n_groups);
   if (kind == -1) return STWO_MULTI(8, -1);
   return STWO_MULTI(8, 0);
 #undef STWO_MULTI

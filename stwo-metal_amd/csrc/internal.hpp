@@ -40,7 +40,9 @@ struct FftPass2 {
   uint32_t vec_io;
   // The compile-time schedule the pass matches (set by make_group_passes):
   // 1 = the 12-layer pass of 3 full stages (wg8), 2 = the 8-layer strided
-  // pass of 2 full stages on a 2^13 tile (wg9), 0 = none (run-time schedule).
+  // pass of 2 full stages on a 2^13 tile (wg9), 3 / 4 / 5 = strided passes
+  // of 5 / 6 / 7 layers (a full stage and a partial one; wg8, wg8, wg9),
+  // 0 = none (run-time schedule).
   uint32_t fixed;
 };
 
@@ -55,8 +57,8 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
                       const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
                       std::vector<PassLaunch> &out);
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log);
-// `kind`: the schedule every group of the launch matches: 1 or 2 (FftPass2
-// fixed, with wg_log 8 or 9), -1 (tiles of 2^10 or more: wave-uniform
+// `kind`: the schedule every group of the launch matches: 1 to 5 (FftPass2
+// fixed, with its workgroup size), -1 (tiles of 2^10 or more: wave-uniform
 // columns) or 0 (any).
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
                      uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind);
