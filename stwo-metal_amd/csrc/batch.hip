@@ -73,10 +73,12 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   STWO_HIP_CHECK_ARG(ctx, tw && tw->device == ctx->device, "batch_create: twiddles missing or from another device");
   STWO_HIP_CHECK_ARG(ctx, tree_kind == STWO_HIP_TREE_MIXED || tree_kind == STWO_HIP_TREE_LIFTED,
                      "batch_create: unknown tree kind %d", tree_kind);
+  STWO_HIP_CHECK_ARG(ctx, hash_mode == STWO_HIP_HASH_BLAKE2S || hash_mode == STWO_HIP_HASH_BLAKE2S_M31,
+                     "batch_create: unknown hash mode %d", hash_mode);
   uint32_t max_log = 0;
   for (size_t i = 0; i < n_cols; i++) {
     STWO_HIP_CHECK_ARG(ctx, cols[i] && outs[i], "batch_create: NULL column %zu", i);
-    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 4 && log_sizes[i] + log_blowup <= 28,
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 4 && log_sizes[i] <= 28 && log_blowup <= 24 && log_sizes[i] + log_blowup <= 28,
                        "batch_create: column %zu log size %u outside [4, 28 - blowup]", i, log_sizes[i]);
     max_log = std::max(max_log, log_sizes[i] + log_blowup);
   }

@@ -1116,7 +1116,8 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx *ctx, const uint32_t *const *coef
   std::map<uint32_t, std::vector<FftCol>> groups;
   for (size_t i = 0; i < n_cols; i++) {
     STWO_HIP_CHECK_ARG(ctx, coeffs[i] && outs[i], "evaluate_polynomials: NULL column %zu", i);
-    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] + log_blowup >= 1 && log_sizes[i] + log_blowup <= 28,
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] <= 28 && log_blowup <= 28 && log_sizes[i] + log_blowup >= 1 &&
+                                log_sizes[i] + log_blowup <= 28,
                        "evaluate_polynomials: bad log size %u + %u", log_sizes[i], log_blowup);
     groups[log_sizes[i]].push_back({coeffs[i], outs[i]});
   }

@@ -188,6 +188,8 @@ static inline size_t layer_off(uint32_t l) { return tree_layer_off(l); }
 
 int launch_merkle_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const ColRef *d_cols, uint32_t n_cols,
                         uint32_t log_size, uint32_t *out, int mode, bool lifted) {
+  STWO_HIP_CHECK_ARG(ctx, mode == STWO_HIP_HASH_BLAKE2S || mode == STWO_HIP_HASH_BLAKE2S_M31,
+                     "merkle: unknown hash mode %d", mode);
   uint32_t n = 1u << log_size;
   const char *name = lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer";
   const uint32_t m31 = (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31);
@@ -215,6 +217,8 @@ static int launch_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const std::vect
 // Hashes node layers top_log-1 .. stop_log of a packed tree whose layer top_log is filled.
 int hash_node_layers(stwo_hip_ctx *ctx, uint32_t *layers, uint32_t top_log, uint32_t stop_log, int mode) {
   // layers [stop_log, top_log) are produced, from top_log - 1 down.
+  STWO_HIP_CHECK_ARG(ctx, mode == STWO_HIP_HASH_BLAKE2S || mode == STWO_HIP_HASH_BLAKE2S_M31,
+                     "merkle: unknown hash mode %d", mode);
   uint32_t l = top_log;
   while (l > stop_log) {
     uint32_t out_log = l - 1;

@@ -269,8 +269,8 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
   *out = nullptr;
   const uint32_t B = config.fri.log_blowup_factor;
   STWO_HIP_CHECK_ARG(ctx, n_cols >= 3 && n_cols <= 4096, "wide_fib_prove: %u columns (3..4096)", n_cols);
-  STWO_HIP_CHECK_ARG(ctx, B >= 1 && log_n >= 4 && log_n + B <= 27,
-                     "wide_fib_prove: log_n %u with log blowup %u (log_n >= 4, blowup >= 1, log_n + blowup <= 27)",
+  STWO_HIP_CHECK_ARG(ctx, B >= 1 && B <= 16 && log_n >= 4 && log_n <= 27 && log_n + B <= 27,
+                     "wide_fib_prove: log_n %u with log blowup %u (log_n >= 4, 1 <= blowup <= 16, log_n + blowup <= 27)",
                      log_n, B);
   STWO_HIP_CHECK_ARG(ctx, config.pow_bits <= 32, "wide_fib_prove: pow_bits %u > 32", config.pow_bits);
   const uint32_t E = log_n + B;  // committed evaluation (and lifting) log size

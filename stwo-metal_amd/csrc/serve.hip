@@ -295,7 +295,8 @@ int stwo_hip_server_create(int device, size_t hbm_budget_bytes, uint32_t max_log
 int stwo_hip_server_submit(stwo_hip_server *s, const stwo_hip_job_desc *desc, uint64_t *job_id) {
   if (!s || !desc || !job_id) return STWO_HIP_EINVAL;
   const stwo_hip_job_desc &d = *desc;
-  if (d.n_cols < 4 || d.log_size < 4 || d.log_blowup < 1 || d.log_size + d.log_blowup > s->max_log ||
+  if (d.n_cols < 4 || d.log_size < 4 || d.log_blowup < 1 || d.log_size > 28 || d.log_blowup > 16 ||
+      d.log_size + d.log_blowup > s->max_log ||
       d.fri.log_blowup_factor != d.log_blowup || d.fri.log_last_layer_degree_bound > 4 ||
       d.fri.line_fold_step < 1 || d.log_size + d.log_blowup - 1 < d.fri.log_last_layer_degree_bound + d.log_blowup)
     return STWO_HIP_EINVAL;

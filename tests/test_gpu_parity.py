@@ -621,6 +621,29 @@ def test_batch_lde_commit_vs_oracle(ctx, tree_kind, use_graph, mode):
     tw.destroy()
 
 
+def test_unknown_hash_mode_and_wrapping_blowup_rejected(ctx):
+    """Hash modes other than Blake2s / Blake2s-M31 and a log blowup whose sum
+    with the column's log size wraps in 32 bits fail with EINVAL before any
+    launch."""
+    tw = ctx.precompute_twiddles(sh.half_odds(8))
+    c = ctx.to_device(np.arange(16, dtype=np.uint32))
+    o = ctx.alloc(32)
+    layers = ctx.alloc(sh.tree_words(5))
+    try:
+        for call in (lambda: ctx.commit_on_layer(4, None, [c], o, mode=2),
+                     lambda: ctx.merkle_commit([c], [4], layers, mode=-1),
+                     lambda: ctx.evaluate_polynomials([c], [4], (1 << 32) - 1, tw, [o]),
+                     lambda: ctx.batch_create([c], [4], 1, tw, [o], layers, mode=3),
+                     lambda: ctx.batch_create([c], [4], (1 << 32) - 2, tw, [o], layers)):
+            with pytest.raises(sh.StwoHipError) as e:
+                call()
+            assert e.value.code == -1
+    finally:
+        for x in (c, o, layers):
+            x.free()
+        tw.destroy()
+
+
 def test_batch_rejects_small_columns(ctx):
     tw = ctx.precompute_twiddles(sh.half_odds(8))
     c = ctx.alloc(8)

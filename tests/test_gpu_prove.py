@@ -117,7 +117,9 @@ def test_prove_rejects_bad_arguments(ctx):
     da, db = ctx.to_device(a), ctx.to_device(b)
     tw = twiddles(ctx, 6)
     try:
-        for n_cols, log_n, cfg in [(2, 5, pcs()), (8, 3, pcs()), (8, 5, pcs(blowup=0)), (8, 5, pcs(pow_bits=33))]:
+        # the last case's log_n + blowup wraps to 4 in 32 bits
+        for n_cols, log_n, cfg in [(2, 5, pcs()), (8, 3, pcs()), (8, 5, pcs(blowup=0)), (8, 5, pcs(pow_bits=33)),
+                                   (8, 5, pcs(blowup=(1 << 32) - 1))]:
             with pytest.raises(sh.StwoHipError) as e:
                 ctx.wide_fib_prove(da, db, n_cols, log_n, tw, cfg)
             assert e.value.code == -1
