@@ -506,9 +506,11 @@ def _low_degree_qm31_eval(ctx, log_size, log_blowup, seed, tw):
 
 # 2^21: layers above 2^19 take node-level launches before the top-levels
 # kernel; fold steps of 5 and 6: the unfused fold path (more than 4 steps),
-# with layers below 2^8 committed outside the single-workgroup tail.
+# with layers below 2^8 committed outside the single-workgroup tail.  2^22
+# with a last layer of 2^4: the benchmark's FRI chain at its full size.
 @pytest.mark.parametrize("log_size,cfg", [(14, (1, 3, 3, 1)), (16, (2, 0, 3, 3)), (12, (1, 2, 3, 2)),
-                                          (21, (1, 0, 3, 1)), (13, (1, 1, 3, 5)), (11, (2, 0, 3, 6))])
+                                          (21, (1, 0, 3, 1)), (13, (1, 1, 3, 5)), (11, (2, 0, 3, 6)),
+                                          (22, (1, 3, 3, 1))])
 def test_fri_commit_vs_oracle(ctx, log_size, cfg):
     lb, lld, nq, step = cfg
     tw = ctx.precompute_twiddles(sh.half_odds(log_size - 1))
