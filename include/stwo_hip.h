@@ -51,7 +51,8 @@ extern "C" {
 
 /* Hash output mode: Blake2sMerkleHasher or Blake2sM31MerkleHasher
  * (STWO/core/vcs/blake2_merkle.rs:8-10): the M31 variant reduces each output
- * word mod P (reduce_to_m31, STWO/core/vcs/blake2_hash.rs:111). */
+ * word mod P (reduce_to_m31, STWO/core/vcs/blake2_hash.rs:111).  Any other
+ * value is STWO_HIP_EINVAL. */
 #define STWO_HIP_HASH_BLAKE2S 0
 #define STWO_HIP_HASH_BLAKE2S_M31 1
 
