@@ -148,8 +148,8 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   void *dmem = nullptr;
   // Upper bound on descriptors: one per group and pass (2 directions, at most
   // 4 passes), plus the 256-byte alignment of two tables per launch (2
-  // directions x 4 levels x 2 workgroup sizes x at most 4 schedule kinds).
-  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 4 * 2 * 256 + 4096;
+  // directions x 4 levels x 2 workgroup sizes x at most 14 schedule kinds).
+  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 14 * 2 * 256 + 4096;
   if (hipMalloc(&dmem, tables_bytes + desc_bytes) != hipSuccess) {
     (void)hipGetLastError();
     return set_error(ctx, STWO_HIP_ENOMEM, "batch_create: out of device memory for tables");

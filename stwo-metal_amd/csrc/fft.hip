@@ -445,10 +445,10 @@ __device__ __forceinline__ void lds_get(const uint32_t *L, uint32_t b0, uint32_t
 // of every one of the 2^rep_log output blocks: the coefficients are read
 // once from HBM (repeats hit L2) and the blocks differ only in twiddles.
 // One stage of a compile-time schedule: the register group at tile bit B0,
-// its register bits [0, RHI) (all 4 layers, or the low RHI of them).  PREV is
+// its register bits [RLO, RHI) (all 4 layers, or some of them).  PREV is
 // the previous stage's group, or -1 when it is only known at run time (the
 // first stage follows the load layout).
-template <bool INV, int B0, int PREV, int RHI = 4, typename GB, typename GI>
+template <bool INV, int B0, int PREV, int RHI = 4, int RLO = 0, typename GB, typename GI>
 __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16], uint32_t *L, uint32_t &cur,
                                             uint32_t local, GB group_base, GI gidx) {
   // Keep the scheduler from hoisting later stages' twiddle loads above this
@@ -459,7 +459,7 @@ __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16]
   const uint32_t gb = gidx(base);
   const uint32_t sh = B0 - p.w + p.lo;
   StageTw t;
-  load_stage_twiddles(t, p.tw_end, p.N, sh, gb, 0, RHI);
+  load_stage_twiddles(t, p.tw_end, p.N, sh, gb, RLO, RHI);
   if constexpr (PREV >= 0) {
     if constexpr (PREV != B0) {
       const uint32_t cb = ((local >> PREV) << (PREV + 4)) | (local & ((1u << PREV) - 1));
@@ -477,7 +477,7 @@ __device__ __forceinline__ void fixed_stage(const FftPass2 &p, uint32_t (&x)[16]
     }
   }
   cur = B0;
-  stage_layers<INV, 0, RHI>(x, t);
+  stage_layers<INV, RLO, RHI>(x, t);
 }
 
 // The fixed schedules need about 125 VGPRs (4 waves per SIMD); asking for 5
@@ -587,7 +587,10 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         }
       }
     }
-    if constexpr (NS == 2 && B0C != 0) {
+    if constexpr (NS == 1) {
+      // One stage: register bits [B0B, B0C) of the group at tile bit B0A.
+      fixed_stage<INV, B0A, -1, B0C, B0B>(p, x, L, cur, local, group_base, gidx);
+    } else if constexpr (NS == 2 && B0C != 0) {
       // A strided pass of 4 + B0C layers: the group at tile bit 6 (= w)
       // applies only its low B0C register bits (first stage when inverse).
       fixed_stage<INV, B0A, -1, INV ? B0C : 4>(p, x, L, cur, local, group_base, gidx);
@@ -797,9 +800,15 @@ static bool partial_schedule(const FftPass2 &p, bool inverse, uint32_t m) {
          b.r_lo == part.r_lo && b.r_hi == part.r_hi;
 }
 
+// The single-stage passes of 1 to 4 strided layers on a 2^10 tile (w = 6,
+// the group at tile bit 6, which is also the load and store layout): the
+// stage's register bits [r_lo, r_hi) for kinds 6 + k.
+static const uint8_t kShort[7][2] = {{0, 4}, {1, 4}, {2, 4}, {3, 4}, {0, 1}, {0, 2}, {0, 3}};
+
 // FftPass2::fixed for a pass: 1 = 12 layers in 3 full stages (wg8), 2 = 8
 // strided layers in 2 full stages on a 2^13 tile (wg9), 3 / 4 / 5 = 5 / 6 / 7
-// strided layers (partial_schedule; wg8, wg8, wg9), 0 = run-time schedule.
+// strided layers (partial_schedule; wg8, wg8, wg9), 6..12 = kShort (wg8),
+// 0 = run-time schedule.
 static uint32_t fixed_schedule(const FftPass2 &p, bool inverse, uint32_t wg_log) {
   static const uint32_t kFwd12[3] = {8, 4, 0}, kInv12[3] = {0, 4, 8}, kFwd13[2] = {9, 5}, kInv13[2] = {5, 9};
   if (wg_log == 8 && p.T == T_FIXED12 && p.w == 0 && p.lo == 0 && p.in_log >= T_FIXED12 &&
@@ -809,6 +818,9 @@ static uint32_t fixed_schedule(const FftPass2 &p, bool inverse, uint32_t wg_log)
   if (wg_log == 8 && partial_schedule(p, inverse, 5)) return 3;
   if (wg_log == 8 && partial_schedule(p, inverse, 6)) return 4;
   if (wg_log == 9 && partial_schedule(p, inverse, 7)) return 5;
+  if (wg_log == 8 && p.T == 10 && p.w == 6 && p.n_stages == 1 && p.st[0].b0 == 6)
+    for (uint32_t k = 0; k < 7; k++)
+      if (p.st[0].r_lo == kShort[k][0] && p.st[0].r_hi == kShort[k][1]) return 6 + k;
   return 0;
 }
 
@@ -884,6 +896,13 @@ int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t
   if (wg_log == 9 && p.fixed == 2)
     return inverse ? launch(ctx, "ifft_pass_wg9", fft_pass_kernel<true, 9, 2, 5, 9>, dim3(wgs), dim3(512), 0, p)
                    : launch(ctx, "rfft_pass_wg9", fft_pass_kernel<false, 9, 2, 9, 5>, dim3(wgs), dim3(512), 0, p);
+#define STWO_SHORT(K, RLO, RHI)                                                                                  \
+  if (wg_log == 8 && p.fixed == K)                                                                              \
+    return inverse ? launch(ctx, "ifft_pass_wg8", fft_pass_kernel<true, 8, 1, 6, RLO, RHI>, dim3(wgs), dim3(256), 0, p) \
+                   : launch(ctx, "rfft_pass_wg8", fft_pass_kernel<false, 8, 1, 6, RLO, RHI>, dim3(wgs), dim3(256), 0, p);
+  STWO_SHORT(6, 0, 4) STWO_SHORT(7, 1, 4) STWO_SHORT(8, 2, 4) STWO_SHORT(9, 3, 4)
+  STWO_SHORT(10, 0, 1) STWO_SHORT(11, 0, 2) STWO_SHORT(12, 0, 3)
+#undef STWO_SHORT
   // Profiling names follow the instantiation: *_wg9 = fft_pass_kernel<INV, 9, ...>.
   // Tiles of 2^10 or more span whole waves (NS = -1: wave-uniform columns).
   const bool uni = p.T >= 10;
@@ -924,6 +943,16 @@ int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, 
                                 : launch(ctx, name, fft_pass_multi_kernel<false, 8, 2, 7, 6, 1>, grid, b8, 0, d_groups, d_wg_start, n_groups);
   if (kind == 4) return inverse ? launch(ctx, name, fft_pass_multi_kernel<true, 8, 2, 6, 8, 2>, grid, b8, 0, d_groups, d_wg_start, n_groups)
                                 : launch(ctx, name, fft_pass_multi_kernel<false, 8, 2, 8, 6, 2>, grid, b8, 0, d_groups, d_wg_start, n_groups);
+  switch (kind) {
+    case 6: return STWO_MULTI(8, 1, 6, 0, 4);
+    case 7: return STWO_MULTI(8, 1, 6, 1, 4);
+    case 8: return STWO_MULTI(8, 1, 6, 2, 4);
+    case 9: return STWO_MULTI(8, 1, 6, 3, 4);
+    case 10: return STWO_MULTI(8, 1, 6, 0, 1);
+    case 11: return STWO_MULTI(8, 1, 6, 0, 2);
+    case 12: return STWO_MULTI(8, 1, 6, 0, 3);
+    default: break;
+  }
   if (kind == -1) return STWO_MULTI(8, -1);
   return STWO_MULTI(8, 0);
 #undef STWO_MULTI

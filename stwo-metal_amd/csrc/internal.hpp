@@ -42,7 +42,8 @@ struct FftPass2 {
   // 1 = the 12-layer pass of 3 full stages (wg8), 2 = the 8-layer strided
   // pass of 2 full stages on a 2^13 tile (wg9), 3 / 4 / 5 = strided passes
   // of 5 / 6 / 7 layers (a full stage and a partial one; wg8, wg8, wg9),
-  // 0 = none (run-time schedule).
+  // 6..12 = single-stage passes of 1 to 4 strided layers on a 2^10 tile
+  // (wg8), 0 = none (run-time schedule).
   uint32_t fixed;
 };
 
