@@ -321,6 +321,73 @@ def measured_traffic(kernel):
     return measured_counters(kernel).get("hbm_bytes_per_launch")
 
 
+def compulsory_bytes():
+    """Compulsory HBM bytes of one step, per transform (twiddle reads and the
+    intermediate passes excluded): the IFFT reads and writes each column's
+    2^n words once (8 * 2^n B), the RFFT reads 2^n coefficients and writes
+    2^(n+b) evaluations once (4 * 2^n + 4 * 2^(n+b) B); Merkle leaves read
+    4 B per element per column and write 32 B per leaf, node layers read
+    64 B and write 32 B per node."""
+    n, b = LOG_N, LOG_BLOWUP
+    n_out = 1 << (n + b)
+    ifft = 8.0 * (1 << n) * N_COLS
+    rfft = (4.0 * (1 << n) + 4.0 * n_out) * N_COLS
+    return {"ifft": ifft, "rfft": rfft, "lde": ifft + rfft,
+            "merkle_leaves": 4.0 * N_COLS * n_out + 32.0 * n_out, "merkle_nodes": 96.0 * (n_out - 1)}
+
+
+def transform_of(name):
+    """The step's transform a launch (profiling name) belongs to."""
+    if name.startswith("ifft_pass"):
+        return "ifft"
+    if name.startswith("rfft_pass"):
+        return "rfft"
+    if name.startswith("lde_pass"):  # IFFT and RFFT passes fused into one launch
+        return "lde"
+    return {"merkle_commit_on_layer": "merkle_leaves", "merkle_nodes": "merkle_nodes"}.get(name)
+
+
+def compulsory_roofline(kernels, steps):
+    """Roofline point of the dominant launch (most device time per step) on
+    compulsory bytes: the compulsory bytes of the transform it belongs to
+    over its mean launch time -- the rate the whole transform would reach if
+    this launch alone moved its bytes.  Also the same rate for the IFFT, the
+    RFFT and the whole LDE over their summed pass times.  Adds each kernel's
+    own read + write per launch ("pass_io_GBps", from the library's pass
+    planner) to `kernels`."""
+    comp = compulsory_bytes()
+    for inverse, n, N in ((True, LOG_N, LOG_N), (False, LOG_N, LOG_N + LOG_BLOWUP)):
+        io = {}
+        for p in sh.fft_plan(n, N, inverse):
+            name = "%s_pass_wg%d" % ("ifft" if inverse else "rfft", p["wg"])
+            io[name] = io.get(name, 0.0) + 4.0 * N_COLS * ((1 << min(p["in_log"], N - p["rep"])) + (1 << N))
+        for k, v in io.items():
+            if k in kernels:
+                kernels[k]["pass_io_GBps"] = round(v / (kernels[k]["ms_total"] / steps * 1e-3) / 1e9, 1)
+    ms = {t: 0.0 for t in comp}
+    for k, v in kernels.items():
+        t = transform_of(k)
+        if t:
+            ms[t] += v["ms_total"] / steps
+    fused = ms["lde"] > 0
+    ms["lde"] += ms["ifft"] + ms["rfft"]
+    transforms = {}
+    for t in (("lde",) if fused else ("ifft", "rfft", "lde")):
+        if ms[t] > 0:
+            a = comp[t] / (ms[t] * 1e-3) / 1e9
+            transforms[t] = {"compulsory_bytes": int(comp[t]), "ms_per_step": ms[t], "achieved": a,
+                             "frac": a / HBM_PEAK_GBS}
+    dom = max((k for k in kernels if transform_of(k)), key=lambda k: kernels[k]["ms_total"])
+    t = transform_of(dom)
+    launches = kernels[dom]["launches"] / steps
+    launch_ms = kernels[dom]["ms_total"] / kernels[dom]["launches"]
+    dom_bytes = comp[t] / launches
+    traffic = measured_traffic(dom)
+    return {"kernel": dom, "algorithmic_of": t, "launch_ms": launch_ms, "algorithmic_bytes": dom_bytes,
+            "achieved": dom_bytes / (launch_ms * 1e-3) / 1e9, "traffic": traffic, "transforms": transforms,
+            "lde_ms_per_step": ms["lde"]}
+
+
 def hbm_kernels(ctx, log_n=26, runs=3):
     """The kernels the HBM bandwidth bounds, at sizes far past the caches:
     bit_reverse_column of a 2^27-word column (8 B per element: one read and
@@ -445,32 +512,10 @@ def main():
 
     # Per-kernel device time (HIP events around every launch on the context stream).
     kernels = {k: {"launches": v[0], "ms_total": round(v[1], 3)} for k, v in prof.items() if k != "twiddle_tree"}
-    # Algorithmic HBM bytes per step for each kernel (twiddle reads excluded):
-    # an FFT pass reads its input domain once and writes its output domain
-    # once (the pass list comes from the library's planner); Merkle leaves
-    # read 4 B per element per column and write 32 B per leaf, node layers
-    # read 64 B and write 32 B per node.
-    alg = {}
-    for inverse, n, N in ((True, LOG_N, LOG_N), (False, LOG_N, LOG_N + LOG_BLOWUP)):
-        for p in sh.fft_plan(n, N, inverse):
-            name = "%s_pass_wg%d" % ("ifft" if inverse else "rfft", p["wg"])
-            rd = 4.0 * N_COLS * (1 << min(p["in_log"], N - p["rep"]))
-            wr = 4.0 * N_COLS * (1 << N)
-            alg[name] = alg.get(name, 0.0) + rd + wr
-    n_out = 1 << (LOG_N + LOG_BLOWUP)
-    alg["merkle_commit_on_layer"] = 4.0 * N_COLS * n_out + 32.0 * n_out
-    alg["merkle_nodes"] = 96.0 * (n_out - 1)
-    for k in kernels:
-        if k in alg:
-            kernels[k]["algorithmic_GBps"] = round(alg[k] / (kernels[k]["ms_total"] / args.steps * 1e-3) / 1e9, 1)
-    # The dominant kernel: the largest device time per step.  Its roofline
-    # point is per launch: algorithmic bytes of one launch / its mean duration.
-    dom_name = max((k for k in kernels if k in alg), key=lambda k: kernels[k]["ms_total"])
-    dom_launches_per_step = kernels[dom_name]["launches"] / args.steps
-    dom_launch_ms = kernels[dom_name]["ms_total"] / kernels[dom_name]["launches"]
-    dom_bytes = alg[dom_name] / dom_launches_per_step
-    achieved = dom_bytes / (dom_launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(dom_name)
+    roof = compulsory_roofline(kernels, args.steps)
+    dom_name = roof["kernel"]
+    dom_launch_ms = roof["launch_ms"]
+    traffic = roof["traffic"]
     # The same kernel against the integer-VALU issue ceiling, which is what
     # bounds it: its VALU wave-instructions per launch (committed counter run)
     # over the live launch time.
@@ -486,14 +531,14 @@ def main():
     # kernels' algorithmic bytes over their device time; Blake2s
     # compressions (2^L * ceil(4 C / 64) leaf blocks + 2^L - 1 nodes) per
     # second of step time and of Merkle kernel time.
-    fft_keys = [k for k in kernels if "fft_pass" in k]
-    fft_ms = sum(kernels[k]["ms_total"] for k in fft_keys) / args.steps
+    n_out = 1 << (LOG_N + LOG_BLOWUP)
+    fft_ms = roof["lde_ms_per_step"]
     merkle_ms = sum(kernels[k]["ms_total"] for k in ("merkle_commit_on_layer", "merkle_nodes") if k in kernels) / args.steps
     n_compress = n_out * -(-4 * N_COLS // 64) + (n_out - 1)
     side = {
         "evaluated_melem_s": round(N_COLS * n_out / (total / args.steps) / 1e6, 1),
         "fft_ms_per_step": round(fft_ms, 3),
-        "fft_algorithmic_GBps": round(sum(alg[k] for k in fft_keys) / (fft_ms * 1e-3) / 1e9, 1) if fft_ms else None,
+        "fft_compulsory_GBps": round(roof["transforms"]["lde"]["achieved"], 1),
         "merkle_ms_per_step": round(merkle_ms, 3),
         "compressions_per_step": n_compress,
         "compressions_per_s_step": round(n_compress / (total / args.steps), 1),
@@ -555,16 +600,20 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": "valu",
             "kernel": dom_name,
             "rocprof_name": ROCPROF_NAMES.get(dom_name, dom_name),
             "launch_ms": round(dom_launch_ms, 4),
-            "algorithmic_bytes": int(dom_bytes),
-            "achieved": round(achieved, 1),
+            "algorithmic_bytes": int(roof["algorithmic_bytes"]),
+            "algorithmic_of": roof["algorithmic_of"],
+            "achieved": round(roof["achieved"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(roof["achieved"] / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_over_algorithmic": round(traffic / roof["algorithmic_bytes"], 3) if traffic else None,
+            "transforms": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                           for k, v in roof["transforms"].items()},
         },
         "valu_roofline": valu_roofline,
         "side_rates": side,

@@ -95,6 +95,12 @@ int set_error(stwo_hip_ctx *ctx, int code, const char *fmt, ...);
 // stream by the next flush_uploads (every launch() makes one), together with
 // the other tables written since the last one.
 int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr);
+// Several tables for one launch sequence, from one contiguous range of the
+// ring (they never overwrite each other before their kernels are enqueued).
+// Every table a sequence of launches reads must come from one such call
+// made before the sequence's first launch.  Sets larger than half the ring
+// go through a scratch buffer with a synchronous copy.
+int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const size_t *bytes, void **dptrs);
 
 // Enqueues the copy of the pending tables (none: nothing to do).
 int flush_uploads(stwo_hip_ctx *ctx);

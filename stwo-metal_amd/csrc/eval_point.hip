@@ -215,10 +215,16 @@ int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, con
   }
   const uint32_t n_basis = 1u << std::min(max_log, kEvalBlockLog);
   void *d_tab = nullptr, *d_work = nullptr, *d_basis = nullptr, *d_part = nullptr, *d_out = nullptr;
-  int rc = upload_table(ctx, tab.data(), tab.size() * sizeof(EvalCol), &d_tab);
-  if (rc) return rc;
-  rc = upload_table(ctx, work.data(), work.size() * sizeof(EvalWork), &d_work);
-  if (rc) return rc;
+  int rc;
+  {
+    const void *srcs[2] = {tab.data(), work.data()};
+    const size_t bytes[2] = {tab.size() * sizeof(EvalCol), work.size() * sizeof(EvalWork)};
+    void *d[2];
+    rc = upload_tables(ctx, 2, srcs, bytes, d);
+    if (rc) return rc;
+    d_tab = d[0];
+    d_work = d[1];
+  }
   rc = get_scratch(ctx, "eval_point_basis", (size_t)n_basis * 16, &d_basis);
   if (rc) return rc;
   rc = get_scratch(ctx, "eval_point_partials", total * 16, &d_part);

@@ -959,15 +959,8 @@ int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, 
 }
 
 // Radix-16 path for N >= 4: one launch per pass over all columns of the group.
-static int run_fft_group2(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, void *dtab, uint32_t n, uint32_t N,
-                          const uint32_t *tw_end, bool inverse) {
-  void *dtab_out = dtab;
-  if (!inverse) {
-    std::vector<FftCol> c2(cols);
-    for (auto &c : c2) c.in = c.out;
-    int rc = upload_table(ctx, c2.data(), c2.size() * sizeof(FftCol), &dtab_out);
-    if (rc) return rc;
-  }
+static int run_fft_group2(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, void *dtab, void *dtab_out,
+                          uint32_t n, uint32_t N, const uint32_t *tw_end, bool inverse) {
   bool aligned16 = true;
   for (const FftCol &c : cols)
     if ((((uintptr_t)c.in) | ((uintptr_t)c.out)) & 15) aligned16 = false;
@@ -989,9 +982,21 @@ static int run_fft_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, uin
                          const stwo_hip_twiddles *tw, bool inverse, stwo_hip_coset dom) {
   if (cols.empty()) return STWO_HIP_OK;
   STWO_HIP_CHECK_ARG(ctx, cols.size() <= 65535, "too many columns in one group (%zu)", cols.size());
-  void *dtab = nullptr;
-  int rc = upload_table(ctx, cols.data(), cols.size() * sizeof(FftCol), &dtab);
-  if (rc) return rc;
+  // The group's table and, forward, the in-place table of the passes after
+  // the first (entries {out, out}), uploaded together.
+  void *dtab = nullptr, *dtab_out = nullptr;
+  int rc;
+  {
+    std::vector<FftCol> c2(cols);
+    for (auto &c : c2) c.in = c.out;
+    const void *srcs[2] = {cols.data(), c2.data()};
+    const size_t bytes[2] = {cols.size() * sizeof(FftCol), cols.size() * sizeof(FftCol)};
+    void *d[2];
+    rc = upload_tables(ctx, inverse ? 1 : 2, srcs, bytes, d);
+    if (rc) return rc;
+    dtab = d[0];
+    dtab_out = inverse ? d[0] : d[1];
+  }
   if (N <= 2) {
     SmallFft a{};
     a.cols = (const FftCol *)dtab;
@@ -1014,20 +1019,13 @@ static int run_fft_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, uin
     }
     return launch(ctx, "fft_small", fft_small_kernel, dim3((uint32_t)cols.size()), dim3(64), 0, a);
   }
-  if (N >= 4) return run_fft_group2(ctx, cols, dtab, n, N, (inverse ? tw->d_itw2 : tw->d_tw2) + tw->len, inverse);
+  if (N >= 4)
+    return run_fft_group2(ctx, cols, dtab, dtab_out, n, N, (inverse ? tw->d_itw2 : tw->d_tw2) + tw->len, inverse);
   const uint32_t *tree = inverse ? tw->d_itw : tw->d_tw;
   const uint32_t *tw_end = tree + tw->len;
   uint32_t L = inverse ? N : n;
   std::vector<PassPlan> plan = plan_passes(N, L);
   if (inverse) std::reverse(plan.begin(), plan.end());
-  // Output-side table (in place after the first forward pass).
-  void *dtab_out = dtab;
-  if (!inverse && plan.size() > 1) {
-    std::vector<FftCol> c2(cols);
-    for (auto &c : c2) c.in = c.out;
-    rc = upload_table(ctx, c2.data(), c2.size() * sizeof(FftCol), &dtab_out);
-    if (rc) return rc;
-  }
   uint32_t n_inv = m31_inv((uint32_t)(((uint64_t)1 << N) % P));
   for (size_t k = 0; k < plan.size(); k++) {
     const PassPlan &pp = plan[k];

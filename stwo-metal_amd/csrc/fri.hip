@@ -563,6 +563,9 @@ static int check_fri_args(stwo_hip_ctx *ctx, const stwo_hip_fri_config &cfg, uin
                           const stwo_hip_twiddles *tw) {
   const uint32_t last_log = cfg.log_last_layer_degree_bound + cfg.log_blowup_factor;
   STWO_HIP_CHECK_ARG(ctx, cfg.line_fold_step >= 1, "Line fold step must be positive.");
+  // A step above the largest line domain (log 27) can only overshoot; larger
+  // values would also wrap the layer-size arithmetic of the schedule.
+  STWO_HIP_CHECK_ARG(ctx, cfg.line_fold_step <= 28, "line_fold_step %u exceeds 28", cfg.line_fold_step);
   STWO_HIP_CHECK_ARG(ctx, cfg.log_last_layer_degree_bound <= 10, "log_last_layer_degree_bound must be <= 10");
   STWO_HIP_CHECK_ARG(ctx, cfg.log_blowup_factor >= 1 && cfg.log_blowup_factor <= 16, "log_blowup_factor not in 1..=16");
   STWO_HIP_CHECK_ARG(ctx, log_size >= 3 && log_size - 1 >= last_log && log_size <= 28,

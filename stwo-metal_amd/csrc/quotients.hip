@@ -238,10 +238,15 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
     pts.push_back(z);
   }
   void *d_pts = nullptr, *d_ents = nullptr;
-  int rc = upload_table(ctx, pts.data(), std::max<size_t>(1, pts.size()) * sizeof(QuotPoint), &d_pts);
-  if (rc) return rc;
-  rc = upload_table(ctx, ents.data(), std::max<size_t>(1, ents.size()) * sizeof(QuotEntry), &d_ents);
-  if (rc) return rc;
+  {
+    const void *srcs[2] = {pts.data(), ents.data()};
+    const size_t bytes[2] = {pts.size() * sizeof(QuotPoint), ents.size() * sizeof(QuotEntry)};
+    void *d[2];
+    int rc = upload_tables(ctx, 2, srcs, bytes, d);
+    if (rc) return rc;
+    d_pts = d[0];
+    d_ents = d[1];
+  }
   GenTable gen;
   cpoint cur = {2, 1268011823u};
   for (int k = 0; k < 31; k++) {

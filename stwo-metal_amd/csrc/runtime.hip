@@ -134,26 +134,60 @@ int flush_uploads(stwo_hip_ctx *ctx) {
   return STWO_HIP_OK;
 }
 
-int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr) {
+static size_t table_span(size_t bytes) {
   size_t sz = (bytes + 255) & ~(size_t)255;
-  if (sz == 0) sz = 256;
-  if (sz > ctx->arena_cap) return set_error(ctx, STWO_HIP_EINVAL, "table of %zu bytes exceeds the arena", bytes);
+  return sz ? sz : 256;
+}
+
+int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const size_t *bytes, void **dptrs) {
+  size_t total = 0;
+  for (size_t i = 0; i < n; i++) total += table_span(bytes[i]);
+  if (total > ctx->arena_cap / 2) {
+    // Too large for the ring: a scratch buffer written synchronously (the
+    // stream is drained first, since kernels of an earlier call may still
+    // read the buffer's previous contents).
+    void *d = nullptr;
+    int rc = get_scratch(ctx, "large_tables", total, &d);
+    if (rc) return rc;
+    rc = flush_uploads(ctx);
+    if (rc) return rc;
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    size_t off = 0;
+    for (size_t i = 0; i < n; i++) {
+      if (bytes[i]) STWO_HIP_TRY(ctx, hipMemcpy((uint8_t *)d + off, srcs[i], bytes[i], hipMemcpyHostToDevice));
+      dptrs[i] = (uint8_t *)d + off;
+      off += table_span(bytes[i]);
+    }
+    return STWO_HIP_OK;
+  }
+  // One contiguous range of the ring for all of the call's tables: a table
+  // handed out earlier is only safe from being overwritten once the kernels
+  // that read it are enqueued (a later copy then follows them in stream
+  // order), so the tables one launch sequence needs must not wrap onto each
+  // other.
   size_t off = ctx->arena_head;
-  if (off + sz > ctx->arena_cap) off = 0;
-  // A table that does not extend the pending range (the ring wrapped) starts
-  // a new one: the pending bytes are copied first, so that the wait below
-  // also covers them.
+  if (off + total > ctx->arena_cap) off = 0;
+  // A range that does not extend the pending one (the ring wrapped) starts a
+  // new one: the pending bytes are copied first, so that the wait below also
+  // covers them.
   int rc = STWO_HIP_OK;
   if (ctx->pend_end > ctx->pend_begin && off != ctx->pend_end) rc = flush_uploads(ctx);
   if (rc) return rc;
-  rc = arena_wait_range(ctx, off, off + sz);
+  rc = arena_wait_range(ctx, off, off + total);
   if (rc) return rc;
-  memcpy(ctx->h_arena + off, src, bytes);
   if (ctx->pend_end <= ctx->pend_begin) ctx->pend_begin = off;
-  ctx->pend_end = off + sz;
-  ctx->arena_head = off + sz;
-  *dptr = ctx->d_arena + off;
+  for (size_t i = 0; i < n; i++) {
+    if (bytes[i]) memcpy(ctx->h_arena + off, srcs[i], bytes[i]);
+    dptrs[i] = ctx->d_arena + off;
+    off += table_span(bytes[i]);
+  }
+  ctx->pend_end = off;
+  ctx->arena_head = off;
   return STWO_HIP_OK;
+}
+
+int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr) {
+  return upload_tables(ctx, 1, &src, &bytes, dptr);
 }
 
 int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr) {

@@ -298,7 +298,7 @@ int stwo_hip_server_submit(stwo_hip_server *s, const stwo_hip_job_desc *desc, ui
   if (d.n_cols < 4 || d.log_size < 4 || d.log_blowup < 1 || d.log_size > 28 || d.log_blowup > 16 ||
       d.log_size + d.log_blowup > s->max_log ||
       d.fri.log_blowup_factor != d.log_blowup || d.fri.log_last_layer_degree_bound > 4 ||
-      d.fri.line_fold_step < 1 || d.log_size + d.log_blowup - 1 < d.fri.log_last_layer_degree_bound + d.log_blowup)
+      d.fri.line_fold_step < 1 || d.fri.line_fold_step > 28 || d.log_size + d.log_blowup - 1 < d.fri.log_last_layer_degree_bound + d.log_blowup)
     return STWO_HIP_EINVAL;
   auto j = std::make_shared<Job>();
   j->desc = d;

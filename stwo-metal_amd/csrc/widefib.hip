@@ -164,12 +164,16 @@ int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *co
     dinv[bitrev32(i, 1)] = m31_inv(x);
   }
   void *d_cols = nullptr, *d_pw = nullptr, *d_dinv = nullptr;
-  int rc = upload_table(ctx, cols, n_cols * sizeof(uint32_t *), &d_cols);
-  if (rc) return rc;
-  rc = upload_table(ctx, powers.data(), powers.size() * 4, &d_pw);
-  if (rc) return rc;
-  rc = upload_table(ctx, dinv, sizeof dinv, &d_dinv);
-  if (rc) return rc;
+  {
+    const void *srcs[3] = {cols, powers.data(), dinv};
+    const size_t bytes[3] = {n_cols * sizeof(uint32_t *), powers.size() * 4, sizeof dinv};
+    void *d[3];
+    int rc = upload_tables(ctx, 3, srcs, bytes, d);
+    if (rc) return rc;
+    d_cols = d[0];
+    d_pw = d[1];
+    d_dinv = d[2];
+  }
   const uint64_t n = 1ull << eval_log;
   return launch(ctx, "wide_fib_constraint_quotients", wide_fib_quotients_kernel, dim3((uint32_t)((n + 255) / 256)),
                 dim3(256), 0, (const uint32_t *const *)d_cols, n_cols, eval_log, log_n, (const uint32_t *)d_pw,

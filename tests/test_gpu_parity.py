@@ -541,6 +541,49 @@ def test_fri_rejects_high_degree(ctx):
     tw.destroy()
 
 
+def test_fri_rejects_oversized_fold_step(ctx):
+    """line_fold_step above 28 is rejected before any launch (a step near
+    2^32 would wrap the schedule's layer arithmetic)."""
+    n = 10
+    tw = ctx.precompute_twiddles(sh.half_odds(n - 1))
+    col = [ctx.to_device(rand(1 << n, 68, k)) for k in range(4)]
+    for step in (29, 1 << 31, 0xFFFFFFFF):
+        with pytest.raises(sh.StwoHipError) as e:
+            ctx.fri_commit(bytes(32), 0, sh.FriConfig(1, 1, 3, step), col, n, tw)
+        assert e.value.code == -1
+    for c in col:
+        c.free()
+    tw.destroy()
+
+
+def test_table_ring_wraps_and_large_tables(ctx):
+    """Per-call tables through the pinned ring: eval_at_point calls whose
+    column and work tables (16 B per column, 16 B per 2^11-coefficient block
+    of each run of same-size columns) take about 3 MiB of the 8 MiB ring
+    wrap it repeatedly, and a call whose tables exceed half of it (7.7 MB)
+    takes the synchronous scratch path; every value must equal the value of
+    the same column evaluated alone (tiny tables)."""
+    a, b = ctx.alloc(1 << 18), ctx.alloc(1 << 17)
+    ctx.fill_random(a, 1 << 18, 31, 0)
+    ctx.fill_random(b, 1 << 17, 31, 1)
+    pt = secure_point_for_tables()
+    va = ctx.eval_at_point([a], [18], pt)[0]
+    vb = ctx.eval_at_point([b], [17], pt)[0]
+    for n_cols in (2000, 1600, 2000, 1200, 2000, 5000, 300):
+        cols = [a if i % 2 == 0 else b for i in range(n_cols)]
+        logs = [18 if i % 2 == 0 else 17 for i in range(n_cols)]
+        got = ctx.eval_at_point(cols, logs, pt)
+        assert np.array_equal(got[0::2], np.tile(va, (len(got[0::2]), 1)))
+        assert np.array_equal(got[1::2], np.tile(vb, (len(got[1::2]), 1)))
+    a.free()
+    b.free()
+
+
+def secure_point_for_tables():
+    # any point works for the equality check; a fixed pair of QM31 coordinates
+    return ([11, 22, 33, 44], [55, 66, 77, 88])
+
+
 # ------------------------------------------------- first benchmark workload --
 
 def test_config1_lde_merkle_full(ctx):
