@@ -437,31 +437,60 @@ def hbm_kernels(ctx, log_n=26, runs=3):
     return out
 
 
-def cpu_baseline(sample_cols=N_COLS, runs=3):
-    """The oracle (C, OpenMP over columns and nodes) on the same workload, on the host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(gpu_root=None, runs=6):
+    """The SimdBackend-equivalent C++ port (oracle/simd_port.cpp: packed M31
+    on AVX-512 or AVX2, vecwise + radix-8 FFT layers cached below 2^16,
+    W-way Blake2s; OpenMP over columns, blocks and leaf chunks on every host
+    core) on the same workload: interpolate 256 x 2^20, evaluate at blowup
+    4, Blake2s Merkle commit.  One warm-up, then the median of `runs`; the
+    root must equal the GPU's."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
+    import simd_port
     init = oracle.half_odds_initial(LOG_N + LOG_BLOWUP - 1)
     tw, itw = oracle.precompute_twiddles(init, LOG_N + LOG_BLOWUP - 1)
-    pristine = [oracle.fill_random(1 << LOG_N, 2, c) for c in range(sample_cols)]
+    port = simd_port.Port(tw, itw)
+    pristine = [oracle.fill_random(1 << LOG_N, 2, c) for c in range(N_COLS)]
+    work = [np.empty_like(p) for p in pristine]
+    outs = [np.empty(1 << (LOG_N + LOG_BLOWUP), dtype=np.uint32) for _ in range(N_COLS)]
+    layers = np.empty(((2 << (LOG_N + LOG_BLOWUP)) - 1) * 8, dtype=np.uint32)
     times = []
-    for _ in range(runs):
-        evs = [p.copy() for p in pristine]
+    for it in range(runs + 1):
+        for w, p in zip(work, pristine):
+            np.copyto(w, p)
         t0 = time.perf_counter()
-        oracle.interpolate_columns(evs, itw)
-        outs = oracle.evaluate_polynomials(evs, LOG_BLOWUP, tw)
-        oracle.merkle_commit(outs)
-        times.append(time.perf_counter() - t0)
-        del outs
+        port.interpolate_columns(work, LOG_N)
+        port.evaluate_columns(work, LOG_N, LOG_BLOWUP, outs)
+        port.merkle_commit(outs, LOG_N + LOG_BLOWUP, layers)
+        if it:
+            times.append(time.perf_counter() - t0)
     dt = sorted(times)[len(times) // 2]
-    return {
-        "value": round(sample_cols * (1 << LOG_N) / dt / 1e6, 3),
+    res = {
+        "value": round(N_COLS * (1 << LOG_N) / dt / 1e6, 3),
         "unit": "Melem/s",
-        "cores": oracle.num_threads(),
+        "cores": port.threads,
         "kind": "port",
-        "sample": "%d columns x 2^20 -> blowup 4 -> Blake2s Merkle commit (the full workload), median of %d "
-                  "runs, %.2f s each" % (sample_cols, runs, dt),
+        "label": "SimdBackend-equivalent C++ (not the Rust binary)",
+        "simd_lanes": port.width,
+        "cpu_model": cpu_model(),
+        "sample": "the full workload (%d columns x 2^20 -> blowup 4 -> Blake2s Merkle commit), median of %d runs "
+                  "after one warm-up, %.3f s each (runs %s)" % (N_COLS, runs, dt,
+                                                                ", ".join("%.3f" % t for t in times)),
     }
+    if gpu_root is not None:
+        res["root_equal_to_gpu"] = bool(np.array_equal(layers[:8], gpu_root))
+    return res
 
 
 def main():
@@ -505,6 +534,7 @@ def main():
         log("step %d: %.2f ms" % (i, dt * 1e3))
     prof = ctx.prof_read()
     ctx.prof_enable(False)
+    gpu_root = layers.download(8)
     total = sum(times)
     ms_per_step = total / args.steps * 1e3
     elems = N_COLS * (1 << LOG_N)
@@ -573,8 +603,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline()
+        log("cpu baseline (SimdBackend-equivalent port) ...")
+        cpu = cpu_baseline(gpu_root)
         log("cpu: %s" % json.dumps(cpu))
 
     tw.destroy()
