@@ -4,7 +4,9 @@
 Workload (one step): 256 M31 columns of 2^20 bit-reversed evaluations on
 CanonicCoset(20), resident in HBM -> interpolate_columns (circle IFFT) ->
 evaluate_polynomials at blowup 4 onto CanonicCoset(22) (RFFT) -> Blake2s
-Merkle commit of the 256 evaluations.  Then, once, the FRI chain on one QM31
+Merkle commit of the 256 evaluations.  The two transforms run through
+stwo_hip_extend_evals (TreeBuilder::extend_evals with the PCS default of not
+storing coefficients).  Then, once, the FRI chain on one QM31
 evaluation of 2^22 points (fold_circle_into_line, fold_line down to 2^4, a
 Merkle commit per layer, Blake2s channel), reported beside.
 
@@ -42,8 +44,11 @@ def run_step(ctx, work, pristine, outs, layers, tw, logs):
         ctx._check(sh.lib().stwo_hip_copy(ctx.h, w.ptr, p.ptr, w.n))
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.interpolate_columns(work, logs, tw)
-    ctx.evaluate_polynomials(work, logs, LOG_BLOWUP, tw, outs)
+    # interpolate_columns + evaluate_polynomials as TreeBuilder::extend_evals
+    # runs them with the PCS default (store_polynomials_coefficients = false):
+    # the IFFT's last pass and the RFFT's first pass fused, the coefficients
+    # not stored.
+    ctx.extend_evals(work, logs, LOG_BLOWUP, tw, outs, store_coeffs=False)
     ctx.merkle_commit(outs, [LOG_N + LOG_BLOWUP] * N_COLS, layers)
     ctx.sync()
     return time.perf_counter() - t0
@@ -232,6 +237,7 @@ ROCPROF_NAMES = {
     "rfft_pass_wg8": "void stwo_hip::fft_pass_kernel<false, 8, 3, 8, 4, 0>(stwo_hip::FftPass2)",
     "rfft_pass_wg9": "void stwo_hip::fft_pass_kernel<false, 9, 2, 9, 5, 0>(stwo_hip::FftPass2)",
     "ifft_pass_wg8": "void stwo_hip::fft_pass_kernel<true, 8, 3, 0, 4, 8>(stwo_hip::FftPass2)",
+    "lde_pass_wg9": "void stwo_hip::lde_pass_kernel<9, 2>(stwo_hip::FftPass2, stwo_hip::FftPass2, unsigned int)",
     "ifft_pass_wg9": "void stwo_hip::fft_pass_kernel<true, 9, 2, 5, 9, 0>(stwo_hip::FftPass2)",
     "merkle_commit_on_layer": "void stwo_hip::merkle_layer_kernel<false>(...)",
     "merkle_nodes": "stwo_hip::merkle_nodes_kernel(...)",

@@ -143,6 +143,20 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx *ctx, const uint32_t *const *coef
                                   size_t n_cols, uint32_t log_blowup, const stwo_hip_twiddles *tw,
                                   uint32_t *const *outs);
 
+/* TreeBuilder::extend_evals (STWO/prover/pcs/mod.rs:707-716: interpolate_columns)
+ * followed by CommitmentTreeProver::new's evaluate_polynomials at log_blowup
+ * (:766-784), as one call: cols[i] holds 2^log_sizes[i] bit-reversed
+ * evaluations on CanonicCoset(log_sizes[i]) and is consumed; outs[i] receives
+ * the 2^(log_sizes[i] + log_blowup) evaluations on CanonicCoset(log_sizes[i] +
+ * log_blowup), bit-reversed.  store_coeffs != 0 (store_polynomials_coefficients,
+ * default false at :347) leaves the coefficients in cols[i] as interpolate_columns
+ * would; with 0 their contents are unspecified.  The IFFT's last pass and the
+ * RFFT's first pass run as one kernel when their tiles match, so the
+ * coefficients are not written to HBM and read back.  Same values as
+ * stwo_hip_interpolate_columns + stwo_hip_evaluate_polynomials. */
+int stwo_hip_extend_evals(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                          uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, int store_coeffs);
+
 /* ---- MerkleOps<Blake2sMerkleHasher> (mixed degree) -----------------------
  * MerkleOps::commit_on_layer (STWO/prover/vcs/ops.rs:24-28; SimdBackend
  * SIMD/blake2s.rs:60-144; no Metal implementation exists).  Node i =
@@ -410,15 +424,21 @@ int stwo_hip_canonicalize(stwo_hip_ctx *ctx, uint32_t *col, size_t n);
  *               STWO_HIP_TREE_LIFTED = MerkleProverLifted::commit at the max size.
  * Columns must have log size >= 4.  Per-call tables are uploaded once; the
  * plan's launches (one per FFT pass level for all sizes, one per tree layer)
- * are captured into a hipGraph when use_graph != 0.  stwo_hip_batch_run
+ * are captured into a hipGraph when flags has STWO_HIP_BATCH_GRAPH.  Each
+ * column's IFFT's last pass and RFFT's first pass run as one fused pass
+ * where their tiles match (above 2^12 values); with STWO_HIP_BATCH_DROP_COEFFS
+ * (store_polynomials_coefficients = false) that pass does not store the
+ * coefficients and cols[i] is left unspecified.  stwo_hip_batch_run
  * enqueues the whole sequence on the context stream; the buffers named at
  * creation must stay valid for the plan's lifetime. */
+#define STWO_HIP_BATCH_GRAPH 1
+#define STWO_HIP_BATCH_DROP_COEFFS 2
 #define STWO_HIP_TREE_MIXED 0
 #define STWO_HIP_TREE_LIFTED 1
 typedef struct stwo_hip_batch stwo_hip_batch;
 int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
                           uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
-                          int tree_kind, int hash_mode, int use_graph, stwo_hip_batch **out);
+                          int tree_kind, int hash_mode, int flags, stwo_hip_batch **out);
 int stwo_hip_batch_run(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
 int stwo_hip_batch_destroy(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
 
@@ -471,6 +491,32 @@ int stwo_hip_server_destroy(stwo_hip_server *server);
  * runs for a transform (16 words per pass, see fft.hip).  Launches nothing. */
 int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, uint32_t max_passes,
                             uint32_t *n_passes);
+/* The same for the planner choices (t0, smax, wg) of stwo_hip_fft_tune_set;
+ * 0 = the default of each. */
+int stwo_hip_debug_fft_plan_tuned(uint32_t n, uint32_t N, int inverse, uint32_t t0, uint32_t smax, uint32_t wg,
+                                  uint32_t *out, uint32_t max_passes, uint32_t *n_passes);
+
+/* ---- FFT autotuning --------------------------------------------------------------
+ * No counterpart in the reference (whose Metal backend fixes its kernel
+ * shapes): picks, per FFT shape of a column set, the fastest of the planner's
+ * candidate plans by timing them on this device, and keeps the choice in the
+ * context, where every later plan of that shape uses it (interpolate,
+ * evaluate, extend_evals, batch_create).  For each distinct log size n of
+ * log_sizes (n >= 4): interpolate of 2^n and evaluate of 2^n at log_blowup.
+ * Candidates: low-pass tile 2^10..2^13 (default 2^12), strided passes of at
+ * most 5..8 layers (default 8), the tile's own workgroup size or 512 lanes.
+ * Each runs on as many synthetic columns as the set has of that size (at most
+ * 2^26 words, in a context scratch buffer); the fastest of 3 timed runs after
+ * a warm-up counts.  report (optional): 8 words per tuned shape, up to
+ * max_report: n, N, inverse, t0, smax, wg, best microseconds, the default
+ * plan's microseconds; *n_report = shapes tuned.  Synchronizes. */
+int stwo_hip_fft_autotune(stwo_hip_ctx *ctx, const uint32_t *log_sizes, size_t n_cols, uint32_t log_blowup,
+                          const stwo_hip_twiddles *tw, uint32_t *report, size_t max_report, size_t *n_report);
+/* Sets planner choices (6 words per entry: n, N, inverse, t0 10..13, smax
+ * 5..8, wg 0 or 9), e.g. a table saved from an earlier stwo_hip_fft_autotune;
+ * stwo_hip_fft_tune_clear returns every shape to the defaults. */
+int stwo_hip_fft_tune_set(stwo_hip_ctx *ctx, const uint32_t *entries, size_t n_entries);
+int stwo_hip_fft_tune_clear(stwo_hip_ctx *ctx);
 
 /* ---- profiling ------------------------------------------------------------
  * When enabled, every kernel launch of the context is bracketed by HIP

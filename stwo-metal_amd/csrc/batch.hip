@@ -51,8 +51,9 @@ struct Staging {
 
 struct LevelLaunch {
   bool inverse;
-  int kind;  // launch_fft_multi's schedule kind for every group of the launch
-  size_t groups_off, wgs_off;  // offsets into the device buffer
+  bool fused;  // the fused IFFT-last + RFFT-first pass (groups_off: IFFT descriptors, fwd_off: RFFT ones)
+  int kind;  // launch_fft_multi's (or launch_lde_multi's) schedule kind for every group of the launch
+  size_t groups_off, fwd_off, wgs_off;  // offsets into the device buffer
   uint32_t n_groups, total_wgs, wg_log;
 };
 
@@ -66,7 +67,8 @@ extern "C" {
 
 int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
                           uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
-                          int tree_kind, int hash_mode, int use_graph, stwo_hip_batch **out) {
+                          int tree_kind, int hash_mode, int flags, stwo_hip_batch **out) {
+  const int use_graph = flags & STWO_HIP_BATCH_GRAPH;
   if (!ctx || !out) return STWO_HIP_EINVAL;
   *out = nullptr;
   STWO_HIP_CHECK_ARG(ctx, n_cols > 0 && cols && log_sizes && outs && layers, "batch_create: NULL or empty arguments");
@@ -149,69 +151,122 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   // Upper bound on descriptors: one per group and pass (2 directions, at most
   // 4 passes), plus the 256-byte alignment of two tables per launch (2
   // directions x 4 levels x 2 workgroup sizes x at most 14 schedule kinds).
-  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 14 * 2 * 256 + 4096;
+  size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 14 * 2 * 256 + 8192;
   if (hipMalloc(&dmem, tables_bytes + desc_bytes) != hipSuccess) {
     (void)hipGetLastError();
     return set_error(ctx, STWO_HIP_ENOMEM, "batch_create: out of device memory for tables");
   }
   uint8_t *dbase = (uint8_t *)dmem;
-  std::vector<LevelLaunch> levels;
-  for (int dir = 0; dir < 2; dir++) {
-    bool inverse = dir == 0;
-    // (level, workgroup size) -> passes of every group
-    std::map<std::pair<uint32_t, uint32_t>, std::vector<PassLaunch>> by_level;
-    for (const GroupTabs &t : gt) {
-      std::vector<PassLaunch> passes;
-      uint32_t n = t.n, N = t.n + (inverse ? 0 : log_blowup);
+  // Both transforms' passes of every group; where the IFFT's last pass and
+  // the RFFT's first pass tile alike they become one fused pass.
+  struct GroupPasses {
+    std::vector<PassLaunch> inv, fwd;
+    bool fused = false;
+  };
+  std::vector<GroupPasses> gp(gt.size());
+  for (size_t gi = 0; gi < gt.size(); gi++) {
+    const GroupTabs &t = gt[gi];
+    for (int dir = 0; dir < 2; dir++) {
+      const bool inverse = dir == 0;
+      const uint32_t n = t.n, N = t.n + (inverse ? 0 : log_blowup);
       const FftCol *first = (const FftCol *)(dbase + (inverse ? t.ifft_off : t.rfft_off));
       const FftCol *inpl = (const FftCol *)(dbase + t.inplace_off);
       int rc = make_group_passes(ctx, inverse ? N : n, N, inverse, first, inpl, t.count, inverse ? itw_end : tw_end,
-                                 t.aligned16, passes);
+                                 t.aligned16, inverse ? gp[gi].inv : gp[gi].fwd);
       if (rc) {
         (void)hipFree(dmem);
         return rc;
       }
-      for (size_t k = 0; k < passes.size(); k++) {
+    }
+    gp[gi].fused = lde_fusable(gp[gi].inv.back(), gp[gi].fwd.front(), log_blowup);
+  }
+  std::vector<LevelLaunch> levels;
+  auto add_level = [&](bool inverse, uint32_t wg_log, std::vector<PassLaunch> &v) -> int {
+    // Groups whose pass matches a compile-time schedule get a launch per
+    // schedule with that schedule's kernel; the rest share one.
+    std::vector<uint32_t> kinds;
+    for (auto &pl : v)
+      if (std::find(kinds.begin(), kinds.end(), pl.p.fixed) == kinds.end()) kinds.push_back(pl.p.fixed);
+    for (uint32_t kd : kinds) {
+      std::vector<FftPass2> descs;
+      std::vector<uint32_t> starts;
+      uint64_t acc = 0;
+      bool uniform = true;
+      for (auto &pl : v) {
+        if (pl.p.fixed != kd) continue;
+        descs.push_back(pl.p);
+        starts.push_back((uint32_t)acc);
+        acc += pl.wgs;
+        if (pl.p.T < 10) uniform = false;
+      }
+      if (descs.empty()) continue;
+      if (acc >= (1ull << 31)) return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
+      LevelLaunch ll{};
+      ll.inverse = inverse;
+      ll.fused = false;
+      ll.kind = kd != 0 ? (int)kd : (uniform ? -1 : 0);
+      ll.groups_off = st.put(descs.data(), descs.size() * sizeof(FftPass2));
+      ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
+      ll.n_groups = (uint32_t)descs.size();
+      ll.total_wgs = (uint32_t)acc;
+      ll.wg_log = wg_log;
+      levels.push_back(ll);
+    }
+    return STWO_HIP_OK;
+  };
+  for (int dir = 0; dir < 2; dir++) {
+    const bool inverse = dir == 0;
+    // (level, workgroup size) -> passes of every group (the fused ones aside)
+    std::map<std::pair<uint32_t, uint32_t>, std::vector<PassLaunch>> by_level;
+    for (const GroupPasses &g : gp) {
+      const std::vector<PassLaunch> &passes = inverse ? g.inv : g.fwd;
+      const size_t k0 = (!inverse && g.fused) ? 1 : 0, k1 = passes.size() - ((inverse && g.fused) ? 1 : 0);
+      for (size_t k = k0; k < k1; k++) {
         uint32_t level = inverse ? (uint32_t)k : (uint32_t)(passes.size() - 1 - k);
         by_level[{level, passes[k].wg_log}].push_back(passes[k]);
+      }
+    }
+    if (!inverse) {
+      // the fused passes run after every IFFT level and before every RFFT one
+      std::map<std::pair<uint32_t, int>, std::vector<size_t>> fused;  // (wg_log, kind) -> groups
+      for (size_t gi = 0; gi < gp.size(); gi++)
+        if (gp[gi].fused)
+          fused[{gp[gi].inv.back().wg_log, lde_kind(gp[gi].inv.back(), gp[gi].fwd.front())}].push_back(gi);
+      for (auto &kv : fused) {
+        std::vector<FftPass2> pis, pfs;
+        std::vector<uint32_t> starts;
+        uint64_t acc = 0;
+        for (size_t gi : kv.second) {
+          pis.push_back(gp[gi].inv.back().p);
+          pfs.push_back(gp[gi].fwd.front().p);
+          starts.push_back((uint32_t)acc);
+          acc += gp[gi].inv.back().wgs;
+        }
+        if (acc >= (1ull << 31)) {
+          (void)hipFree(dmem);
+          return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
+        }
+        LevelLaunch ll{};
+        ll.inverse = false;
+        ll.fused = true;
+        ll.kind = kv.first.second;
+        ll.groups_off = st.put(pis.data(), pis.size() * sizeof(FftPass2));
+        ll.fwd_off = st.put(pfs.data(), pfs.size() * sizeof(FftPass2));
+        ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
+        ll.n_groups = (uint32_t)pis.size();
+        ll.total_wgs = (uint32_t)acc;
+        ll.wg_log = kv.first.first;
+        levels.push_back(ll);
       }
     }
     std::vector<std::pair<uint32_t, uint32_t>> order;
     for (auto &kv : by_level) order.push_back(kv.first);
     if (!inverse) std::reverse(order.begin(), order.end());  // forward: highest level first
     for (const auto &key : order) {
-      auto &v = by_level[key];
-      // Groups whose pass matches a compile-time schedule get a launch per
-      // schedule with that schedule's kernel; the rest share one.
-      std::vector<uint32_t> kinds;
-      for (auto &pl : v)
-        if (std::find(kinds.begin(), kinds.end(), pl.p.fixed) == kinds.end()) kinds.push_back(pl.p.fixed);
-      for (uint32_t kd : kinds) {
-        std::vector<FftPass2> descs;
-        std::vector<uint32_t> starts;
-        uint64_t acc = 0;
-        bool uniform = true;
-        for (auto &pl : v) {
-          if (pl.p.fixed != kd) continue;
-          descs.push_back(pl.p);
-          starts.push_back((uint32_t)acc);
-          acc += pl.wgs;
-          if (pl.p.T < 10) uniform = false;
-        }
-        if (descs.empty()) continue;
-        if (acc >= (1ull << 31)) {
-          (void)hipFree(dmem);
-          return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
-        }
-        LevelLaunch ll;
-        ll.inverse = inverse;
-        ll.kind = kd != 0 ? (int)kd : (uniform ? -1 : 0);
-        ll.groups_off = st.put(descs.data(), descs.size() * sizeof(FftPass2));
-        ll.wgs_off = st.put(starts.data(), starts.size() * sizeof(uint32_t));
-        ll.n_groups = (uint32_t)descs.size();
-        ll.total_wgs = (uint32_t)acc;
-        ll.wg_log = key.second;
-        levels.push_back(ll);
+      int rc = add_level(inverse, key.second, by_level[key]);
+      if (rc) {
+        (void)hipFree(dmem);
+        return rc;
       }
     }
   }
@@ -233,11 +288,16 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   b->d_mem = dmem;
   const int mode = hash_mode;
   const int kind = tree_kind;
-  b->issue = [levels, lt, dbase, layers, max_log, mode, kind](stwo_hip_ctx *c) -> int {
+  const uint32_t keep_coeffs = (flags & STWO_HIP_BATCH_DROP_COEFFS) ? 0u : 1u;
+  b->issue = [levels, lt, dbase, layers, max_log, mode, kind, keep_coeffs](stwo_hip_ctx *c) -> int {
     for (const LevelLaunch &ll : levels) {
-      int rc = launch_fft_multi(c, ll.inverse, (const FftPass2 *)(dbase + ll.groups_off),
-                                (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs, ll.wg_log,
-                                ll.kind);
+      int rc = ll.fused ? launch_lde_multi(c, (const FftPass2 *)(dbase + ll.groups_off),
+                                           (const FftPass2 *)(dbase + ll.fwd_off),
+                                           (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs,
+                                           ll.wg_log, ll.kind, keep_coeffs)
+                        : launch_fft_multi(c, ll.inverse, (const FftPass2 *)(dbase + ll.groups_off),
+                                           (const uint32_t *)(dbase + ll.wgs_off), ll.n_groups, ll.total_wgs,
+                                           ll.wg_log, ll.kind);
       if (rc) return rc;
     }
     if (kind == STWO_HIP_TREE_LIFTED) {

@@ -26,6 +26,19 @@ struct stwo_hip_twiddles {
 
 namespace stwo_hip {
 
+// Planner choices for one FFT shape (interpolate or evaluate of 2^n values on
+// a domain of 2^N), kept per context; stwo_hip_fft_autotune sets them from
+// device timings and every plan made afterwards on that context uses them.
+struct FftTune {
+  uint8_t t0 = 12;   // tile log of the contiguous low pass (10..13)
+  uint8_t smax = 8;  // most layers per strided pass (5..8)
+  uint8_t wg = 0;    // workgroup log: 0 = max(8, T - 4); 9 = 512 lanes for every tile size
+  uint8_t pad = 0;
+};
+inline uint64_t fft_tune_key(uint32_t n, uint32_t N, bool inverse) {
+  return ((uint64_t)n << 16) | ((uint64_t)N << 1) | (inverse ? 1u : 0u);
+}
+
 struct ProfPending {
   const char *name;
   hipEvent_t start, stop;
@@ -71,6 +84,9 @@ struct stwo_hip_ctx {
 
   // Scratch device buffers kept across calls, keyed by purpose.
   std::map<std::string, std::pair<void *, size_t>> scratch;
+
+  // FFT planner choices by shape (fft_tune_key); absent shapes use the defaults.
+  std::map<uint64_t, stwo_hip::FftTune> fft_tune;
 };
 
 namespace stwo_hip {

@@ -299,6 +299,22 @@ __device__ __forceinline__ uint32_t gld(const uint32_t *base, uint32_t byte_off)
 __device__ __forceinline__ void gst(uint32_t *base, uint32_t byte_off, uint32_t v) {
   *(g_u32 *)((g_u8 *)as_global(base) + byte_off) = v;
 }
+// The same with a constant part `imm`: the 32-bit lane offset is widened
+// before the constant is added, so the constant (below 4096) folds into the
+// instruction's immediate offset instead of taking a VALU add per load (a
+// 32-bit sum could wrap, which keeps the compiler from folding it).
+__device__ __forceinline__ uint32_t gld_imm(const uint32_t *base, uint32_t lane_off, uint32_t imm) {
+  return *(gconst_u32 *)((gconst_u8 *)as_global(base) + (uint64_t)lane_off + imm);
+}
+__device__ __forceinline__ void gst_imm(uint32_t *base, uint32_t lane_off, uint32_t imm, uint32_t v) {
+  *(g_u32 *)((g_u8 *)as_global(base) + (uint64_t)lane_off + imm) = v;
+}
+__device__ __forceinline__ u32x4 gld4_imm(const uint32_t *base, uint32_t lane_off, uint32_t imm) {
+  return *(gconst_u4 *)((gconst_u8 *)as_global(base) + (uint64_t)lane_off + imm);
+}
+__device__ __forceinline__ void gst4_imm(uint32_t *base, uint32_t lane_off, uint32_t imm, u32x4 v) {
+  *(g_u4 *)((g_u8 *)as_global(base) + (uint64_t)lane_off + imm) = v;
+}
 __device__ __forceinline__ u32x4 gld4(const uint32_t *base, uint32_t byte_off) {
   return *(gconst_u4 *)((gconst_u8 *)as_global(base) + byte_off);
 }
@@ -555,7 +571,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         const uint32_t vo = (gidx(local << 4) & in_mask) << 2;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const u32x4 v = gld4(in, vo + 16 * k);
+          const u32x4 v = gld4_imm(in, vo, 16 * k);
           x[4 * k] = v.x;
           x[4 * k + 1] = v.y;
           x[4 * k + 2] = v.z;
@@ -580,7 +596,8 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         const uint32_t vo = (gb & in_mask) << 2, jmask = in_mask >> sh;
         if (NS == 3) {
 #pragma unroll
-          for (int j = 0; j < 16; j++) x[j] = gld(in, vo + ((uint32_t)j << (T_FIXED12 - 4 + 2)));
+          for (int j = 0; j < 16; j++)
+            x[j] = gld_imm(in + ((j >> 2) << (T_FIXED12 - 2)), vo, (uint32_t)(j & 3) << (T_FIXED12 - 4 + 2));
         } else {
 #pragma unroll
           for (int j = 0; j < 16; j++) x[j] = gld(in, vo + ((((uint32_t)j) & jmask) << (sh + 2)));
@@ -627,7 +644,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           const u32x4 v = {x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]};
-          gst4(out, vo + 16 * k, v);
+          gst4_imm(out, vo, 16 * k, v);
         }
       }
       continue;
@@ -647,7 +664,8 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         // fixed 12-layer passes end in the top layout (B0C is 0 or 8; 0 was
         // moved there above) with w = lo = 0: sh = 8
 #pragma unroll
-        for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (T_FIXED12 - 4 + 2)), x[j]);
+        for (int j = 0; j < 16; j++)
+          gst_imm(out + ((j >> 2) << (T_FIXED12 - 2)), vo, (uint32_t)(j & 3) << (T_FIXED12 - 4 + 2), x[j]);
       } else {
 #pragma unroll
         for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (sh + 2)), x[j]);
@@ -683,6 +701,184 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
   fft_pass_body<INV, WG_LOG, NS, B0A, B0B, B0C>(p, blockIdx.x - wg_start[g], lds);
 }
 
+// ---- fused LDE pass ----------------------------------------------------------
+//
+// The IFFT's last pass and the RFFT's first (blowup) pass of the same column
+// cover the same layers on the same tile geometry (interpolate ends with the
+// top layers, evaluate starts with them), so one kernel runs both on a tile
+// held in registers: load the tile of evaluations, run the inverse stages,
+// scale by 1/2^n (the coefficients; stored only when the caller keeps them),
+// then for each of the 2^b output blocks run the forward stages with that
+// block's twiddles and store.  The coefficients never go to HBM and back:
+// TreeBuilder::extend_evals (STWO/prover/pcs/mod.rs:718-735) drops them
+// unless store_polynomials_coefficients is set (:347).
+//
+// NS = 2: the 8-layer strided pair on a 2^13 tile (inverse stages at tile
+// bits 5 then 9, forward 9 then 5: the inverse ends in the layout the
+// forward starts in, so the blocks need one LDS exchange each); NS = 0: the
+// stage lists of both descriptors at run time.  Both passes tile the
+// coefficient domain (pf.rep_log = b), and pi.T >= 10 (wave-uniform columns).
+template <bool INV>
+__device__ __forceinline__ void runtime_stages(const FftPass2 &p, uint32_t (&x)[16], uint32_t *L, uint32_t &cur,
+                                               uint32_t local, uint32_t tile_gb) {
+  const uint32_t wmask = (1u << p.w) - 1;
+  auto gidx = [&](uint32_t b) { return tile_gb | ((b >> p.w) << p.lo) | (b & wmask); };
+  auto group_base = [&](uint32_t b0) { return ((local >> b0) << (b0 + 4)) | (local & ((1u << b0) - 1)); };
+  for (uint32_t si = 0; si < p.n_stages; si++) {
+    const FftStage S = p.st[si];
+    const uint32_t base = group_base(S.b0);
+    const uint32_t sh = S.b0 - p.w + p.lo;
+    StageTw t;
+    load_stage_twiddles(t, p.tw_end, p.N, sh, gidx(base), S.r_lo, S.r_hi);
+    if (S.b0 != cur) {
+      __syncthreads();
+      lds_put(L, cur, group_base(cur), x);
+      __syncthreads();
+      lds_get(L, S.b0, base, x);
+      cur = S.b0;
+    }
+    stage_dispatch<INV>(x, t, S.r_lo, S.r_hi);
+  }
+}
+
+template <int WG_LOG, int NS>
+__device__ __forceinline__ void lde_pass_body(const FftPass2 &pi, const FftPass2 &pf, uint32_t store_coeffs,
+                                              uint32_t wg, uint32_t *lds) {
+  const uint32_t tpt_log = pi.T - 4;
+  const uint32_t local = threadIdx.x & ((1u << tpt_log) - 1);
+  const uint32_t slot = threadIdx.x >> tpt_log;
+  const uint64_t gtile = ((uint64_t)wg << (WG_LOG - tpt_log)) | slot;
+  const uint32_t dom_log = pi.N;  // the coefficient domain; pf.N - pf.rep_log is the same
+  const uint32_t tpc_log = dom_log - pi.T;
+  const bool active = gtile < ((uint64_t)pi.n_cols << tpc_log);
+  uint32_t tile = 0, col_idx = 0;
+  if (active) {
+    tile = (uint32_t)(gtile & ((1ull << tpc_log) - 1));
+    col_idx = (uint32_t)(gtile >> tpc_log);
+  }
+  // Tiles of 2^10 or more span whole waves: the column is wave-uniform.
+  const uint32_t *ev = (const uint32_t *)readfirstlane_u64(active ? (uint64_t)pi.cols[col_idx].in : 0);
+  uint32_t *co = (uint32_t *)readfirstlane_u64(active ? (uint64_t)pi.cols[col_idx].out : 0);
+  uint32_t *out = (uint32_t *)readfirstlane_u64(active ? (uint64_t)pf.cols[col_idx].out : 0);
+  uint32_t *L = lds + slot * ((1u << pi.T) + (1u << (pi.T - 4)));
+  const uint32_t ih_bits = pi.lo - pi.w;
+  const uint32_t ih = tile & ((1u << ih_bits) - 1);
+  const uint32_t o = tile >> ih_bits;
+  const uint32_t wmask = (1u << pi.w) - 1;
+  const uint32_t top_b0 = pi.T - 4;
+  const uint32_t tile_gb0 = (o << (pi.lo + pi.m)) | (ih << pi.w);
+  auto gidx0 = [&](uint32_t b) { return tile_gb0 | ((b >> pi.w) << pi.lo) | (b & wmask); };
+  auto group_base = [&](uint32_t b0) { return ((local >> b0) << (b0 + 4)) | (local & ((1u << b0) - 1)); };
+  uint32_t x[16];
+  const uint32_t sh_top = top_b0 - pi.w + pi.lo;
+  if (active) {
+    const uint32_t vo = gidx0(local) << 2;
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = gld(ev, vo + ((uint32_t)j << (sh_top + 2)));
+  }
+  uint32_t cur = top_b0;
+  if constexpr (NS == 2) {
+    fixed_stage<true, 5, -1>(pi, x, L, cur, local, group_base, gidx0);
+    fixed_stage<true, 9, 5>(pi, x, L, cur, local, group_base, gidx0);
+  } else {
+    runtime_stages<true>(pi, x, L, cur, local, tile_gb0);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; j++) x[j] = m31_mul(x[j], pi.scale);
+  if (store_coeffs && active) {
+    // store in a lane-consecutive layout (group bit >= 5 or the top layout)
+    uint32_t y[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) y[j] = x[j];
+    uint32_t c2 = cur;
+    if (c2 != top_b0 && c2 < 5) {
+      __syncthreads();
+      lds_put(L, c2, group_base(c2), y);
+      __syncthreads();
+      lds_get(L, top_b0, local, y);
+      c2 = top_b0;
+    }
+    const uint32_t vo = gidx0(group_base(c2)) << 2, sh = c2 - pi.w + pi.lo;
+#pragma unroll
+    for (int j = 0; j < 16; j++) gst(co, vo + ((uint32_t)j << (sh + 2)), y[j]);
+  }
+  const uint32_t last_inv = cur;
+  const uint32_t n_rep = 1u << pf.rep_log;
+  for (uint32_t rep = 0; rep < n_rep; rep++) {
+    const uint32_t tile_gb = (rep << dom_log) | tile_gb0;
+    auto gidx = [&](uint32_t b) { return tile_gb | ((b >> pi.w) << pi.lo) | (b & wmask); };
+    uint32_t y[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) y[j] = x[j];
+    cur = last_inv;
+    if constexpr (NS == 2) {
+      fixed_stage<false, 9, 9>(pf, y, L, cur, local, group_base, gidx);
+      fixed_stage<false, 5, 9>(pf, y, L, cur, local, group_base, gidx);
+    } else {
+      runtime_stages<false>(pf, y, L, cur, local, tile_gb);
+    }
+    if (cur != top_b0 && cur < 5) {
+      __syncthreads();
+      lds_put(L, cur, group_base(cur), y);
+      __syncthreads();
+      lds_get(L, top_b0, local, y);
+      cur = top_b0;
+    }
+    if (active) {
+      const uint32_t vo = gidx(group_base(cur)) << 2, sh = cur - pi.w + pi.lo;
+#pragma unroll
+      for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (sh + 2)), y[j]);
+    }
+  }
+}
+
+template <int WG_LOG, int NS>
+__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(4)))
+lde_pass_kernel(FftPass2 pi, FftPass2 pf, uint32_t store_coeffs) {
+  __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
+  lde_pass_body<WG_LOG, NS>(pi, pf, store_coeffs, blockIdx.x, lds);
+}
+
+// The fused pass of several column groups in one launch (the batched path):
+// group g owns workgroups [wg_start[g], wg_start[g + 1]) and the pair
+// (pis[g], pfs[g]); every group matches the kernel's schedule (NS).
+template <int WG_LOG, int NS>
+__global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(4)))
+lde_pass_multi_kernel(const FftPass2 *pis, const FftPass2 *pfs, const uint32_t *wg_start, uint32_t n_groups,
+                      uint32_t store_coeffs) {
+  __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
+  uint32_t g = 0, hi = n_groups;
+  while (hi - g > 1) {
+    const uint32_t mid = (g + hi) >> 1;
+    if (wg_start[mid] <= blockIdx.x) g = mid; else hi = mid;
+  }
+  const FftPass2 pi = pis[g], pf = pfs[g];
+  lde_pass_body<WG_LOG, NS>(pi, pf, store_coeffs, blockIdx.x - wg_start[g], lds);
+}
+
+int launch_lde_multi(stwo_hip_ctx *ctx, const FftPass2 *d_pis, const FftPass2 *d_pfs, const uint32_t *d_wg_start,
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind, uint32_t store_coeffs) {
+  if (wg_log == 9 && kind == 2)
+    return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<9, 2>, dim3(total_wgs), dim3(512), 0, d_pis, d_pfs,
+                  d_wg_start, n_groups, store_coeffs);
+  if (wg_log == 9)
+    return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<9, 0>, dim3(total_wgs), dim3(512), 0, d_pis, d_pfs,
+                  d_wg_start, n_groups, store_coeffs);
+  return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<8, 0>, dim3(total_wgs), dim3(256), 0, d_pis, d_pfs,
+                d_wg_start, n_groups, store_coeffs);
+}
+
+bool lde_fusable(const PassLaunch &last_inv, const PassLaunch &first_fwd, uint32_t log_blowup) {
+  const FftPass2 &li = last_inv.p, &ff = first_fwd.p;
+  return li.T == ff.T && li.lo == ff.lo && li.w == ff.w && li.m == ff.m && li.lo > 0 && li.T >= 10 &&
+         ff.rep_log == log_blowup && last_inv.wg_log == first_fwd.wg_log && last_inv.wgs == first_fwd.wgs &&
+         (last_inv.wg_log == 8 || last_inv.wg_log == 9);
+}
+
+int lde_kind(const PassLaunch &last_inv, const PassLaunch &first_fwd) {
+  return (last_inv.wg_log == 9 && last_inv.p.fixed == 2 && first_fwd.p.fixed == 2) ? 2 : 0;
+}
+
 // Stages covering layer tile bits [A, B) of a tile of T bits, forward order.
 static std::vector<FftStage> plan_stages(uint32_t A, uint32_t B, uint32_t T) {
   std::vector<FftStage> st;
@@ -706,21 +902,23 @@ constexpr uint32_t kStridedMaxLayers2 = 8;
 constexpr uint32_t kMaxTileLog = 13;  // 8192-element tiles: 34 KB of LDS, 512 lanes
 
 // Passes of the radix-16 kernel covering layers [0, L) of a domain of log
-// size N (N >= 4), forward order: strided passes of <= 8 layers, then the
-// contiguous low pass of min(N, 12) tile bits.
-static std::vector<PassPlan> plan_passes2(uint32_t N, uint32_t L) {
+// size N (N >= 4), forward order: strided passes of <= tune.smax (default 8)
+// layers, then the contiguous low pass of min(N, tune.t0) (default 12) tile
+// bits.
+static std::vector<PassPlan> plan_passes2(uint32_t N, uint32_t L, const FftTune &tune) {
   std::vector<PassPlan> fwd;
-  uint32_t t0 = std::min(N, kTileLog);
+  const uint32_t smax = std::min<uint32_t>(kStridedMaxLayers2, std::max<uint32_t>(5, tune.smax));
+  uint32_t t0 = std::min<uint32_t>(N, std::min<uint32_t>(kMaxTileLog, std::max<uint32_t>(10, tune.t0)));
   if (L > t0) {
     uint32_t rest = L - t0;
-    uint32_t npass = (rest + kStridedMaxLayers2 - 1) / kStridedMaxLayers2;
+    uint32_t npass = (rest + smax - 1) / smax;
     uint32_t hi = L;
     // Full 8-layer passes from the top (the compile-time two-stage schedule
     // of a 2^13 tile, which also serves a blowup's first pass), the
     // remainder in the last strided pass.
     for (uint32_t k = 0; k < npass; k++) {
       uint32_t left = hi - t0;
-      uint32_t m = k + 1 < npass ? kStridedMaxLayers2 : left;
+      uint32_t m = k + 1 < npass ? smax : left;
       uint32_t lo = hi - m;
       // The tile's row bits must hold a whole 4-bit register group: a pass of
       // fewer than 4 layers takes 4 rows, the extra ones (batch bits) being
@@ -735,7 +933,7 @@ static std::vector<PassPlan> plan_passes2(uint32_t N, uint32_t L) {
   }
   // With a blowup (L < N) and no strided pass, the low pass tiles the
   // coefficient domain so that its output blocks can repeat the tile.
-  uint32_t tl = (L < N && L <= kTileLog && L >= 4) ? L : t0;
+  uint32_t tl = (L < N && L <= t0 && L >= 4) ? L : t0;
   fwd.push_back({0, tl, 0, 0, std::min(L, tl)});
   return fwd;
 }
@@ -830,11 +1028,18 @@ static uint32_t fixed_schedule(const FftPass2 &p, bool inverse, uint32_t wg_log)
 // `d_tab_inplace` (entries {out, out}).
 int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, const FftCol *d_tab,
                       const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
-                      std::vector<PassLaunch> &out) {
+                      std::vector<PassLaunch> &out, const FftTune *tune) {
   out.clear();
   STWO_HIP_CHECK_ARG(ctx, N >= 4 && N <= 28 && n <= N, "internal: radix-16 path needs 4 <= N <= 28 (N = %u)", N);
+  FftTune tn;
+  if (tune) {
+    tn = *tune;
+  } else {
+    auto it = ctx->fft_tune.find(fft_tune_key(n, N, inverse));
+    if (it != ctx->fft_tune.end()) tn = it->second;
+  }
   uint32_t L = inverse ? N : n;
-  std::vector<PassPlan> plan = plan_passes2(N, L);
+  std::vector<PassPlan> plan = plan_passes2(N, L, tn);
   if (inverse) std::reverse(plan.begin(), plan.end());
   uint32_t n_inv = m31_inv((uint32_t)(((uint64_t)1 << N) % P));
   for (size_t k = 0; k < plan.size(); k++) {
@@ -864,6 +1069,7 @@ int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, c
     // domain: one workgroup writes all 2^(N-n) output blocks of its tile.
     p.rep_log = (!inverse && k == 0 && N > n && pp.lo + pp.m <= n) ? N - n : 0;
     uint32_t wg_log = std::max<uint32_t>(8, p.T - 4);
+    if (tn.wg == 9) wg_log = 9;
     uint64_t total_tiles = (uint64_t)n_cols << (N - p.rep_log - p.T);
     uint32_t tiles_per_wg = 1u << (wg_log + 4 - p.T);
     uint64_t wgs = (total_tiles + tiles_per_wg - 1) / tiles_per_wg;
@@ -970,6 +1176,58 @@ static int run_fft_group2(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, vo
   if (rc) return rc;
   for (const PassLaunch &pl : passes) {
     rc = launch_fft_pass(ctx, inverse, pl.p, pl.wgs, pl.wg_log);
+    if (rc) return rc;
+  }
+  return STWO_HIP_OK;
+}
+
+// interpolate + evaluate of a group of same-size columns (log n, n >= 4)
+// with the IFFT's last pass and the RFFT's first pass fused when their tiles
+// match; otherwise the two transforms one after the other.
+static int run_lde_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, const std::vector<uint32_t *> &outs,
+                         uint32_t n, uint32_t b, const stwo_hip_twiddles *tw, bool store_coeffs) {
+  const uint32_t N = n + b;
+  // tables: IFFT {c, c}; first RFFT pass {c, out}; later RFFT passes {out, out}
+  std::vector<FftCol> tf(cols.size()), to(cols.size());
+  bool aligned16 = true;
+  for (size_t i = 0; i < cols.size(); i++) {
+    tf[i] = {cols[i].in, outs[i]};
+    to[i] = {outs[i], outs[i]};
+    if ((((uintptr_t)cols[i].in) | ((uintptr_t)outs[i])) & 15) aligned16 = false;
+  }
+  void *d[3];
+  {
+    const void *srcs[3] = {cols.data(), tf.data(), to.data()};
+    const size_t bytes[3] = {cols.size() * sizeof(FftCol), tf.size() * sizeof(FftCol), to.size() * sizeof(FftCol)};
+    int rc = upload_tables(ctx, 3, srcs, bytes, d);
+    if (rc) return rc;
+  }
+  std::vector<PassLaunch> inv, fwd;
+  int rc = make_group_passes(ctx, n, n, true, (const FftCol *)d[0], (const FftCol *)d[0], (uint32_t)cols.size(),
+                             tw->d_itw2 + tw->len, aligned16, inv);
+  if (rc) return rc;
+  rc = make_group_passes(ctx, n, N, false, (const FftCol *)d[1], (const FftCol *)d[2], (uint32_t)cols.size(),
+                         tw->d_tw2 + tw->len, aligned16, fwd);
+  if (rc) return rc;
+  const FftPass2 &li = inv.back().p, &ff = fwd.front().p;
+  const bool fuse = lde_fusable(inv.back(), fwd.front(), b);
+  for (size_t k = 0; k + (fuse ? 1 : 0) < inv.size(); k++) {
+    rc = launch_fft_pass(ctx, true, inv[k].p, inv[k].wgs, inv[k].wg_log);
+    if (rc) return rc;
+  }
+  if (fuse) {
+    const uint32_t wgs = inv.back().wgs, wg_log = inv.back().wg_log;
+    const uint32_t sc = store_coeffs ? 1 : 0;
+    if (lde_kind(inv.back(), fwd.front()) == 2)
+      rc = launch(ctx, "lde_pass_wg9", lde_pass_kernel<9, 2>, dim3(wgs), dim3(512), 0, li, ff, sc);
+    else if (wg_log == 9)
+      rc = launch(ctx, "lde_pass_wg9", lde_pass_kernel<9, 0>, dim3(wgs), dim3(512), 0, li, ff, sc);
+    else
+      rc = launch(ctx, "lde_pass_wg8", lde_pass_kernel<8, 0>, dim3(wgs), dim3(256), 0, li, ff, sc);
+    if (rc) return rc;
+  }
+  for (size_t k = fuse ? 1 : 0; k < fwd.size(); k++) {
+    rc = launch_fft_pass(ctx, false, fwd[k].p, fwd[k].wgs, fwd[k].wg_log);
     if (rc) return rc;
   }
   return STWO_HIP_OK;
@@ -1163,17 +1421,228 @@ int stwo_hip_evaluate_polynomials(stwo_hip_ctx *ctx, const uint32_t *const *coef
   return STWO_HIP_OK;
 }
 
+int stwo_hip_extend_evals(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                          uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, int store_coeffs) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, n_cols == 0 || (cols && log_sizes && outs), "extend_evals: NULL arrays");
+  std::map<uint32_t, std::vector<size_t>> groups;
+  for (size_t i = 0; i < n_cols; i++) {
+    STWO_HIP_CHECK_ARG(ctx, cols[i] && outs[i], "extend_evals: NULL column %zu", i);
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 1 && log_sizes[i] <= 28 && log_blowup <= 28 &&
+                                log_sizes[i] + log_blowup <= 28,
+                       "extend_evals: bad log size %u + %u", log_sizes[i], log_blowup);
+    groups[log_sizes[i]].push_back(i);
+  }
+  for (auto &g : groups) {
+    const uint32_t n = g.first, N = n + log_blowup;
+    const stwo_hip_coset dom_i = {half_odds_initial(n - 1), n - 1}, dom_f = {half_odds_initial(N - 1), N - 1};
+    int rc = check_domain(ctx, dom_i, tw);
+    if (rc) return rc;
+    rc = check_domain(ctx, dom_f, tw);
+    if (rc) return rc;
+    for (size_t off = 0; off < g.second.size(); off += 65535) {
+      const size_t e = std::min(g.second.size(), off + (size_t)65535);
+      std::vector<FftCol> part;
+      std::vector<uint32_t *> po;
+      for (size_t k = off; k < e; k++) {
+        part.push_back({cols[g.second[k]], cols[g.second[k]]});
+        po.push_back(outs[g.second[k]]);
+      }
+      if (n >= 4) {
+        rc = run_lde_group(ctx, part, po, n, log_blowup, tw, store_coeffs != 0);
+      } else {
+        rc = run_fft_group(ctx, part, n, n, tw, true, dom_i);
+        if (rc) return rc;
+        std::vector<FftCol> pf;
+        for (size_t k = 0; k < part.size(); k++) pf.push_back({part[k].in, po[k]});
+        rc = run_fft_group(ctx, pf, n, N, tw, false, dom_f);
+      }
+      if (rc) return rc;
+    }
+  }
+  return STWO_HIP_OK;
+}
+
+// ---- autotuning ----------------------------------------------------------------
+//
+// For each FFT shape of a column set (interpolate of 2^n, evaluate of 2^n at
+// the blowup), every distinct plan among the candidate low-pass tiles
+// (2^10..2^13), strided-pass lengths (5..8 layers) and workgroup sizes (the
+// tile's own, or 512 lanes) is run on synthetic columns of that size on the
+// device -- as many columns as the set has of that size, up to 2^26 words --
+// and timed with HIP events (the fastest of 3 runs after a warm-up); the
+// fastest plan becomes the context's choice for the shape.
+namespace stwo_hip {
+namespace {
+
+struct PlanKey {
+  std::vector<uint32_t> v;
+  bool operator<(const PlanKey &o) const { return v < o.v; }
+};
+PlanKey plan_key(const std::vector<PassLaunch> &ps) {
+  PlanKey k;
+  for (const PassLaunch &pl : ps) {
+    const FftPass2 &p = pl.p;
+    k.v.insert(k.v.end(), {p.T, p.lo, p.w, p.m, pl.wg_log, p.n_stages, p.fixed});
+    for (uint32_t i = 0; i < p.n_stages; i++) k.v.insert(k.v.end(), {p.st[i].b0, p.st[i].r_lo, p.st[i].r_hi});
+  }
+  return k;
+}
+
+int time_passes(stwo_hip_ctx *ctx, bool inverse, const std::vector<PassLaunch> &ps, float *best_ms) {
+  hipEvent_t e0 = take_event(ctx), e1 = take_event(ctx);
+  if (!e0 || !e1) return set_error(ctx, STWO_HIP_EDEVICE, "autotune: no events");
+  int rc = STWO_HIP_OK;
+  float best = 1e30f;
+  const bool prof = ctx->prof;
+  ctx->prof = false;
+  for (int it = 0; it < 4 && rc == STWO_HIP_OK; it++) {
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = STWO_HIP_EDEVICE;
+    for (const PassLaunch &pl : ps)
+      if (rc == STWO_HIP_OK) rc = launch_fft_pass(ctx, inverse, pl.p, pl.wgs, pl.wg_log);
+    if (rc == STWO_HIP_OK && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = STWO_HIP_EDEVICE;
+    if (rc == STWO_HIP_OK && hipEventSynchronize(e1) != hipSuccess) rc = STWO_HIP_EDEVICE;
+    float ms = 0;
+    if (rc == STWO_HIP_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = STWO_HIP_EDEVICE;
+    if (rc == STWO_HIP_OK && it > 0) best = std::min(best, ms);
+  }
+  ctx->prof = prof;
+  ctx->event_pool.push_back(e0);
+  ctx->event_pool.push_back(e1);
+  if (rc) return set_error(ctx, rc, "autotune: timing failed");
+  *best_ms = best;
+  return STWO_HIP_OK;
+}
+
+}  // namespace
+}  // namespace stwo_hip
+
+int stwo_hip_fft_autotune(stwo_hip_ctx *ctx, const uint32_t *log_sizes, size_t n_cols, uint32_t log_blowup,
+                          const stwo_hip_twiddles *tw, uint32_t *report, size_t max_report, size_t *n_report) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, (log_sizes || !n_cols) && tw && tw->device == ctx->device, "fft_autotune: bad arguments");
+  std::map<uint32_t, size_t> count;
+  for (size_t i = 0; i < n_cols; i++) {
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] <= 28 && log_sizes[i] + log_blowup <= 28, "fft_autotune: bad log size %u",
+                       log_sizes[i]);
+    count[log_sizes[i]]++;
+  }
+  size_t nr = 0;
+  for (auto &kv : count) {
+    const uint32_t n = kv.first, N = n + log_blowup;
+    if (n < 4) continue;
+    for (int dir = 0; dir < 2; dir++) {
+      const bool inverse = dir == 0;
+      const uint32_t Nd = inverse ? n : N;
+      int rc = check_domain(ctx, {half_odds_initial(Nd - 1), Nd - 1}, tw);
+      if (rc) return rc;
+      const size_t c = std::max<size_t>(1, std::min(kv.second, ((size_t)1 << 26) >> Nd));
+      void *mem = nullptr;
+      rc = get_scratch(ctx, "fft_autotune", (c << n) * 4 + (inverse ? 0 : (c << Nd) * 4), &mem);
+      if (rc) return rc;
+      uint32_t *x = (uint32_t *)mem, *y = x + (c << n);
+      rc = stwo_hip_fill_random(ctx, x, c << n, 0x7u, n);
+      if (rc) return rc;
+      std::vector<FftCol> ta(c), tb(c), tc(c);
+      for (size_t i = 0; i < c; i++) {
+        ta[i] = {x + (i << n), x + (i << n)};
+        tb[i] = {x + (i << n), y + (i << Nd)};
+        tc[i] = {y + (i << Nd), y + (i << Nd)};
+      }
+      void *d[3];
+      const void *srcs[3] = {ta.data(), tb.data(), tc.data()};
+      const size_t bytes[3] = {c * sizeof(FftCol), c * sizeof(FftCol), c * sizeof(FftCol)};
+      rc = upload_tables(ctx, 3, srcs, bytes, d);
+      if (rc) return rc;
+      const FftCol *first = (const FftCol *)(inverse ? d[0] : d[1]);
+      const FftCol *inpl = (const FftCol *)(inverse ? d[0] : d[2]);
+      const uint32_t *tw_end = (inverse ? tw->d_itw2 : tw->d_tw2) + tw->len;
+      std::map<PlanKey, FftTune> cands;
+      FftTune def;
+      std::vector<PassLaunch> ps;
+      rc = make_group_passes(ctx, n, Nd, inverse, first, inpl, (uint32_t)c, tw_end, true, ps, &def);
+      if (rc) return rc;
+      const PlanKey def_key = plan_key(ps);
+      cands[def_key] = def;
+      for (uint32_t t0 = 10; t0 <= 13; t0++)
+        for (uint32_t sm = 5; sm <= 8; sm++)
+          for (uint32_t wg : {0u, 9u}) {
+            FftTune t;
+            t.t0 = (uint8_t)t0;
+            t.smax = (uint8_t)sm;
+            t.wg = (uint8_t)wg;
+            rc = make_group_passes(ctx, n, Nd, inverse, first, inpl, (uint32_t)c, tw_end, true, ps, &t);
+            if (rc) return rc;
+            PlanKey k = plan_key(ps);
+            if (!cands.count(k)) cands[k] = t;
+          }
+      float best = 1e30f, def_ms = 0;
+      FftTune best_t = def;
+      for (auto &ck : cands) {
+        rc = make_group_passes(ctx, n, Nd, inverse, first, inpl, (uint32_t)c, tw_end, true, ps, &ck.second);
+        if (rc) return rc;
+        float ms = 0;
+        rc = time_passes(ctx, inverse, ps, &ms);
+        if (rc) return rc;
+        if (!(ck.first < def_key) && !(def_key < ck.first)) def_ms = ms;
+        if (ms < best) {
+          best = ms;
+          best_t = ck.second;
+        }
+      }
+      ctx->fft_tune[fft_tune_key(n, Nd, inverse)] = best_t;
+      if (report && nr < max_report) {
+        uint32_t *o = report + 8 * nr;
+        o[0] = n; o[1] = Nd; o[2] = inverse; o[3] = best_t.t0; o[4] = best_t.smax; o[5] = best_t.wg;
+        o[6] = (uint32_t)(best * 1e3f + 0.5f);    // microseconds
+        o[7] = (uint32_t)(def_ms * 1e3f + 0.5f);  // the default plan's microseconds
+      }
+      nr++;
+    }
+  }
+  if (n_report) *n_report = nr;
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_fft_tune_set(stwo_hip_ctx *ctx, const uint32_t *entries, size_t n_entries) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, entries || !n_entries, "fft_tune_set: NULL entries");
+  for (size_t i = 0; i < n_entries; i++) {
+    const uint32_t *e = entries + 6 * i;
+    STWO_HIP_CHECK_ARG(ctx, e[1] >= 4 && e[1] <= 28 && e[0] <= e[1] && e[2] <= 1 && e[3] >= 10 && e[3] <= 13 &&
+                                e[4] >= 5 && e[4] <= 8 && (e[5] == 0 || e[5] == 9),
+                       "fft_tune_set: entry %zu invalid", i);
+    FftTune t;
+    t.t0 = (uint8_t)e[3];
+    t.smax = (uint8_t)e[4];
+    t.wg = (uint8_t)e[5];
+    ctx->fft_tune[fft_tune_key(e[0], e[1], e[2] != 0)] = t;
+  }
+  return STWO_HIP_OK;
+}
+
+int stwo_hip_fft_tune_clear(stwo_hip_ctx *ctx) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  ctx->fft_tune.clear();
+  return STWO_HIP_OK;
+}
+
 // Describes the passes the radix-16 path would run for a transform of log
 // size N (coefficient log size n), in execution order, 16 words per pass:
 // T, lo, w, m, rep_log, wg_log, in_log, scaled, n_stages, 4 packed stages
 // (b0 << 16 | r_lo << 8 | r_hi), workgroups for one column, 0, 0.
 // Host-only (launches nothing); lets tests check the planner without a device.
-int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, uint32_t max_passes,
-                            uint32_t *n_passes) {
+int stwo_hip_debug_fft_plan_tuned(uint32_t n, uint32_t N, int inverse, uint32_t t0, uint32_t smax, uint32_t wg,
+                                  uint32_t *out, uint32_t max_passes, uint32_t *n_passes) {
   if (!out || !n_passes || N < 4 || N > 28 || n > N) return STWO_HIP_EINVAL;
   std::vector<PassLaunch> passes;
   stwo_hip_ctx dummy;
-  int rc = make_group_passes(&dummy, inverse ? N : n, N, inverse != 0, nullptr, nullptr, 1, nullptr, true, passes);
+  FftTune tn;
+  if (t0) tn.t0 = (uint8_t)t0;
+  if (smax) tn.smax = (uint8_t)smax;
+  tn.wg = (uint8_t)wg;
+  int rc = make_group_passes(&dummy, inverse ? N : n, N, inverse != 0, nullptr, nullptr, 1, nullptr, true, passes,
+                             &tn);
   if (rc) return rc;
   if (passes.size() > max_passes) return STWO_HIP_EINVAL;
   for (size_t k = 0; k < passes.size(); k++) {
@@ -1185,10 +1654,15 @@ int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, 
       o[9 + i] = i < p.n_stages ? (p.st[i].b0 << 16) | (p.st[i].r_lo << 8) | p.st[i].r_hi : 0;
     o[13] = passes[k].wgs;
     o[14] = p.vec_io;
-    o[15] = 0;
+    o[15] = p.fixed;
   }
   *n_passes = (uint32_t)passes.size();
   return STWO_HIP_OK;
+}
+
+int stwo_hip_debug_fft_plan(uint32_t n, uint32_t N, int inverse, uint32_t *out, uint32_t max_passes,
+                            uint32_t *n_passes) {
+  return stwo_hip_debug_fft_plan_tuned(n, N, inverse, 0, 0, 0, out, max_passes, n_passes);
 }
 
 }  // extern "C"

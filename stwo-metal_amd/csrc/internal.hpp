@@ -54,15 +54,25 @@ struct PassLaunch {
 };
 
 // `aligned16`: every column pointer of the group is 16-byte aligned.
+// `tune`: the planner choices to use; NULL = the context's choice for the
+// shape (stwo_hip_fft_autotune), else the defaults.
 int make_group_passes(stwo_hip_ctx *ctx, uint32_t n, uint32_t N, bool inverse, const FftCol *d_tab,
                       const FftCol *d_tab_inplace, uint32_t n_cols, const uint32_t *tw_end, bool aligned16,
-                      std::vector<PassLaunch> &out);
+                      std::vector<PassLaunch> &out, const FftTune *tune = nullptr);
 int launch_fft_pass(stwo_hip_ctx *ctx, bool inverse, const FftPass2 &p, uint32_t wgs, uint32_t wg_log);
 // `kind`: the schedule every group of the launch matches: 1 to 5 (FftPass2
 // fixed, with its workgroup size), -1 (tiles of 2^10 or more: wave-uniform
 // columns) or 0 (any).
 int launch_fft_multi(stwo_hip_ctx *ctx, bool inverse, const FftPass2 *d_groups, const uint32_t *d_wg_start,
                      uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind);
+
+// The IFFT's last pass and the RFFT's first pass of one column group run as
+// one fused kernel when their tiles match (lde_fusable); kind 2 = the
+// compile-time 8-layer pair, 0 = run-time stage lists.
+bool lde_fusable(const PassLaunch &last_inv, const PassLaunch &first_fwd, uint32_t log_blowup);
+int lde_kind(const PassLaunch &last_inv, const PassLaunch &first_fwd);
+int launch_lde_multi(stwo_hip_ctx *ctx, const FftPass2 *d_pis, const FftPass2 *d_pfs, const uint32_t *d_wg_start,
+                     uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind, uint32_t store_coeffs);
 
 // ---- decommitment: batched device gather -----------------------------------
 

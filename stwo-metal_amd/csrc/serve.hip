@@ -159,7 +159,7 @@ int run_job(stwo_hip_server *s, stwo_hip_ctx *ctx, Job &j) {
   }
   stwo_hip_batch *plan = nullptr;
   int rc = stwo_hip_batch_create(ctx, cols.data(), logs.data(), d.n_cols, d.log_blowup, s->tw, outs.data(), tree,
-                                 STWO_HIP_TREE_LIFTED, STWO_HIP_HASH_BLAKE2S, 0, &plan);
+                                 STWO_HIP_TREE_LIFTED, STWO_HIP_HASH_BLAKE2S, STWO_HIP_BATCH_DROP_COEFFS, &plan);
   if (rc) return rc;
   rc = stwo_hip_batch_run(ctx, plan);
   if (rc == STWO_HIP_OK) rc = stwo_hip_download(ctx, j.res.trace_root, tree, 8);

@@ -21,6 +21,7 @@ P = (1 << 31) - 1
 HASH_BLAKE2S = 0
 HASH_BLAKE2S_M31 = 1
 TREE_MIXED = 0
+BATCH_GRAPH, BATCH_DROP_COEFFS = 1, 2
 TREE_LIFTED = 1
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -114,6 +115,8 @@ def lib():
             "stwo_hip_evaluate": [_vp, _vp, ctypes.c_uint32, Coset, _vp, _vp],
             "stwo_hip_evaluate_polynomials": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
                                               ctypes.POINTER(_vp)],
+            "stwo_hip_extend_evals": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
+                                      ctypes.POINTER(_vp), ctypes.c_int],
             "stwo_hip_merkle_commit_on_layer": [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(_vp), ctypes.c_size_t, _vp,
                                                 ctypes.c_int],
             "stwo_hip_merkle_commit": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, _vp, ctypes.c_int],
@@ -173,6 +176,12 @@ def lib():
             "stwo_hip_stark_proof_destroy": [_vp],
             "stwo_hip_debug_fft_plan": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.c_uint32,
                                         _u32p],
+            "stwo_hip_debug_fft_plan_tuned": [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p],
+            "stwo_hip_fft_autotune": [_vp, _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp, _u32p, ctypes.c_size_t,
+                                      ctypes.POINTER(ctypes.c_size_t)],
+            "stwo_hip_fft_tune_set": [_vp, _u32p, ctypes.c_size_t],
+            "stwo_hip_fft_tune_clear": [_vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -339,6 +348,30 @@ class Context:
         ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
         self._check(lib().stwo_hip_evaluate_polynomials(self.h, _ptrs(coeffs), ls.ctypes.data_as(_u32p), len(coeffs),
                                                         log_blowup, tw.h, _ptrs(outs)))
+
+    def fft_autotune(self, log_sizes, log_blowup, tw):
+        """stwo_hip_fft_autotune: returns [(n, N, inverse, t0, smax, wg, best_us, default_us)]."""
+        ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
+        rep = np.zeros(8 * 64, dtype=np.uint32)
+        nr = ctypes.c_size_t()
+        self._check(lib().stwo_hip_fft_autotune(self.h, ls.ctypes.data_as(_u32p), len(log_sizes), log_blowup, tw.h,
+                                                rep.ctypes.data_as(_u32p), 64, ctypes.byref(nr)))
+        return [tuple(int(v) for v in rep[8 * i: 8 * i + 8]) for i in range(min(nr.value, 64))]
+
+    def fft_tune_set(self, entries):
+        """entries: [(n, N, inverse, t0, smax, wg)]."""
+        e = np.ascontiguousarray([v for t in entries for v in t] or [0], dtype=np.uint32)
+        self._check(lib().stwo_hip_fft_tune_set(self.h, e.ctypes.data_as(_u32p), len(entries)))
+
+    def fft_tune_clear(self):
+        self._check(lib().stwo_hip_fft_tune_clear(self.h))
+
+    def extend_evals(self, cols, log_sizes, log_blowup, tw, outs, store_coeffs=False):
+        """interpolate_columns + evaluate_polynomials in one call (TreeBuilder::extend_evals);
+        cols are consumed (coefficients afterwards only with store_coeffs)."""
+        ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
+        self._check(lib().stwo_hip_extend_evals(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols), log_blowup,
+                                                tw.h, _ptrs(outs), int(bool(store_coeffs))))
 
     # -- Merkle -----------------------------------------------------------------
     def commit_on_layer(self, log_size, prev, cols, out, mode=HASH_BLAKE2S):
@@ -519,13 +552,16 @@ class Context:
 
     # -- batched commitment -------------------------------------------------------
     def batch_create(self, cols, log_sizes, log_blowup, tw, outs, layers, tree_kind=TREE_MIXED,
-                     mode=HASH_BLAKE2S, use_graph=True):
+                     mode=HASH_BLAKE2S, use_graph=True, drop_coeffs=False):
+        """stwo_hip_batch_create; drop_coeffs: the fused pass does not store the
+        coefficients (cols are left unspecified)."""
         ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
         h = _vp()
+        flags = (BATCH_GRAPH if use_graph else 0) | (BATCH_DROP_COEFFS if drop_coeffs else 0)
         self._check(lib().stwo_hip_batch_create(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols), log_blowup,
                                                 tw.h, _ptrs(outs),
                                                 layers.ptr if isinstance(layers, DeviceBuf) else layers,
-                                                tree_kind, mode, int(use_graph), ctypes.byref(h)))
+                                                tree_kind, mode, flags, ctypes.byref(h)))
         return Batch(self, h.value)
 
     # -- profiling --------------------------------------------------------------
@@ -593,13 +629,14 @@ class FriProver:
             self.h = None
 
 
-def fft_plan(n, N, inverse):
+def fft_plan(n, N, inverse, t0=0, smax=0, wg=0):
     """The FFT passes the library runs for a transform of 2^n values (RFFT:
     coefficients) on a domain of log size N: one dict per pass, in launch
-    order (stwo_hip_debug_fft_plan; launches nothing)."""
+    order (stwo_hip_debug_fft_plan_tuned; launches nothing).  t0, smax, wg:
+    planner choices as for Context.fft_tune_set (0 = default)."""
     buf = (ctypes.c_uint32 * (16 * 8))()
     k = ctypes.c_uint32()
-    rc = lib().stwo_hip_debug_fft_plan(n, N, int(inverse), buf, 8, ctypes.byref(k))
+    rc = lib().stwo_hip_debug_fft_plan_tuned(n, N, int(inverse), t0, smax, wg, buf, 8, ctypes.byref(k))
     if rc != 0:
         raise StwoHipError(rc, "fft plan (n=%d, N=%d) rejected" % (n, N))
     out = []
@@ -607,7 +644,7 @@ def fft_plan(n, N, inverse):
         w = list(buf[16 * i: 16 * i + 16])
         stages = [(v >> 16, (v >> 8) & 0xFF, v & 0xFF) for v in w[9: 9 + w[8]]]
         out.append(dict(T=w[0], lo=w[1], w=w[2], m=w[3], rep=w[4], wg=w[5], in_log=w[6], scaled=w[7],
-                        stages=stages, wgs=w[13], vec_io=w[14]))
+                        stages=stages, wgs=w[13], vec_io=w[14], fixed=w[15]))
     return out
 
 
