@@ -119,6 +119,29 @@ __host__ __device__ __forceinline__ qm31 qm31_mul(const qm31 &x, const qm31 &y) 
   return r;
 }
 
+// CM31 inverse: (a + bi)^-1 = (a - bi) / (a^2 + b^2) (STWO/core/fields/cm31.rs).
+__host__ __device__ __forceinline__ void cm31_inv(uint32_t a, uint32_t b, uint32_t &ra, uint32_t &rb) {
+  const uint32_t n = m31_inv(m31_add(m31_mul(a, a), m31_mul(b, b)));
+  ra = m31_mul(a, n);
+  rb = m31_mul(m31_neg(b), n);
+}
+
+// QM31 inverse: (x + yu)^-1 = (x - yu) / (x^2 - (2 + i) y^2), the denominator
+// in CM31 (STWO/core/fields/qm31.rs).
+__host__ __device__ __forceinline__ qm31 qm31_inv(const qm31 &v) {
+  uint32_t x2r, x2i, y2r, y2i;
+  cm31_mul(v.v[0], v.v[1], v.v[0], v.v[1], x2r, x2i);
+  cm31_mul(v.v[2], v.v[3], v.v[2], v.v[3], y2r, y2i);
+  // (2 + i)(y2r + y2i i) = (2 y2r - y2i) + (y2r + 2 y2i) i
+  const uint32_t ry = m31_sub(m31_add(y2r, y2r), y2i), iy = m31_add(y2r, m31_add(y2i, y2i));
+  uint32_t dr, di;
+  cm31_inv(m31_sub(x2r, ry), m31_sub(x2i, iy), dr, di);
+  uint32_t ar, ai, br, bi;
+  cm31_mul(v.v[0], v.v[1], dr, di, ar, ai);
+  cm31_mul(m31_neg(v.v[2]), m31_neg(v.v[3]), dr, di, br, bi);
+  return qm31_make(ar, ai, br, bi);
+}
+
 // ---- circle group (STWO/core/circle.rs) ----------------------------------
 
 struct cpoint {

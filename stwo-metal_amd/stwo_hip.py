@@ -115,6 +115,12 @@ def lib():
             "stwo_hip_evaluate": [_vp, _vp, ctypes.c_uint32, Coset, _vp, _vp],
             "stwo_hip_evaluate_polynomials": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
                                               ctypes.POINTER(_vp)],
+            "stwo_hip_barycentric_weights": [_vp, ctypes.c_uint32, _u32p, ctypes.POINTER(_vp)],
+            "stwo_hip_barycentric_eval_at_point": [_vp, ctypes.POINTER(_vp), ctypes.c_size_t, ctypes.c_uint32,
+                                                   ctypes.POINTER(_vp), _u32p],
+            "stwo_hip_eval_at_point_by_folding": [_vp, _vp, ctypes.c_uint32, _u32p, _vp, _u32p],
+            "stwo_hip_split_at_mid": [_vp, _vp, ctypes.c_uint32, _vp, _vp],
+            "stwo_hip_fri_decompose": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(_vp), _u32p],
             "stwo_hip_extend_evals": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
                                       ctypes.POINTER(_vp), ctypes.c_int],
             "stwo_hip_merkle_commit_on_layer": [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(_vp), ctypes.c_size_t, _vp,
@@ -348,6 +354,41 @@ class Context:
         ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
         self._check(lib().stwo_hip_evaluate_polynomials(self.h, _ptrs(coeffs), ls.ctypes.data_as(_u32p), len(coeffs),
                                                         log_blowup, tw.h, _ptrs(outs)))
+
+    # -- the other PolyOps / FriOps methods ---------------------------------------
+    @staticmethod
+    def _pt(point):
+        return np.ascontiguousarray(list(point[0]) + list(point[1]), dtype=np.uint32)
+
+    def barycentric_weights(self, log_size, point, out4):
+        pt = self._pt(point)
+        self._check(lib().stwo_hip_barycentric_weights(self.h, log_size, pt.ctypes.data_as(_u32p), _ptrs(out4)))
+
+    def barycentric_eval_at_point(self, evals, log_size, weights4):
+        """[n][4] values (PolyOps::barycentric_eval_at_point of each column)."""
+        out = np.zeros(4 * max(1, len(evals)), dtype=np.uint32)
+        self._check(lib().stwo_hip_barycentric_eval_at_point(self.h, _ptrs(evals), len(evals), log_size,
+                                                             _ptrs(weights4), out.ctypes.data_as(_u32p)))
+        return out[: 4 * len(evals)].reshape(-1, 4)
+
+    def eval_at_point_by_folding(self, evals, log_size, point, tw):
+        pt = self._pt(point)
+        out = np.zeros(4, dtype=np.uint32)
+        self._check(lib().stwo_hip_eval_at_point_by_folding(self.h, evals.ptr if isinstance(evals, DeviceBuf)
+                                                            else evals, log_size, pt.ctypes.data_as(_u32p), tw.h,
+                                                            out.ctypes.data_as(_u32p)))
+        return tuple(int(v) for v in out)
+
+    def split_at_mid(self, coeffs, log_size, left, right):
+        self._check(lib().stwo_hip_split_at_mid(self.h, *[b.ptr if isinstance(b, DeviceBuf) else b
+                                                          for b in (coeffs,)], log_size,
+                                                *[b.ptr if isinstance(b, DeviceBuf) else b for b in (left, right)]))
+
+    def fri_decompose(self, eval4, log_size, g4):
+        lam = np.zeros(4, dtype=np.uint32)
+        self._check(lib().stwo_hip_fri_decompose(self.h, _ptrs(eval4), log_size, _ptrs(g4),
+                                                 lam.ctypes.data_as(_u32p)))
+        return tuple(int(v) for v in lam)
 
     def fft_autotune(self, log_sizes, log_blowup, tw):
         """stwo_hip_fft_autotune: returns [(n, N, inverse, t0, smax, wg, best_us, default_us)]."""
