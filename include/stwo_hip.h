@@ -318,6 +318,45 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
 int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
                            const stwo_hip_secure_point *point, uint32_t *values);
 
+/* ---- the other PolyOps / FriOps methods -----------------------------------------
+ * PolyOps::barycentric_weights (STWO/prover/poly/circle/ops.rs:67-71; CpuBackend
+ * CPUB/circle.rs:101-132; Metal SH/barycentric.metal): the weights of
+ * CanonicCoset(log_size) (bit-reversed order) for the secure point p, which
+ * must not lie in the domain; out = 4 device coordinate columns of
+ * 2^log_size.  Asynchronous. */
+int stwo_hip_barycentric_weights(stwo_hip_ctx *ctx, uint32_t log_size, const stwo_hip_secure_point *p,
+                                 uint32_t *const out[4]);
+/* PolyOps::barycentric_eval_at_point (ops.rs:73-80; CPUB/circle.rs:134-141) for a
+ * batch of columns of one size: values (host, 4 words per column) = sum_i
+ * evals[c][i] * weights[i], evals[c] holding 2^log_size bit-reversed
+ * evaluations on CanonicCoset(log_size) and weights those of
+ * stwo_hip_barycentric_weights for that size and point -- the PCS's sampling
+ * path when coefficients are not stored (STWO/prover/pcs/mod.rs:481-497, 638-657).
+ * Synchronizes. */
+int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *evals, size_t n_cols,
+                                       uint32_t log_size, const uint32_t *const weights[4], uint32_t *values);
+/* PolyOps::eval_at_point_by_folding (ops.rs:82-88; CPUB/circle.rs:143-180): the
+ * evaluation (2^log_size bit-reversed values on CanonicCoset(log_size), device)
+ * folded circle -> line and line -> point with get_folding_alphas(p)
+ * (core/poly/utils.rs:45-70), divided by 2^log_size; value: 4 words (host).
+ * log_size 3..28; tw must cover the domain.  Synchronizes. */
+int stwo_hip_eval_at_point_by_folding(stwo_hip_ctx *ctx, const uint32_t *evals, uint32_t log_size,
+                                      const stwo_hip_secure_point *p, const stwo_hip_twiddles *tw,
+                                      uint32_t value[4]);
+/* PolyOps::split_at_mid (ops.rs:163; CPUB/circle.rs:283-291): the first and the
+ * second half of 2^log_size coefficients into left and right (each 2^(log_size-1)
+ * words; left may be coeffs itself), so that p(z) = left(z) + pi^(L-2)(z.x) right(z).
+ * Asynchronous. */
+int stwo_hip_split_at_mid(stwo_hip_ctx *ctx, const uint32_t *coeffs, uint32_t log_size, uint32_t *left,
+                          uint32_t *right);
+/* FriOps::decompose (STWO/prover/fri.rs:86-94; CPUB/fri.rs:41-63,141-156): for a
+ * secure evaluation (4 coordinate columns of 2^log_size, bit-reversed on
+ * CanonicCoset(log_size)), lambda = (sum of the first half - sum of the second
+ * half) / 2^log_size (host, 4 words) and g = eval - lambda on the first half,
+ * eval + lambda on the second (4 device columns; may be eval).  Synchronizes. */
+int stwo_hip_fri_decompose(stwo_hip_ctx *ctx, const uint32_t *const eval[4], uint32_t log_size, uint32_t *const g[4],
+                           uint32_t lambda[4]);
+
 /* ---- wide-Fibonacci AIR (the prove benchmark's component) ----------------------
  * vendor/stwo-upstream-dev-62b228e/crates/examples/src/wide_fibonacci/mod.rs.
  * stwo_hip_wide_fib_trace: generate_trace (mod.rs:27-50) -- row i of the n_cols

@@ -46,6 +46,15 @@ def test_every_declared_symbol_is_exported(lib):
         getattr(lib, n)
 
 
+def test_every_exported_symbol_is_declared(lib):
+    """The reverse direction: no C-ABI entry point the library exports is
+    missing from include/."""
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    undeclared = sorted(n for n in exported if n.startswith("stwo_hip_") and n not in declared_functions())
+    assert not undeclared, undeclared
+
+
 def test_null_arguments_are_rejected_without_a_device(lib):
     vp = ctypes.c_void_p
     lib.stwo_hip_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
