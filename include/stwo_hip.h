@@ -318,6 +318,23 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
 int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
                            const stwo_hip_secure_point *point, uint32_t *values);
 
+/* stwo_hip_eval_at_point for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference: its only purpose is to make
+ * no allocating call (no hipMalloc / hipFree / allocating library routine;
+ * with launch profiling off).  Same arguments and results as
+ * stwo_hip_eval_at_point.  The kernels' scratch (basis table, block
+ * partials, per-column sums) is a fixed 1 MiB array per device, resolved at
+ * context creation and held until the process ends: the call is rejected
+ * with STWO_HIP_EINVAL when 8192 + 4 * (n_cols + blocks) words exceed 2^18
+ * (blocks = sum over columns of max(1, 2^(log_size - 11)); e.g. up to 256
+ * columns of 2^18 coefficients, or 48 of 2^20) or its tables exceed half
+ * the context's 8 MiB table arena.  Every call on one device shares the
+ * array: calls that overlap on one GPU (from two contexts or threads) race
+ * and must be serialized by the caller.  Synchronizes. */
+int stwo_hip_eval_at_point_noalloc(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes,
+                                   size_t n_cols, const stwo_hip_secure_point *point, uint32_t *values);
+
 /* ---- the other PolyOps / FriOps methods -----------------------------------------
  * PolyOps::barycentric_weights (STWO/prover/poly/circle/ops.rs:67-71; CpuBackend
  * CPUB/circle.rs:101-132; Metal SH/barycentric.metal): the weights of

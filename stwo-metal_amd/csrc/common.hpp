@@ -120,6 +120,13 @@ int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const si
 
 // Enqueues the copy of the pending tables (none: nothing to do).
 int flush_uploads(stwo_hip_ctx *ctx);
+// Copies the pending tables, drains the stream and returns every copy's
+// event to the pool (which context creation stocks), so that the next
+// upload and launch take pooled events and create none.
+int arena_quiesce(stwo_hip_ctx *ctx);
+// Resolves, once per device, the address of the fixed scratch array of
+// stwo_hip_eval_at_point_noalloc (context creation calls it).
+int eval_point_scratch_setup(stwo_hip_ctx *ctx);
 
 // Returns a device scratch buffer of at least `bytes` for `key` (grown on
 // demand; growing synchronizes the stream before freeing the old buffer).

@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -169,14 +170,50 @@ __global__ void __launch_bounds__(256) eval_point_sum_kernel(const EvalCol *cols
     for (int w = 0; w < 4; w++) out[4 * (size_t)ci + w] = red[0].v[w];
 }
 
+// ---- fixed scratch for the entry point that allocates nothing -------------
+//
+// stwo_hip_eval_at_point_noalloc keeps its basis table, block partials and
+// per-column sums in this file-scope array (one copy per device, held until
+// the process ends) instead of buffers it would allocate.  Layout in words:
+// basis [0, kNaBasisWords), sums [kNaBasisWords, + 4 n_cols), partials after
+// them.  Host code never passes the array's name to a kernel or a copy (on
+// the host it is only a stand-in for the symbol): it uses the per-device
+// address that hipGetSymbolAddress returned at context creation.
+constexpr size_t kNaScratchWords = 1u << 18;  // 1 MiB
+constexpr size_t kNaBasisWords = (size_t)4 << kEvalBlockLog;
+static __device__ __align__(256) uint32_t scratch[kNaScratchWords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_na_mu;
+static uint32_t *g_na_ptr[kMaxDevices];
+
 }  // namespace stwo_hip
 
-using namespace stwo_hip;
+namespace stwo_hip {
 
-extern "C" {
+// Resolves this device's address of `scratch` (the device is current).
+// Called once per device, from context creation.
+int eval_point_scratch_setup(stwo_hip_ctx *ctx) {
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(ctx, STWO_HIP_EDEVICE, "device %d beyond the %d the library supports", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr[dev]) return STWO_HIP_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_error(ctx, STWO_HIP_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                     hipGetErrorString(e));
+  g_na_ptr[dev] = (uint32_t *)ptr;
+  return STWO_HIP_OK;
+}
 
-int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
-                           const stwo_hip_secure_point *point, uint32_t *values) {
+// `na_scratch`: NULL = buffers from the context's scratch pool (allocated on
+// first use or growth); otherwise the fixed per-device array, with every
+// size checked against it and no allocating call made.
+static int eval_at_point_impl(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes,
+                              size_t n_cols, const stwo_hip_secure_point *point, uint32_t *values,
+                              uint32_t *na_scratch) {
   if (!ctx) return STWO_HIP_EINVAL;
   STWO_HIP_CHECK_ARG(ctx, point && values && (coeffs || !n_cols) && (log_sizes || !n_cols),
                      "eval_at_point: NULL argument");
@@ -214,6 +251,17 @@ int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, con
     c = e;
   }
   const uint32_t n_basis = 1u << std::min(max_log, kEvalBlockLog);
+  if (na_scratch) {
+    // Everything must fit the fixed array, and the tables the ring arena
+    // (larger ones would take an allocated buffer).
+    STWO_HIP_CHECK_ARG(ctx, kNaBasisWords + 4 * ((uint64_t)n_cols + total) <= kNaScratchWords,
+                       "eval_at_point_noalloc: %zu columns and %llu blocks of 2^11 coefficients need %llu words of "
+                       "scratch, more than the %zu available",
+                       n_cols, (unsigned long long)total,
+                       (unsigned long long)(kNaBasisWords + 4 * ((uint64_t)n_cols + total)), kNaScratchWords);
+    STWO_HIP_CHECK_ARG(ctx, tab.size() * sizeof(EvalCol) + work.size() * sizeof(EvalWork) + 512 <= ctx->arena_cap / 2,
+                       "eval_at_point_noalloc: %zu work items exceed the table arena", work.size());
+  }
   void *d_tab = nullptr, *d_work = nullptr, *d_basis = nullptr, *d_part = nullptr, *d_out = nullptr;
   int rc;
   {
@@ -225,12 +273,18 @@ int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, con
     d_tab = d[0];
     d_work = d[1];
   }
-  rc = get_scratch(ctx, "eval_point_basis", (size_t)n_basis * 16, &d_basis);
-  if (rc) return rc;
-  rc = get_scratch(ctx, "eval_point_partials", total * 16, &d_part);
-  if (rc) return rc;
-  rc = get_scratch(ctx, "eval_point_out", n_cols * 16, &d_out);
-  if (rc) return rc;
+  if (na_scratch) {
+    d_basis = na_scratch;
+    d_out = na_scratch + kNaBasisWords;
+    d_part = na_scratch + kNaBasisWords + 4 * n_cols;
+  } else {
+    rc = get_scratch(ctx, "eval_point_basis", (size_t)n_basis * 16, &d_basis);
+    if (rc) return rc;
+    rc = get_scratch(ctx, "eval_point_partials", total * 16, &d_part);
+    if (rc) return rc;
+    rc = get_scratch(ctx, "eval_point_out", n_cols * 16, &d_out);
+    if (rc) return rc;
+  }
   rc = launch(ctx, "eval_at_point_basis", eval_basis_kernel, dim3((n_basis + 255) / 256), dim3(256), 0, f,
               (uint32_t *)d_basis, n_basis);
   if (rc) return rc;
@@ -241,6 +295,31 @@ int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, con
               (const EvalCol *)d_tab, (uint32_t)n_cols, (uint32_t)total, (const uint32_t *)d_part, (uint32_t *)d_out);
   if (rc) return rc;
   return stwo_hip_download(ctx, values, (const uint32_t *)d_out, 4 * n_cols);
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
+                           const stwo_hip_secure_point *point, uint32_t *values) {
+  return eval_at_point_impl(ctx, coeffs, log_sizes, n_cols, point, values, nullptr);
+}
+
+int stwo_hip_eval_at_point_noalloc(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes,
+                                   size_t n_cols, const stwo_hip_secure_point *point, uint32_t *values) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  uint32_t *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (ctx->device >= 0 && ctx->device < kMaxDevices) ptr = g_na_ptr[ctx->device];
+  }
+  STWO_HIP_CHECK_ARG(ctx, ptr != nullptr, "eval_at_point_noalloc: scratch of device %d not set up", ctx->device);
+  int rc = arena_quiesce(ctx);
+  if (rc) return rc;
+  return eval_at_point_impl(ctx, coeffs, log_sizes, n_cols, point, values, ptr);
 }
 
 }  // extern "C"

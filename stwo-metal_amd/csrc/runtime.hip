@@ -62,6 +62,7 @@ void prof_end(stwo_hip_ctx *ctx, const char *name, hipEvent_t start) {
 
 static const size_t kArenaBytes = 8u << 20;
 static const size_t kStageBytes = 1u << 20;
+static const int kEventPoolInit = 16;
 
 static int arena_init(stwo_hip_ctx *ctx) {
   STWO_HIP_TRY(ctx, hipHostMalloc((void **)&ctx->h_arena, kArenaBytes, hipHostMallocDefault));
@@ -186,6 +187,17 @@ int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const si
   return STWO_HIP_OK;
 }
 
+int arena_quiesce(stwo_hip_ctx *ctx) {
+  int rc = flush_uploads(ctx);
+  if (rc) return rc;
+  STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  while (!ctx->inflight.empty()) {
+    give_event(ctx, ctx->inflight.front().ev);
+    ctx->inflight.pop_front();
+  }
+  return STWO_HIP_OK;
+}
+
 int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr) {
   return upload_tables(ctx, 1, &src, &bytes, dptr);
 }
@@ -254,6 +266,17 @@ int stwo_hip_ctx_create(int device, stwo_hip_ctx **out) {
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return STWO_HIP_ENOMEM;
+  }
+  // Events for the arena's copies, created up front so that calls which
+  // must not allocate find the pool stocked.
+  for (int i = 0; i < kEventPoolInit; i++) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) break;
+    ctx->event_pool.push_back(e);
+  }
+  if (eval_point_scratch_setup(ctx) != STWO_HIP_OK) {
+    stwo_hip_ctx_destroy(ctx);
+    return STWO_HIP_EDEVICE;
   }
   *out = ctx;
   return STWO_HIP_OK;

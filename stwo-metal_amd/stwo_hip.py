@@ -170,6 +170,7 @@ def lib():
             "stwo_hip_compute_fri_quotients": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_void_p,
                                                _u32p, _u32p, ctypes.c_uint32, ctypes.POINTER(_vp)],
             "stwo_hip_eval_at_point": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, _u32p, _u32p],
+            "stwo_hip_eval_at_point_noalloc": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, _u32p, _u32p],
             "stwo_hip_wide_fib_trace": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32, _vp, _vp],
             "stwo_hip_wide_fib_constraint_quotients": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
                                                        _u32p, ctypes.POINTER(_vp)],
@@ -512,14 +513,15 @@ class Context:
                                                          fl.ctypes.data, cn.ctypes.data_as(_u32p),
                                                          rc.ctypes.data_as(_u32p), lifting_log, _ptrs(out4)))
 
-    def eval_at_point(self, coeff_cols, log_sizes, point):
-        """PolyOps::eval_at_point of each polynomial at the secure point (x[4], y[4]); returns [n][4]."""
+    def eval_at_point(self, coeff_cols, log_sizes, point, noalloc=False):
+        """PolyOps::eval_at_point of each polynomial at the secure point (x[4], y[4]); returns [n][4].
+        noalloc: through stwo_hip_eval_at_point_noalloc (fixed per-device scratch)."""
         pt = np.ascontiguousarray(list(point[0]) + list(point[1]), dtype=np.uint32)
         ls = np.ascontiguousarray(list(log_sizes) or [0], dtype=np.uint32)
         out = np.zeros(4 * max(1, len(coeff_cols)), dtype=np.uint32)
-        self._check(lib().stwo_hip_eval_at_point(self.h, _ptrs(coeff_cols), ls.ctypes.data_as(_u32p),
-                                                 len(coeff_cols), pt.ctypes.data_as(_u32p),
-                                                 out.ctypes.data_as(_u32p)))
+        fn = lib().stwo_hip_eval_at_point_noalloc if noalloc else lib().stwo_hip_eval_at_point
+        self._check(fn(self.h, _ptrs(coeff_cols), ls.ctypes.data_as(_u32p), len(coeff_cols),
+                       pt.ctypes.data_as(_u32p), out.ctypes.data_as(_u32p)))
         return out[: 4 * len(coeff_cols)].reshape(-1, 4)
 
     def wide_fib_trace(self, cols, log_n, a, b):
