@@ -86,7 +86,11 @@ def uneven_logs(n_cols=1024, lo=10, hi=22, seed=3):
 
 def batched_vs_plain(ctx, steps=3):
     """1,024 uneven columns (log 10..22), LDE at blowup 2, one mixed-degree tree:
-    one batched plan replayed as a hipGraph vs per-column dispatch."""
+    the plain per-column form (stwo_hip_lde_commit_per_column, a C loop of one
+    interpolate and one evaluate per column, then the commit) against one
+    batched plan replayed as a hipGraph (coefficients not stored, the PCS
+    default), with the planner's defaults and after stwo_hip_fft_autotune
+    timed every candidate plan of the set's 26 FFT shapes on the device."""
     logs = uneven_logs()
     b = 1
     max_log = max(logs) + b
@@ -112,54 +116,65 @@ def batched_vs_plain(ctx, steps=3):
         ctx._check(sh.lib().stwo_hip_copy(ctx.h, work.ptr, pristine.ptr, acc))
         ctx.sync()
 
-    # plain: per-column interpolate + evaluate, then the mixed-degree commit
-    plain = []
-    for it in range(steps + 1):
-        restore()
-        t0 = time.perf_counter()
-        for c, o, l in zip(cols, out_cols, logs):
-            ctx.interpolate(c, sh.canonic_domain(l), tw)
-            ctx.evaluate(c, l, sh.canonic_domain(l + b), tw, o)
-        ctx.merkle_commit(out_cols, [l + b for l in logs], layers)
-        ctx.sync()
-        if it:
-            plain.append(time.perf_counter() - t0)
-    root_plain = layers.download(8)
-    plan = ctx.batch_create(cols, logs, b, tw, out_cols, layers, sh.TREE_MIXED, sh.HASH_BLAKE2S, True)
-    batched = []
-    for it in range(steps + 1):
-        restore()
-        t0 = time.perf_counter()
-        plan.run()
-        ctx.sync()
-        if it:
-            batched.append(time.perf_counter() - t0)
-    root_batched = layers.download(8)
+    def timed(fn):
+        ts = []
+        for it in range(steps + 1):
+            restore()
+            t0 = time.perf_counter()
+            fn()
+            ctx.sync()
+            if it:
+                ts.append(time.perf_counter() - t0)
+        return min(ts), layers.download(8)
+
+    tp, root_plain = timed(lambda: ctx.lde_commit_per_column(cols, logs, b, tw, out_cols, layers))
+    ctx.fft_tune_clear()
+    plan = ctx.batch_create(cols, logs, b, tw, out_cols, layers, sh.TREE_MIXED, sh.HASH_BLAKE2S, True,
+                            drop_coeffs=True)
+    tb, root_batched = timed(plan.run)
     plan.destroy()
+    t0 = time.perf_counter()
+    tuned = ctx.fft_autotune(logs, b, tw)
+    tune_s = time.perf_counter() - t0
+    plan = ctx.batch_create(cols, logs, b, tw, out_cols, layers, sh.TREE_MIXED, sh.HASH_BLAKE2S, True,
+                            drop_coeffs=True)
+    tt, root_tuned = timed(plan.run)
+    plan.destroy()
+    ctx.fft_tune_clear()
     for x in (pristine, work, outs, layers):
         x.free()
     tw.destroy()
     hist = {str(l): logs.count(l) for l in sorted(set(logs))}
-    tp, tb = min(plain), min(batched)
+    changed = [{"n": t[0], "N": t[1], "inverse": t[2], "t0": t[3], "smax": t[4], "wg": t[5], "best_us": t[6],
+                "default_us": t[7]} for t in tuned if (t[3], t[4], t[5]) != (12, 8, 0)]
+    best = min(tb, tt)
     return {
         "workload": "1,024 M31 columns, log_size 10..22 (P(10+j) ~ 0.7^j, seed 3), LDE blowup 2, one "
                     "mixed-degree Blake2s tree",
         "input_elems": elems,
         "log_size_histogram": hist,
         "plain_ms": round(tp * 1e3, 3),
-        "batched_ms": round(tb * 1e3, 3),
+        "plain_form": "C loop: per column stwo_hip_interpolate + stwo_hip_evaluate, then stwo_hip_merkle_commit",
+        "batched_ms": round(best * 1e3, 3),
+        "batched_default_ms": round(tb * 1e3, 3),
+        "batched_tuned_ms": round(tt * 1e3, 3),
+        "autotune_s": round(tune_s, 2),
+        "autotune_shapes": len(tuned),
+        "autotune_changed": changed,
         "plain_melem_s": round(elems / tp / 1e6, 1),
-        "batched_melem_s": round(elems / tb / 1e6, 1),
-        "batched_over_plain": round(tp / tb, 2),
-        "roots_equal": bool(np.array_equal(root_plain, root_batched)),
+        "batched_melem_s": round(elems / best / 1e6, 1),
+        "batched_over_plain": round(tp / best, 2),
+        "roots_equal": bool(np.array_equal(root_plain, root_batched) and np.array_equal(root_plain, root_tuned)),
     }
 
 
-def serving(rates=(10.0, 30.0, 60.0), n_jobs=30, budget=96 << 30, n_slots=4):
+def serving(rates=(10.0, 30.0, 60.0), n_jobs=200, budget=96 << 30, n_slots=4):
     """Poisson streams of commit + FRI jobs (8-512 columns of log 12-22, blowup 2 or 4,
-    seeded values) through the continuous-batching server; p50/p99 latency and jobs/s
-    per arrival rate.  Three jobs are re-run alone on a 1-slot server and must give
-    identical results; the two smallest are checked against the oracle."""
+    seeded values) through the continuous-batching server (jobs admitted together
+    share one batched plan); p50/p99 latency, jobs/s and the mean group size per
+    arrival rate.  Three jobs are re-run alone on a 1-slot server without
+    co-batching and must give identical results; the two smallest are checked
+    against the oracle."""
     import threading
     rng = np.random.default_rng(5)
     out = {"workload": "Poisson arrivals; per job 8-512 M31 columns of log_size 12-22, blowup 2 or 4, "
@@ -197,14 +212,16 @@ def serving(rates=(10.0, 30.0, 60.0), n_jobs=30, budget=96 << 30, n_slots=4):
         out["rates"].append({
             "arrival_rate_jobs_s": rate, "jobs": n_jobs, "jobs_s": round(n_jobs / (span_ms / 1e3), 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "mean_group_size": round(float(np.mean([r.group_size for r in res])), 2),
             "mean_queue_ms": round(float(np.mean([r.t_admit_ms - r.t_submit_ms for r in res])), 2),
             "input_melem_s": round(elems / (span_ms / 1e3) / 1e6, 1)})
+        log("serving: rate %.0f jobs/s done: %s" % (rate, json.dumps(out["rates"][-1])))
         all_jobs += list(zip(jobs, res))
     srv.close()
     # alone vs mix, and the oracle on the two smallest jobs
     picks = sorted(all_jobs, key=lambda jr: jr[0][0] << jr[0][1])[:3]
     alone_ok = True
-    with sh.Server(0, budget, 24, 1) as s1:
+    with sh.Server(0, budget, 24, 1, max_group=1) as s1:
         for (job, r) in picks:
             n, lg_, b_, seed, f = job
             a = s1.wait(s1.submit(n, lg_, b_, seed, sh.FriConfig(*f)))
@@ -262,29 +279,43 @@ def prove_wide_fib(ctx, runs=3, cpu=True):
     ctx.fill_random(b, n, 4, 1)
     cfg = sh.PcsConfig.default()
     tw = ctx.precompute_twiddles(sh.half_odds(PROVE_LOG_N + cfg.fri.log_blowup_factor - 1))
-    ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg)
-    times, phases, data = [], [], None
-    for _ in range(runs):
-        ctx.sync()
-        t0 = time.perf_counter()
-        data, ph, _, _, _ = ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg)
-        times.append((time.perf_counter() - t0) * 1e3)
-        phases.append(ph)
-    order = sorted(range(runs), key=lambda i: times[i])
-    mid = order[runs // 2]
-    ctx.prof_enable(True)
-    ctx.prof_reset()
-    ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg)
-    prof = ctx.prof_read()
-    ctx.prof_enable(False)
+    def timed(store):
+        ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg, store_coeffs=store)  # warm-up, plans built
+        times, phases, data = [], [], None
+        for _ in range(runs):
+            ctx.sync()
+            t0 = time.perf_counter()
+            data, ph, _, _, _ = ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg, store_coeffs=store)
+            times.append((time.perf_counter() - t0) * 1e3)
+            phases.append(ph)
+        mid = sorted(range(runs), key=lambda i: times[i])[runs // 2]
+        ctx.prof_enable(True)
+        ctx.prof_reset()
+        ctx.wide_fib_prove(a, b, PROVE_COLS, PROVE_LOG_N, tw, cfg, store_coeffs=store)
+        prof = ctx.prof_read()
+        ctx.prof_enable(False)
+        return times, phases[mid], data, prof, times[mid]
+
+    times, ph, data, prof, ms = timed(True)
+    times2, ph2, data2, prof2, ms2 = timed(False)
     res = {
         "workload": "wide-Fibonacci prove: 256 columns x 2^18 rows (first two seeded random), PcsConfig::default() "
-                    "(log blowup 1, 3 queries, 10 PoW bits, fold step 1), Blake2s channel, bincode proof",
-        "ms": round(times[mid], 3),
+                    "(log blowup 1, 3 queries, 10 PoW bits, fold step 1), Blake2s channel, bincode proof; "
+                    "coefficients stored (set_store_polynomials_coefficients, as the reference's benchmark)",
+        "ms": round(ms, 3),
         "runs_ms": [round(t, 3) for t in times],
-        "phases_ms": {k: round(v, 3) for k, v in phases[mid].items()},
+        "phases_ms": {k: round(v, 3) for k, v in ph.items()},
         "proof_bytes": len(data),
         "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+        "default_pcs": {
+            "path": "store_polynomials_coefficients = false: LDE without the coefficient store, OODS samples "
+                    "from the evaluations by barycentric weights",
+            "ms": round(ms2, 3),
+            "runs_ms": [round(t, 3) for t in times2],
+            "phases_ms": {k: round(v, 3) for k, v in ph2.items()},
+            "kernels_ms": {k: round(v[1], 3) for k, v in sorted(prof2.items(), key=lambda kv: -kv[1][1])},
+            "proof_equal_to_stored_path": data2 == data,
+        },
     }
     a.free()
     b.free()

@@ -410,8 +410,20 @@ int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *co
  * Returns STWO_HIP_ECONSTRAINTS when prove_ex's own OODS self-check fails (a
  * trace that does not satisfy the AIR).  Synchronizes.  The prove's device
  * buffers (4 + 4 * 2^log_blowup_factor bytes per trace cell, plus the trees
- * and the composition and FRI columns) are kept by the context and
- * reused by its next prove; stwo_hip_ctx_destroy releases them.
+ * and the composition and FRI columns) and its launch plans (the trace and
+ * composition commits as hipGraphs) are kept by the context and reused by
+ * its next prove of the same shape; stwo_hip_ctx_destroy releases them.
+ *
+ * stwo_hip_wide_fib_prove keeps the coefficients
+ * (set_store_polynomials_coefficients, as the reference's wide-Fibonacci
+ * benchmark does, MB/benchmark.rs:576) and samples them with eval_at_point.
+ * stwo_hip_wide_fib_prove_ex with flags 0 is the PCS default
+ * (store_polynomials_coefficients = false, STWO/prover/pcs/mod.rs:347): the
+ * trace LDE does not store coefficients and the OODS samples come from the
+ * committed evaluations with barycentric weights (:540-655); the proof bytes
+ * are the same.  Without STWO_HIP_PROVE_STORE_COEFFS the log blowup must be 1
+ * (a larger one extends the trace from its coefficients; the reference
+ * panics there, constraint-framework/src/prover/component_prover.rs:56-77).
  *
  * stwo_hip_stark_proof_bytes writes the StarkProof<Blake2sMerkleHasher> in the
  * layout bincode 1's default options give the reference's serde derives
@@ -432,6 +444,10 @@ typedef struct stwo_hip_stark_proof stwo_hip_stark_proof;
 int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols, uint32_t log_n,
                             stwo_hip_pcs_config config, const stwo_hip_twiddles *tw, uint8_t channel_digest[32],
                             uint32_t *channel_n_draws, stwo_hip_stark_proof **out);
+#define STWO_HIP_PROVE_STORE_COEFFS 1
+int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols,
+                               uint32_t log_n, stwo_hip_pcs_config config, const stwo_hip_twiddles *tw, int flags,
+                               uint8_t channel_digest[32], uint32_t *channel_n_draws, stwo_hip_stark_proof **out);
 int stwo_hip_stark_proof_bytes(const stwo_hip_stark_proof *proof, uint8_t *buf, size_t cap, size_t *len);
 int stwo_hip_stark_proof_info(const stwo_hip_stark_proof *proof, double phase_ms[STWO_HIP_PROVE_PHASES],
                               uint32_t *query_positions, size_t cap, size_t *n_positions);
@@ -495,29 +511,91 @@ typedef struct stwo_hip_batch stwo_hip_batch;
 int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
                           uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
                           int tree_kind, int hash_mode, int flags, stwo_hip_batch **out);
+/* Several column sets in one plan, each with its own blowup and tree (e.g. the
+ * jobs a server admits together): columns of the same log size and blowup
+ * from every set share each FFT pass launch; each set's tree is built on its
+ * own.  Same rules per set as stwo_hip_batch_create.  stage: LDE (interpolate
+ * in place, evaluate into outs, commit -- stwo_hip_batch_create's work),
+ * IFFT (interpolate in place only: outs, layers and blowup unused) or RFFT
+ * (cols are coefficients: evaluate into outs, then commit if layers is not
+ * NULL).  Every IFFT pass of the plan runs before any RFFT pass, so an RFFT
+ * set may read coefficients an IFFT set of the same plan produces (a
+ * composition polynomial's interpolation, then its halves' evaluations). */
+#define STWO_HIP_BATCH_LDE 0
+#define STWO_HIP_BATCH_IFFT 1
+#define STWO_HIP_BATCH_RFFT 2
+typedef struct {
+  uint32_t *const *cols;
+  const uint32_t *log_sizes;
+  size_t n_cols;
+  uint32_t log_blowup;
+  uint32_t *const *outs;
+  uint32_t *layers;
+  int tree_kind, hash_mode;
+  int stage;
+} stwo_hip_batch_set;
+int stwo_hip_batch_create_multi(stwo_hip_ctx *ctx, const stwo_hip_batch_set *sets, size_t n_sets,
+                                const stwo_hip_twiddles *tw, int flags, stwo_hip_batch **out);
 int stwo_hip_batch_run(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
+/* The plain form the batched plan replaces: the same work dispatched column by
+ * column, as the reference's evaluate_polynomials and interpolate_columns submit
+ * one transform per column (MB/poly.rs:715-779, 364-391) -- interpolate then
+ * evaluate each column (its own launches, tables and argument checks), then one
+ * commit of the evaluations (mixed or lifted).  Same results as a batch plan. */
+int stwo_hip_lde_commit_per_column(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes,
+                                   size_t n_cols, uint32_t log_blowup, const stwo_hip_twiddles *tw,
+                                   uint32_t *const *outs, uint32_t *layers, int tree_kind, int hash_mode);
 int stwo_hip_batch_destroy(stwo_hip_ctx *ctx, stwo_hip_batch *batch);
 
-/* ---- serving: continuous batching of commit + FRI jobs --------------------
- * No counterpart in the reference (its prover handles one proof at a time,
- * MB/prove_runtime_v1.rs); this is the north star's serving mode.  A server
- * owns `n_slots` contexts (streams), a resident twiddle tree for domains up
- * to 2^max_log_size, and an HBM budget allocated once and sub-allocated per
- * job.  Submitted jobs are admitted in arrival order when a slot is idle and
- * the job fits; admitted jobs run concurrently.  A job: n_cols synthetic
- * columns of 2^log_size (column c = stwo_hip_fill_random(seed, c)), LDE at
- * log_blowup, lifted Blake2s commit (trace_root), then FriProver::commit on
- * the QM31 column formed by the first four evaluations, with a channel that
- * starts as H(0^32 || trace_root).  Times are milliseconds since the server
- * was created. */
+/* ---- serving: continuous batching of commit + FRI and prove jobs ----------
+ * No counterpart in the reference (its prover handles one proof at a time;
+ * the request loop follows its caller's per-job body, MB/prove_runtime_v1.rs:
+ * 1181-1240); this is the north star's serving mode.  A server owns `n_slots`
+ * contexts (streams), a resident twiddle tree for domains up to
+ * 2^max_log_size, and an HBM budget allocated once and sub-allocated per
+ * admitted group of jobs.
+ *
+ * Admission: whenever a slot is idle, the queue's head job and the commit
+ * jobs queued right behind it (up to max_group, default 16, in arrival
+ * order) are admitted together if their memory fits the budget; a head that
+ * does not fit waits for memory to be released (later jobs do not overtake
+ * it).  A group's commit jobs share one batched plan: their columns' FFT
+ * passes of equal size and blowup run in the same launches, each job's tree
+ * on its own; then each job's FRI.  A prove job runs alone in its group.
+ * Groups on different slots run concurrently on the device.  Every device
+ * buffer a job needs -- its columns, evaluations, trees, the plan's tables,
+ * FRI and prove scratch -- comes from its group's block of the budget, so
+ * running jobs never allocate device memory and never exceed the budget.
+ *
+ * STWO_HIP_JOB_COMMIT_FRI: n_cols columns of 2^log_size evaluations (the
+ * caller's device columns `cols`, or host data `host` -- column after
+ * column -- copied into the job's memory when it starts, or else column c =
+ * stwo_hip_fill_random(seed, c)), LDE at log_blowup, lifted Blake2s commit
+ * (trace_root), then FriProver::commit (fold steps <= 4) on the QM31 column
+ * formed by the first four evaluations, with a channel that starts as
+ * H(0^32 || trace_root).
+ * STWO_HIP_JOB_PROVE: stwo_hip_wide_fib_prove of n_cols columns of
+ * 2^log_size rows with `pcs`, fresh channel; the two first columns a, b from
+ * cols[0], cols[1] (device), host (2 * 2^log_size words) or the seed; the
+ * proof bytes go to proof_out (proof_cap bytes; proof_len is set, or
+ * STWO_HIP_EINVAL if it does not fit).
+ * Caller data (cols, host, proof_out) must stay valid until the job is done.
+ * Times are milliseconds since the server was created. */
+#define STWO_HIP_JOB_COMMIT_FRI 0
+#define STWO_HIP_JOB_PROVE 1
 typedef struct stwo_hip_server stwo_hip_server;
 typedef struct {
-  uint32_t n_cols;     /* >= 4 */
+  uint32_t kind;       /* STWO_HIP_JOB_COMMIT_FRI or STWO_HIP_JOB_PROVE */
+  uint32_t n_cols;     /* commit: >= 4; prove: 3..4096 */
   uint32_t log_size;   /* trace log size, >= 4 */
-  uint32_t log_blowup; /* >= 1, log_size + log_blowup <= max_log_size */
-  uint32_t pad;
+  uint32_t log_blowup; /* commit: >= 1, log_size + log_blowup <= max_log_size; prove: pcs.fri.log_blowup_factor */
   uint64_t seed;
-  stwo_hip_fri_config fri; /* fri.log_blowup_factor == log_blowup, log_last_layer_degree_bound <= 4 */
+  const uint32_t *const *cols; /* optional device columns (see above) */
+  const uint32_t *host;        /* optional host data (see above) */
+  stwo_hip_fri_config fri;     /* commit: fri.log_blowup_factor == log_blowup, log_last_layer_degree_bound <= 4 */
+  stwo_hip_pcs_config pcs;     /* prove */
+  uint8_t *proof_out;          /* prove */
+  size_t proof_cap;
 } stwo_hip_job_desc;
 typedef struct {
   int status; /* STWO_HIP_OK, or the error of the job */
@@ -526,10 +604,20 @@ typedef struct {
   uint32_t fri_roots[8 * 32];
   uint32_t last_layer_poly[4 * 16];
   uint8_t final_digest[32];
+  size_t proof_len;    /* prove jobs */
+  uint32_t group_size; /* jobs admitted and run together with this one (itself included) */
   double t_submit_ms, t_admit_ms, t_start_ms, t_done_ms;
 } stwo_hip_job_result;
 int stwo_hip_server_create(int device, size_t hbm_budget_bytes, uint32_t max_log_size, uint32_t n_slots,
                            stwo_hip_server **out);
+/* Most jobs admitted as one group (1 = no co-batching; 1..64; default 16). */
+int stwo_hip_server_set_max_group(stwo_hip_server *server, uint32_t max_group);
+/* hold != 0: admit no job until called again with 0 (a caller's burst of
+ * submissions is then grouped as one queue; destroying the server releases
+ * the hold). */
+int stwo_hip_server_hold(stwo_hip_server *server, int hold);
+/* Bytes of the budget a job takes while it runs (its share of a group). */
+int stwo_hip_server_job_bytes(stwo_hip_server *server, const stwo_hip_job_desc *desc, size_t *bytes);
 /* Non-blocking.  STWO_HIP_EBUSY if the queue is full, STWO_HIP_ENOMEM if the
  * job can never fit the budget, STWO_HIP_EINVAL for a bad description. */
 int stwo_hip_server_submit(stwo_hip_server *server, const stwo_hip_job_desc *desc, uint64_t *job_id);

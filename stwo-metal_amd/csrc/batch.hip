@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <functional>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -28,6 +29,7 @@
 struct stwo_hip_batch {
   int device = 0;
   void *d_mem = nullptr;
+  bool mem_from_job = false;  // tables carved from a serving job's memory block
   std::function<int(stwo_hip_ctx *)> issue;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -57,105 +59,140 @@ struct LevelLaunch {
   uint32_t n_groups, total_wgs, wg_log;
 };
 
-}  // namespace
-
-}  // namespace stwo_hip
-
-using namespace stwo_hip;
-
-extern "C" {
-
-int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
-                          uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
-                          int tree_kind, int hash_mode, int flags, stwo_hip_batch **out) {
-  const int use_graph = flags & STWO_HIP_BATCH_GRAPH;
-  if (!ctx || !out) return STWO_HIP_EINVAL;
-  *out = nullptr;
-  STWO_HIP_CHECK_ARG(ctx, n_cols > 0 && cols && log_sizes && outs && layers, "batch_create: NULL or empty arguments");
-  STWO_HIP_CHECK_ARG(ctx, tw && tw->device == ctx->device, "batch_create: twiddles missing or from another device");
-  STWO_HIP_CHECK_ARG(ctx, tree_kind == STWO_HIP_TREE_MIXED || tree_kind == STWO_HIP_TREE_LIFTED,
-                     "batch_create: unknown tree kind %d", tree_kind);
-  STWO_HIP_CHECK_ARG(ctx, hash_mode == STWO_HIP_HASH_BLAKE2S || hash_mode == STWO_HIP_HASH_BLAKE2S_M31,
-                     "batch_create: unknown hash mode %d", hash_mode);
-  uint32_t max_log = 0;
-  for (size_t i = 0; i < n_cols; i++) {
-    STWO_HIP_CHECK_ARG(ctx, cols[i] && outs[i], "batch_create: NULL column %zu", i);
-    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 4 && log_sizes[i] <= 28 && log_blowup <= 24 && log_sizes[i] + log_blowup <= 28,
-                       "batch_create: column %zu log size %u outside [4, 28 - blowup]", i, log_sizes[i]);
-    max_log = std::max(max_log, log_sizes[i] + log_blowup);
-  }
-  // Every CanonicCoset(k).circle_domain(), k <= max_log, must be a doubling of the root.
-  uint32_t rl = tw->root.log_size;
-  STWO_HIP_CHECK_ARG(ctx, max_log - 1 <= rl && tw->root.initial_index == half_odds_initial(rl),
-                     "batch_create: twiddle root must be half_odds(r) with r >= %u", max_log - 1);
-  // The radix-16 passes take the doubled trees.
-  const uint32_t *itw_end = tw->d_itw2 + tw->len, *tw_end = tw->d_tw2 + tw->len;
-
-  // Groups by log size, largest first (their workgroups start first).
-  std::map<uint32_t, std::vector<size_t>, std::greater<uint32_t>> groups;
-  for (size_t i = 0; i < n_cols; i++) groups[log_sizes[i]].push_back(i);
-
-  Staging st;
-  struct GroupTabs {
-    uint32_t n;
-    size_t ifft_off, rfft_off, inplace_off;
-    uint32_t count;
-    bool aligned16;  // every column pointer of the group is 16-byte aligned
-  };
-  std::vector<GroupTabs> gt;
-  for (auto &g : groups) {
-    std::vector<FftCol> a, b, c;
-    for (size_t i : g.second) {
-      a.push_back({cols[i], cols[i]});
-      b.push_back({cols[i], outs[i]});
-      c.push_back({outs[i], outs[i]});
-    }
-    GroupTabs t;
-    t.aligned16 = true;
-    for (size_t i : g.second)
-      if ((((uintptr_t)cols[i]) | ((uintptr_t)outs[i])) & 15) t.aligned16 = false;
-    t.n = g.first;
-    t.count = (uint32_t)g.second.size();
-    t.ifft_off = st.put(a.data(), a.size() * sizeof(FftCol));
-    t.rfft_off = st.put(b.data(), b.size() * sizeof(FftCol));
-    t.inplace_off = st.put(c.data(), c.size() * sizeof(FftCol));
-    gt.push_back(t);
-  }
-  // Merkle column tables.
+// One tree's Merkle launches: its layer tables and shape.
+struct TreeLaunches {
   struct LayerTab {
     uint32_t log_size;
     size_t off;
     uint32_t count;
   };
   std::vector<LayerTab> lt;
-  if (tree_kind == STWO_HIP_TREE_MIXED) {
-    for (int l = (int)max_log; l >= 0; l--) {
-      std::vector<ColRef> refs;
-      for (size_t i = 0; i < n_cols; i++)
-        if (log_sizes[i] + log_blowup == (uint32_t)l) refs.push_back({outs[i], 0, 0});
-      if (!refs.empty()) lt.push_back({(uint32_t)l, st.put(refs.data(), refs.size() * sizeof(ColRef)),
-                                       (uint32_t)refs.size()});
+  uint32_t *layers;
+  uint32_t max_log;
+  int mode, kind;
+};
+
+}  // namespace
+
+int batch_create_trees(stwo_hip_ctx *ctx, const BatchTree *trees, size_t n_trees, const stwo_hip_twiddles *tw,
+                       int flags, stwo_hip_batch **out) {
+  const int use_graph = flags & STWO_HIP_BATCH_GRAPH;
+  if (!ctx || !out) return STWO_HIP_EINVAL;
+  *out = nullptr;
+  STWO_HIP_CHECK_ARG(ctx, n_trees > 0 && trees, "batch_create: no column sets");
+  STWO_HIP_CHECK_ARG(ctx, tw && tw->device == ctx->device, "batch_create: twiddles missing or from another device");
+  uint32_t all_max = 0;
+  for (size_t t = 0; t < n_trees; t++) {
+    const BatchTree &T = trees[t];
+    STWO_HIP_CHECK_ARG(ctx, T.stage == BATCH_LDE || T.stage == BATCH_IFFT || T.stage == BATCH_RFFT,
+                       "batch_create: unknown stage %d (set %zu)", T.stage, t);
+    const bool has_outs = T.stage != BATCH_IFFT, has_tree = T.stage == BATCH_LDE || T.layers;
+    STWO_HIP_CHECK_ARG(ctx, T.n_cols > 0 && T.cols && T.log_sizes && (T.outs || !has_outs) &&
+                                (T.layers || T.stage != BATCH_LDE),
+                       "batch_create: NULL or empty arguments (set %zu)", t);
+    if (has_tree) {
+      STWO_HIP_CHECK_ARG(ctx, T.tree_kind == STWO_HIP_TREE_MIXED || T.tree_kind == STWO_HIP_TREE_LIFTED,
+                         "batch_create: unknown tree kind %d", T.tree_kind);
+      STWO_HIP_CHECK_ARG(ctx, T.hash_mode == STWO_HIP_HASH_BLAKE2S || T.hash_mode == STWO_HIP_HASH_BLAKE2S_M31,
+                         "batch_create: unknown hash mode %d", T.hash_mode);
     }
-  } else {
-    std::vector<size_t> order(n_cols);
-    for (size_t i = 0; i < n_cols; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return log_sizes[a] < log_sizes[b]; });
-    std::vector<ColRef> refs;
-    for (size_t i : order) refs.push_back({outs[i], max_log - (log_sizes[i] + log_blowup), 0});
-    lt.push_back({max_log, st.put(refs.data(), refs.size() * sizeof(ColRef)), (uint32_t)refs.size()});
+    const uint32_t b = T.stage == BATCH_IFFT ? 0 : T.log_blowup;
+    for (size_t i = 0; i < T.n_cols; i++) {
+      STWO_HIP_CHECK_ARG(ctx, T.cols[i] && (!has_outs || T.outs[i]), "batch_create: NULL column %zu", i);
+      STWO_HIP_CHECK_ARG(ctx, T.log_sizes[i] >= 4 && T.log_sizes[i] <= 28 && b <= 24 && T.log_sizes[i] + b <= 28,
+                         "batch_create: column %zu log size %u outside [4, 28 - blowup]", i, T.log_sizes[i]);
+      all_max = std::max(all_max, T.log_sizes[i] + b);
+    }
+  }
+  // Every CanonicCoset(k).circle_domain(), k <= max_log, must be a doubling of the root.
+  uint32_t rl = tw->root.log_size;
+  STWO_HIP_CHECK_ARG(ctx, all_max - 1 <= rl && tw->root.initial_index == half_odds_initial(rl),
+                     "batch_create: twiddle root must be half_odds(r) with r >= %u", all_max - 1);
+  // The radix-16 passes take the doubled trees.
+  const uint32_t *itw_end = tw->d_itw2 + tw->len, *tw_end = tw->d_tw2 + tw->len;
+
+  // Groups by (log size, blowup, stage) over every set, largest first (their
+  // workgroups start first): columns of different sets share launches.
+  std::map<std::tuple<uint32_t, uint32_t, int>, std::vector<std::pair<size_t, size_t>>,
+           std::greater<std::tuple<uint32_t, uint32_t, int>>> groups;
+  for (size_t t = 0; t < n_trees; t++)
+    for (size_t i = 0; i < trees[t].n_cols; i++)
+      groups[{trees[t].log_sizes[i], trees[t].stage == BATCH_IFFT ? 0u : trees[t].log_blowup, trees[t].stage}]
+          .push_back({t, i});
+
+  Staging st;
+  struct GroupTabs {
+    uint32_t n, b;
+    int stage;
+    size_t ifft_off, rfft_off, inplace_off;
+    uint32_t count;
+    bool aligned16;  // every column pointer of the group is 16-byte aligned
+  };
+  std::vector<GroupTabs> gt;
+  for (auto &g : groups) {
+    std::vector<FftCol> a, bb, c;
+    GroupTabs t;
+    t.aligned16 = true;
+    t.stage = std::get<2>(g.first);
+    for (auto &ti : g.second) {
+      uint32_t *col = trees[ti.first].cols[ti.second];
+      uint32_t *o = t.stage == BATCH_IFFT ? col : trees[ti.first].outs[ti.second];
+      a.push_back({col, col});
+      bb.push_back({col, o});
+      c.push_back({o, o});
+      if ((((uintptr_t)col) | ((uintptr_t)o)) & 15) t.aligned16 = false;
+    }
+    t.n = std::get<0>(g.first);
+    t.b = std::get<1>(g.first);
+    t.count = (uint32_t)g.second.size();
+    t.ifft_off = st.put(a.data(), a.size() * sizeof(FftCol));
+    t.rfft_off = st.put(bb.data(), bb.size() * sizeof(FftCol));
+    t.inplace_off = st.put(c.data(), c.size() * sizeof(FftCol));
+    gt.push_back(t);
+  }
+  // Merkle column tables, per set.
+  std::vector<TreeLaunches> tl;
+  for (size_t t = 0; t < n_trees; t++) {
+    const BatchTree &T = trees[t];
+    if (T.stage == BATCH_IFFT || !T.layers) continue;
+    tl.emplace_back();
+    TreeLaunches &L = tl.back();
+    L.layers = T.layers;
+    L.mode = T.hash_mode;
+    L.kind = T.tree_kind;
+    L.max_log = 0;
+    for (size_t i = 0; i < T.n_cols; i++) L.max_log = std::max(L.max_log, T.log_sizes[i] + T.log_blowup);
+    if (T.tree_kind == STWO_HIP_TREE_MIXED) {
+      for (int l = (int)L.max_log; l >= 0; l--) {
+        std::vector<ColRef> refs;
+        for (size_t i = 0; i < T.n_cols; i++)
+          if (T.log_sizes[i] + T.log_blowup == (uint32_t)l) refs.push_back({T.outs[i], 0, 0});
+        if (!refs.empty())
+          L.lt.push_back({(uint32_t)l, st.put(refs.data(), refs.size() * sizeof(ColRef)), (uint32_t)refs.size()});
+      }
+    } else {
+      std::vector<size_t> order(T.n_cols);
+      for (size_t i = 0; i < T.n_cols; i++) order[i] = i;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](size_t a, size_t b) { return T.log_sizes[a] < T.log_sizes[b]; });
+      std::vector<ColRef> refs;
+      for (size_t i : order) refs.push_back({T.outs[i], L.max_log - (T.log_sizes[i] + T.log_blowup), 0});
+      L.lt.push_back({L.max_log, st.put(refs.data(), refs.size() * sizeof(ColRef)), (uint32_t)refs.size()});
+    }
   }
   // Reserve the pass descriptor area after the tables: the descriptors hold
   // device addresses of the tables, known once the buffer exists.
   size_t tables_bytes = (st.host.size() + 255) & ~(size_t)255;
   void *dmem = nullptr;
+  bool from_job = false;
   // Upper bound on descriptors: one per group and pass (2 directions, at most
-  // 4 passes), plus the 256-byte alignment of two tables per launch (2
-  // directions x 4 levels x 2 workgroup sizes x at most 14 schedule kinds).
+  // 4 passes), plus the 256-byte alignment of two or three tables per launch
+  // (2 directions x 4 levels x 2 workgroup sizes x at most 14 schedule kinds,
+  // and the fused launches).
   size_t desc_bytes = 2 * 4 * gt.size() * (sizeof(FftPass2) + 8) + 2 * 4 * 2 * 14 * 2 * 256 + 8192;
-  if (hipMalloc(&dmem, tables_bytes + desc_bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    return set_error(ctx, STWO_HIP_ENOMEM, "batch_create: out of device memory for tables");
-  }
+  int arc = device_alloc(ctx, tables_bytes + desc_bytes, &dmem, &from_job);
+  if (arc) return arc;
+  auto release = [&]() { device_free(dmem, from_job); };
   uint8_t *dbase = (uint8_t *)dmem;
   // Both transforms' passes of every group; where the IFFT's last pass and
   // the RFFT's first pass tile alike they become one fused pass.
@@ -168,17 +205,18 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     const GroupTabs &t = gt[gi];
     for (int dir = 0; dir < 2; dir++) {
       const bool inverse = dir == 0;
-      const uint32_t n = t.n, N = t.n + (inverse ? 0 : log_blowup);
+      if ((inverse && t.stage == BATCH_RFFT) || (!inverse && t.stage == BATCH_IFFT)) continue;
+      const uint32_t n = t.n, N = t.n + (inverse ? 0 : t.b);
       const FftCol *first = (const FftCol *)(dbase + (inverse ? t.ifft_off : t.rfft_off));
       const FftCol *inpl = (const FftCol *)(dbase + t.inplace_off);
       int rc = make_group_passes(ctx, inverse ? N : n, N, inverse, first, inpl, t.count, inverse ? itw_end : tw_end,
                                  t.aligned16, inverse ? gp[gi].inv : gp[gi].fwd);
       if (rc) {
-        (void)hipFree(dmem);
+        release();
         return rc;
       }
     }
-    gp[gi].fused = lde_fusable(gp[gi].inv.back(), gp[gi].fwd.front(), log_blowup);
+    gp[gi].fused = t.stage == BATCH_LDE && lde_fusable(gp[gi].inv.back(), gp[gi].fwd.front(), t.b);
   }
   std::vector<LevelLaunch> levels;
   auto add_level = [&](bool inverse, uint32_t wg_log, std::vector<PassLaunch> &v) -> int {
@@ -243,7 +281,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
           acc += gp[gi].inv.back().wgs;
         }
         if (acc >= (1ull << 31)) {
-          (void)hipFree(dmem);
+          release();
           return set_error(ctx, STWO_HIP_EINVAL, "batch_create: grid too large");
         }
         LevelLaunch ll{};
@@ -265,31 +303,36 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     for (const auto &key : order) {
       int rc = add_level(inverse, key.second, by_level[key]);
       if (rc) {
-        (void)hipFree(dmem);
+        release();
         return rc;
       }
     }
   }
   if (st.host.size() > tables_bytes + desc_bytes) {
-    (void)hipFree(dmem);
+    release();
     return set_error(ctx, STWO_HIP_EINVAL, "internal: batch descriptor area overflow");
   }
-  if (hipMemcpy(dmem, st.host.data(), st.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(dmem);
+  if (from_job) {
+    // within a job: an ordered copy on the job's stream
+    int rc = flush_uploads(ctx);
+    if (rc) return rc;
+    STWO_HIP_TRY(ctx, hipMemcpyAsync(dmem, st.host.data(), st.host.size(), hipMemcpyHostToDevice, ctx->stream));
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  } else if (hipMemcpy(dmem, st.host.data(), st.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    release();
     return set_error(ctx, STWO_HIP_EDEVICE, "batch_create: table upload failed");
   }
 
   stwo_hip_batch *b = new (std::nothrow) stwo_hip_batch();
   if (!b) {
-    (void)hipFree(dmem);
+    release();
     return set_error(ctx, STWO_HIP_ENOMEM, "host allocation failed");
   }
   b->device = ctx->device;
   b->d_mem = dmem;
-  const int mode = hash_mode;
-  const int kind = tree_kind;
+  b->mem_from_job = from_job;
   const uint32_t keep_coeffs = (flags & STWO_HIP_BATCH_DROP_COEFFS) ? 0u : 1u;
-  b->issue = [levels, lt, dbase, layers, max_log, mode, kind, keep_coeffs](stwo_hip_ctx *c) -> int {
+  b->issue = [levels, tl, dbase, keep_coeffs](stwo_hip_ctx *c) -> int {
     for (const LevelLaunch &ll : levels) {
       int rc = ll.fused ? launch_lde_multi(c, (const FftPass2 *)(dbase + ll.groups_off),
                                            (const FftPass2 *)(dbase + ll.fwd_off),
@@ -300,30 +343,36 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
                                            ll.wg_log, ll.kind);
       if (rc) return rc;
     }
-    if (kind == STWO_HIP_TREE_LIFTED) {
-      const LayerTab &t = lt[0];
-      int rc = launch_merkle_layer(c, nullptr, (const ColRef *)(dbase + t.off), t.count, max_log,
-                                   layers + tree_layer_off(max_log), mode, true);
-      if (rc) return rc;
-      return hash_node_layers(c, layers, max_log, 0, mode);
-    }
-    // mixed degree: layers max_log .. 0; runs of pure node layers are fused.
-    int l = (int)max_log;
-    size_t ti = 0;
-    while (l >= 0) {
-      const uint32_t *prev = (l == (int)max_log) ? nullptr : layers + tree_layer_off((uint32_t)l + 1);
-      if (ti < lt.size() && lt[ti].log_size == (uint32_t)l) {
-        const LayerTab &t = lt[ti++];
-        int rc = launch_merkle_layer(c, prev, (const ColRef *)(dbase + t.off), t.count, (uint32_t)l,
-                                     layers + tree_layer_off((uint32_t)l), mode, false);
+    for (const TreeLaunches &L : tl) {
+      uint32_t *layers = L.layers;
+      const uint32_t max_log = L.max_log;
+      if (L.kind == STWO_HIP_TREE_LIFTED) {
+        const auto &t = L.lt[0];
+        int rc = launch_merkle_layer(c, nullptr, (const ColRef *)(dbase + t.off), t.count, max_log,
+                                     layers + tree_layer_off(max_log), L.mode, true);
         if (rc) return rc;
-        l--;
+        rc = hash_node_layers(c, layers, max_log, 0, L.mode);
+        if (rc) return rc;
         continue;
       }
-      int next = ti < lt.size() ? (int)lt[ti].log_size : -1;
-      int rc = hash_node_layers(c, layers, (uint32_t)l + 1, (uint32_t)(next + 1), mode);
-      if (rc) return rc;
-      l = next;
+      // mixed degree: layers max_log .. 0; runs of pure node layers are fused.
+      int l = (int)max_log;
+      size_t ti = 0;
+      while (l >= 0) {
+        const uint32_t *prev = (l == (int)max_log) ? nullptr : layers + tree_layer_off((uint32_t)l + 1);
+        if (ti < L.lt.size() && L.lt[ti].log_size == (uint32_t)l) {
+          const auto &t = L.lt[ti++];
+          int rc = launch_merkle_layer(c, prev, (const ColRef *)(dbase + t.off), t.count, (uint32_t)l,
+                                       layers + tree_layer_off((uint32_t)l), L.mode, false);
+          if (rc) return rc;
+          l--;
+          continue;
+        }
+        int next = ti < L.lt.size() ? (int)L.lt[ti].log_size : -1;
+        int rc = hash_node_layers(c, layers, (uint32_t)l + 1, (uint32_t)(next + 1), L.mode);
+        if (rc) return rc;
+        l = next;
+      }
     }
     return STWO_HIP_OK;
   };
@@ -332,7 +381,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     // Tables of earlier calls still pending are copied outside the graph.
     int frc = flush_uploads(ctx);
     if (frc) {
-      (void)hipFree(dmem);
+      release();
       delete b;
       return frc;
     }
@@ -351,7 +400,7 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
     }
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
-      (void)hipFree(dmem);
+      release();
       delete b;
       (void)hipGetLastError();
       return set_error(ctx, rc, "batch_create: graph capture failed (%s)",
@@ -360,6 +409,58 @@ int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
   }
   *out = b;
   return STWO_HIP_OK;
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_batch_create(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes, size_t n_cols,
+                          uint32_t log_blowup, const stwo_hip_twiddles *tw, uint32_t *const *outs, uint32_t *layers,
+                          int tree_kind, int hash_mode, int flags, stwo_hip_batch **out) {
+  const BatchTree t = {cols, log_sizes, n_cols, log_blowup, outs, layers, tree_kind, hash_mode};
+  return batch_create_trees(ctx, &t, 1, tw, flags, out);
+}
+
+int stwo_hip_batch_create_multi(stwo_hip_ctx *ctx, const stwo_hip_batch_set *sets, size_t n_sets,
+                                const stwo_hip_twiddles *tw, int flags, stwo_hip_batch **out) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, sets && n_sets, "batch_create_multi: no column sets");
+  std::vector<BatchTree> t(n_sets);
+  for (size_t i = 0; i < n_sets; i++)
+    t[i] = {sets[i].cols,   sets[i].log_sizes, sets[i].n_cols,    sets[i].log_blowup, sets[i].outs,
+            sets[i].layers, sets[i].tree_kind, sets[i].hash_mode, sets[i].stage};
+  return batch_create_trees(ctx, t.data(), n_sets, tw, flags, out);
+}
+
+int stwo_hip_lde_commit_per_column(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32_t *log_sizes,
+                                   size_t n_cols, uint32_t log_blowup, const stwo_hip_twiddles *tw,
+                                   uint32_t *const *outs, uint32_t *layers, int tree_kind, int hash_mode) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, n_cols > 0 && cols && log_sizes && outs && layers, "lde_commit_per_column: NULL arguments");
+  STWO_HIP_CHECK_ARG(ctx, tree_kind == STWO_HIP_TREE_MIXED || tree_kind == STWO_HIP_TREE_LIFTED,
+                     "lde_commit_per_column: unknown tree kind %d", tree_kind);
+  uint32_t max_log = 0;
+  std::vector<uint32_t> out_logs(n_cols);
+  for (size_t i = 0; i < n_cols; i++) {
+    STWO_HIP_CHECK_ARG(ctx, log_sizes[i] >= 1 && log_sizes[i] + log_blowup <= 28,
+                       "lde_commit_per_column: column %zu log size %u", i, log_sizes[i]);
+    const uint32_t n = log_sizes[i], N = n + log_blowup;
+    // one interpolate and one evaluate call per column, as the reference's
+    // plain path submits them
+    int rc = stwo_hip_interpolate(ctx, cols[i], {half_odds_initial(n - 1), n - 1}, tw);
+    if (rc) return rc;
+    rc = stwo_hip_evaluate(ctx, cols[i], n, {half_odds_initial(N - 1), N - 1}, tw, outs[i]);
+    if (rc) return rc;
+    out_logs[i] = N;
+    max_log = std::max(max_log, N);
+  }
+  std::vector<const uint32_t *> oc(outs, outs + n_cols);
+  if (tree_kind == STWO_HIP_TREE_MIXED) return stwo_hip_merkle_commit(ctx, oc.data(), out_logs.data(), n_cols, layers,
+                                                                      hash_mode);
+  return stwo_hip_merkle_lifted_commit(ctx, oc.data(), out_logs.data(), n_cols, max_log, layers, hash_mode);
 }
 
 int stwo_hip_batch_run(stwo_hip_ctx *ctx, stwo_hip_batch *b) {
@@ -380,7 +481,7 @@ int stwo_hip_batch_destroy(stwo_hip_ctx *ctx, stwo_hip_batch *b) {
   (void)hipStreamSynchronize(ctx->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->d_mem) (void)hipFree(b->d_mem);
+  device_free(b->d_mem, b->mem_from_job);
   delete b;
   return STWO_HIP_OK;
 }

@@ -87,6 +87,19 @@ struct stwo_hip_ctx {
 
   // FFT planner choices by shape (fft_tune_key); absent shapes use the defaults.
   std::map<uint64_t, stwo_hip::FftTune> fft_tune;
+
+  // Job memory (serving): while set, every device buffer the library would
+  // allocate for its own use (get_scratch, batch-plan tables) is carved from
+  // [job_base, job_base + job_cap) instead, and dropped when the job ends
+  // (job_mem_end), so a job's device use stays inside its budget block.
+  uint8_t *job_base = nullptr;
+  size_t job_cap = 0, job_used = 0;
+  std::vector<std::string> job_keys;  // scratch entries carved from the job block
+
+  // Launch plans of the last prove shape, kept for the next prove of that
+  // shape; prove_plans_free destroys them (shape change, context destruction).
+  void *prove_plans = nullptr;
+  void (*prove_plans_free)(stwo_hip_ctx *, void *) = nullptr;
 };
 
 namespace stwo_hip {
@@ -130,7 +143,17 @@ int eval_point_scratch_setup(stwo_hip_ctx *ctx);
 
 // Returns a device scratch buffer of at least `bytes` for `key` (grown on
 // demand; growing synchronizes the stream before freeing the old buffer).
+// Within a job (job_mem_begin), carved from the job's memory block instead.
 int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr);
+// Device memory owned by the caller of device_alloc (a batch plan's tables):
+// from the job block within a job (device_free then does nothing), else
+// hipMalloc.  *from_job tells which.
+int device_alloc(stwo_hip_ctx *ctx, size_t bytes, void **dptr, bool *from_job);
+void device_free(void *p, bool from_job);
+// Starts / ends a job whose library-internal device buffers come from
+// [base, base + cap).  job_mem_end drops every scratch entry carved from it.
+void job_mem_begin(stwo_hip_ctx *ctx, void *base, size_t cap);
+void job_mem_end(stwo_hip_ctx *ctx);
 // Several device-to-host reads with one stream synchronization (through the
 // pinned staging buffer when they fit).
 struct DownloadReq {

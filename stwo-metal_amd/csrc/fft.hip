@@ -1472,6 +1472,8 @@ int stwo_hip_extend_evals(stwo_hip_ctx *ctx, uint32_t *const *cols, const uint32
 // device -- as many columns as the set has of that size, up to 2^26 words --
 // and timed with HIP events (the fastest of 3 runs after a warm-up); the
 // fastest plan becomes the context's choice for the shape.
+}  // extern "C"
+
 namespace stwo_hip {
 namespace {
 
@@ -1516,6 +1518,8 @@ int time_passes(stwo_hip_ctx *ctx, bool inverse, const std::vector<PassLaunch> &
 
 }  // namespace
 }  // namespace stwo_hip
+
+extern "C" {
 
 int stwo_hip_fft_autotune(stwo_hip_ctx *ctx, const uint32_t *log_sizes, size_t n_cols, uint32_t log_blowup,
                           const stwo_hip_twiddles *tw, uint32_t *report, size_t max_report, size_t *n_report) {

@@ -74,6 +74,39 @@ int lde_kind(const PassLaunch &last_inv, const PassLaunch &first_fwd);
 int launch_lde_multi(stwo_hip_ctx *ctx, const FftPass2 *d_pis, const FftPass2 *d_pfs, const uint32_t *d_wg_start,
                      uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind, uint32_t store_coeffs);
 
+// Upper bound on the device bytes stwo_hip_wide_fib_prove takes for itself
+// (its buffers, the FRI prover's memory, OODS scratch, a margin for tables).
+size_t wide_fib_prove_device_bytes(uint32_t n_cols, uint32_t log_n, const stwo_hip_pcs_config &config);
+
+// One column set of a batched plan (stwo_hip_batch_create's arguments).
+// A column set of a batch plan.  stage: what the plan does to its columns --
+// BATCH_LDE interpolates them in place and evaluates them at the blowup into
+// outs; BATCH_IFFT only interpolates them in place (no outs, no tree);
+// BATCH_RFFT takes them as coefficients and evaluates them into outs.  Every
+// set's IFFT passes run before any set's RFFT pass, so a BATCH_RFFT set may
+// read what a BATCH_IFFT set of the same plan writes.  layers: the set's
+// tree (required for BATCH_LDE, optional for BATCH_RFFT).
+enum BatchStage { BATCH_LDE = 0, BATCH_IFFT = 1, BATCH_RFFT = 2 };
+struct BatchTree {
+  uint32_t *const *cols;
+  const uint32_t *log_sizes;
+  size_t n_cols;
+  uint32_t log_blowup;
+  uint32_t *const *outs;
+  uint32_t *layers;
+  int tree_kind, hash_mode;
+  int stage = BATCH_LDE;
+};
+int batch_create_trees(stwo_hip_ctx *ctx, const BatchTree *trees, size_t n_trees, const stwo_hip_twiddles *tw,
+                       int flags, stwo_hip_batch **out);
+
+// stwo_hip_wide_fib_constraint_quotients, writing the quotients over accum
+// instead of adding them when !accumulate (a single component needs no zeroed
+// accumulator).
+int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
+                                       uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4],
+                                       bool accumulate);
+
 // ---- decommitment: batched device gather -----------------------------------
 
 // Collects (device source, word count) reads; run() gathers them all with

@@ -160,6 +160,77 @@ __global__ void __launch_bounds__(256) merkle_layer_quad_kernel(const uint32_t *
   o[4 + j] = hi;
 }
 
+// Small mixed-degree layers with many columns (up to 2^14 nodes): the chain
+// of blocks per node is the critical path and, with one block loaded ahead,
+// each block waits on a table load followed by a column load.  Here a
+// workgroup owns 64 consecutive nodes (a quad each) and stages its columns
+// through LDS in chunks of MQ_CHUNK columns: the chunk's column pointers
+// first, then every column's 64 words (one coalesced 256-byte run per wave
+// and load, 32 loads in flight per lane), so a chunk costs about two memory
+// round trips and its blocks then read their words from LDS.  Same inputs
+// and outputs as merkle_layer_kernel (mixed layers only, not lifted).
+constexpr uint32_t MQ_CHUNK = 240;  // 15 blocks; 61,440 B of words + 1,920 B of pointers
+__global__ void __launch_bounds__(256) merkle_layer_quad_lds_kernel(const uint32_t *prev, const ColRef *cols,
+                                                                    uint32_t n_cols, uint32_t log_size,
+                                                                    uint32_t *out, uint32_t m31) {
+  __shared__ const uint32_t *ptrs[MQ_CHUNK];
+  __shared__ uint32_t tile[MQ_CHUNK * 64];
+  const uint32_t n = 1u << log_size;
+  const uint32_t tid = threadIdx.x, node = tid >> 2, j = tid & 3;
+  const uint32_t i0 = blockIdx.x * 64, i = i0 + node;
+  const bool active = i < n;  // every lane stays for the barriers
+  uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+  uint32_t m[16];
+  uint32_t t = 0;
+  if (prev) {
+    if (active) {
+      load_block64(m, prev + 16 * (size_t)i);
+      b2_compress_quad(lo, hi, m, 64, n_cols == 0);
+    }
+    t = 64;
+  } else if (n_cols == 0) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) m[k] = 0;
+    if (active) b2_compress_quad(lo, hi, m, 0, true);
+  }
+  const uint32_t w = tid & 63, row = i0 + w;
+  for (uint32_t cb = 0; cb < n_cols; cb += MQ_CHUNK) {
+    const uint32_t cnt = min(MQ_CHUNK, n_cols - cb), words = cnt * 64;
+    __syncthreads();  // the previous chunk's words are consumed
+    for (uint32_t c = tid; c < cnt; c += 256) ptrs[c] = cols[cb + c].p;
+    __syncthreads();
+    for (uint32_t e0 = 0; e0 < words; e0 += 256 * 32) {
+      uint32_t v[32];
+#pragma unroll
+      for (uint32_t k = 0; k < 32; k++) {
+        const uint32_t e = e0 + tid + 256 * k;  // column e >> 6 is uniform over the wave
+        v[k] = (e < words && row < n) ? ld_col_word(ptrs[e >> 6], row) : 0u;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < 32; k++) {
+        const uint32_t e = e0 + tid + 256 * k;
+        if (e < words) tile[e] = v[k];
+      }
+    }
+    __syncthreads();
+    for (uint32_t b = 0; b < cnt; b += 16) {
+      const uint32_t bc = min(16u, cnt - b);
+#pragma unroll
+      for (uint32_t k = 0; k < 16; k++) m[k] = k < bc ? tile[(b + k) * 64 + node] : 0u;
+      t += 4 * bc;
+      if (active) b2_compress_quad(lo, hi, m, t, cb + b + 16 >= n_cols);
+    }
+  }
+  if (!active) return;
+  if (m31) {
+    lo = b2_reduce_word(lo);
+    hi = b2_reduce_word(hi);
+  }
+  g_u32 *o = as_global(out + 8 * (size_t)i);
+  o[j] = lo;
+  o[4 + j] = hi;
+}
+
 // Hashes `n_layers` (<= 8) consecutive node layers below `prev` (of log size
 // out_log + 1).  Layer k (k = 0 first) has log size out_log - k and is
 // written at out_base + offset(out_log - k) in the packed layout.
@@ -194,8 +265,12 @@ int launch_merkle_layer(stwo_hip_ctx *ctx, const uint32_t *prev, const ColRef *d
   const char *name = lifted ? "merkle_lifted_leaves" : "merkle_commit_on_layer";
   const uint32_t m31 = (uint32_t)(mode == STWO_HIP_HASH_BLAKE2S_M31);
   // Up to 2^14 nodes with more than one block per node, the layer is
-  // latency-bound even at four lanes per node: a quad per node.  Below 2^16
+  // latency-bound even at four lanes per node: a quad per node (mixed layers
+  // with the columns staged through LDS, lifted ones loading a block ahead).  Below 2^16
   // nodes, a lane per node with the loads one block ahead.
+  if (log_size <= 14 && n_cols > 16 && !lifted)
+    return launch(ctx, name, merkle_layer_quad_lds_kernel, dim3((n + 63) / 64), dim3(256), 0, prev, d_cols, n_cols,
+                  log_size, out, m31);
   if (log_size <= 14 && n_cols > 16)
     return launch(ctx, name, merkle_layer_quad_kernel, dim3((4 * n + 255) / 256), dim3(256), 0, prev, d_cols, n_cols,
                   log_size, out, m31, (uint32_t)lifted);

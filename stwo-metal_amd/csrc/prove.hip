@@ -29,9 +29,25 @@
 //      constraint-framework/src/component.rs:271-300; PointEvaluationAccumulator,
 //      STWO/core/air/accumulation.rs).
 //
-// The OODS samples are computed from the coefficients (eval_at_point); the
-// reference's prover without stored coefficients uses barycentric weights on
-// the evaluations, which give the same field element for these polynomials.
+// Two sampling paths, as the PCS has them (STWO/prover/pcs/mod.rs:540-655):
+// with STWO_HIP_PROVE_STORE_COEFFS (set_store_polynomials_coefficients, what
+// the reference's wide-Fibonacci benchmark sets, MB/benchmark.rs:576) the
+// coefficients are kept and sampled with eval_at_point; without it (the PCS
+// default, store_polynomials_coefficients = false, :347) the trace's LDE pass
+// does not store them and every column is sampled from its committed
+// evaluation with the barycentric weights of CanonicCoset(log_n + blowup) at
+// the OODS point (build_weights_hash_map, :455-510; component_prover.rs:50-67).
+// Both give the same field elements, so the proof bytes are the same.  As in
+// the reference, the default path needs the blowup to be 1: a larger blowup
+// extends the trace to the constraint domain from its coefficients
+// (constraint-framework/src/prover/component_prover.rs:56-77, which panics
+// without them; here an error).
+//
+// Outside a serving job the trace commit (trace LDE + lifted tree) and the
+// composition commit (interpolation of the 4 coordinate columns, evaluation of
+// their 8 halves, lifted tree) each run as a batch plan replayed as a
+// hipGraph; the plans are kept on the context for the next prove of the same
+// shape and buffers.  Within a job they are issued eagerly.
 #include <string.h>
 
 #include <array>
@@ -193,6 +209,68 @@ struct ProveMem {
   }
 };
 
+// The trace-commit and composition-commit plans of one prove shape and
+// buffer set (the key), cached on the context.
+struct ProvePlans {
+  const uint32_t *mem;
+  uint32_t n_cols, log_n, B;
+  const uint32_t *tw_d;
+  uint32_t tw_initial, tw_log;
+  bool store;
+  stwo_hip_batch *trace = nullptr, *comp = nullptr;
+  bool same(const ProvePlans &o) const {
+    return mem == o.mem && n_cols == o.n_cols && log_n == o.log_n && B == o.B && tw_d == o.tw_d &&
+           tw_initial == o.tw_initial && tw_log == o.tw_log && store == o.store;
+  }
+};
+
+static void prove_plans_free(stwo_hip_ctx *ctx, void *p) {
+  ProvePlans *pp = (ProvePlans *)p;
+  if (!pp) return;
+  if (pp->trace) (void)stwo_hip_batch_destroy(ctx, pp->trace);
+  if (pp->comp) (void)stwo_hip_batch_destroy(ctx, pp->comp);
+  delete pp;
+}
+
+// The context's plans for `key`, built (and the previous ones destroyed) when
+// the key differs.  trace: LDE of the trace columns into the evaluations and
+// their lifted tree (coefficients dropped unless key.store); comp: the
+// composition columns interpolated in place, then their halves evaluated
+// into the composition evaluations and its lifted tree.
+static int prove_plans_get(stwo_hip_ctx *ctx, const ProvePlans &key, std::vector<uint32_t *> &coef,
+                           std::vector<uint32_t *> &eval, uint32_t *tree1, uint32_t *const comp[4],
+                           std::vector<uint32_t *> &halves, std::vector<uint32_t *> &ceval, uint32_t *tree2,
+                           const stwo_hip_twiddles *tw, ProvePlans **out) {
+  ProvePlans *cur = (ProvePlans *)ctx->prove_plans;
+  if (cur && cur->same(key)) {
+    *out = cur;
+    return STWO_HIP_OK;
+  }
+  if (ctx->prove_plans_free) ctx->prove_plans_free(ctx, ctx->prove_plans);
+  ctx->prove_plans = nullptr;
+  ctx->prove_plans_free = nullptr;
+  std::unique_ptr<ProvePlans> pp(new ProvePlans(key));
+  const uint32_t C = key.log_n + 1;
+  const std::vector<uint32_t> logs_n(key.n_cols, key.log_n), logs_c(4, C), logs_h(8, key.log_n);
+  BatchTree t{coef.data(), logs_n.data(), key.n_cols, key.B, eval.data(), tree1, STWO_HIP_TREE_LIFTED,
+              STWO_HIP_HASH_BLAKE2S, BATCH_LDE};
+  int rc = batch_create_trees(ctx, &t, 1, tw, STWO_HIP_BATCH_GRAPH | (key.store ? 0 : STWO_HIP_BATCH_DROP_COEFFS),
+                              &pp->trace);
+  if (rc) return rc;
+  BatchTree c[2] = {{comp, logs_c.data(), 4, 0, nullptr, nullptr, 0, 0, BATCH_IFFT},
+                    {halves.data(), logs_h.data(), 8, key.B, ceval.data(), tree2, STWO_HIP_TREE_LIFTED,
+                     STWO_HIP_HASH_BLAKE2S, BATCH_RFFT}};
+  rc = batch_create_trees(ctx, c, 2, tw, STWO_HIP_BATCH_GRAPH, &pp->comp);
+  if (rc) {
+    prove_plans_free(ctx, pp.release());
+    return rc;
+  }
+  ctx->prove_plans = pp.get();
+  ctx->prove_plans_free = prove_plans_free;
+  *out = pp.release();
+  return STWO_HIP_OK;
+}
+
 static int fetch_layer(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *fri, uint32_t k, ProofLayer &out) {
   size_t nf = 0, nh = 0, npos = 0;
   int rc = stwo_hip_fri_prover_layer_proof(ctx, fri, k, out.commitment, nullptr, 0, &nf, nullptr, 0, &nh, nullptr,
@@ -235,32 +313,65 @@ static int decommit_trees(stwo_hip_ctx *ctx, uint32_t L, const std::vector<uint3
   return STWO_HIP_OK;
 }
 
+// Device words of the prove's main buffer (the layout in stwo_hip_wide_fib_prove).
+static size_t prove_layout_words(uint32_t n_cols, uint32_t log_n, uint32_t B) {
+  const uint32_t E = log_n + B, C = log_n + 1;
+  const size_t n = (size_t)1 << log_n, e = (size_t)1 << E, nc = (size_t)1 << C;
+  const size_t tree_words = tree_layer_off(E + 1);
+  return n_cols * n + n_cols * e + tree_words + (B > 1 ? n_cols * nc : 0) + 4 * nc + 8 * e + tree_words + 4 * e;
+}
+
+size_t wide_fib_prove_device_bytes(uint32_t n_cols, uint32_t log_n, const stwo_hip_pcs_config &config) {
+  const uint32_t B = config.fri.log_blowup_factor, E = log_n + B;
+  size_t words = prove_layout_words(n_cols, log_n, B);
+  // the FRI prover's trees and folded evaluations (fri_prover_commit_mem's layout)
+  const std::vector<FriLayerShape> sched = fri_schedule(config.fri, E);
+  uint32_t lg = E;
+  for (const FriLayerShape &l : sched) {
+    words += tree_layer_off(l.log_size + 1);
+    lg -= l.fold_step;
+    words += (size_t)4 << lg;
+  }
+  words += 32 + 8 * 32;  // the FRI channel state
+  // OODS samples (eval_at_point): basis table, block partials, sums
+  const size_t blocks = (size_t)(n_cols + 8) << (log_n > 11 ? log_n - 11 : 0);
+  words += ((size_t)4 << 11) + 4 * blocks + 4 * (n_cols + 8);
+  // proof of work, decommitment gathers and tables: bounded by a margin
+  return words * 4 + ((size_t)16 << 20);
+}
+
 }  // namespace stwo_hip
 
 using namespace stwo_hip;
 
 extern "C" {
 
-int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols,
-                            uint32_t log_n, stwo_hip_pcs_config config, const stwo_hip_twiddles *tw,
-                            uint8_t channel_digest[32], uint32_t *channel_n_draws, stwo_hip_stark_proof **out) {
+int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols,
+                               uint32_t log_n, stwo_hip_pcs_config config, const stwo_hip_twiddles *tw, int flags,
+                               uint8_t channel_digest[32], uint32_t *channel_n_draws, stwo_hip_stark_proof **out) {
   if (!ctx) return STWO_HIP_EINVAL;
   STWO_HIP_CHECK_ARG(ctx, a && b && tw && channel_digest && channel_n_draws && out, "wide_fib_prove: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, (flags & ~STWO_HIP_PROVE_STORE_COEFFS) == 0, "wide_fib_prove: unknown flags 0x%x", flags);
   *out = nullptr;
+  const bool store = flags & STWO_HIP_PROVE_STORE_COEFFS;
   const uint32_t B = config.fri.log_blowup_factor;
   STWO_HIP_CHECK_ARG(ctx, n_cols >= 3 && n_cols <= 4096, "wide_fib_prove: %u columns (3..4096)", n_cols);
   STWO_HIP_CHECK_ARG(ctx, B >= 1 && B <= 16 && log_n >= 4 && log_n <= 27 && log_n + B <= 27,
                      "wide_fib_prove: log_n %u with log blowup %u (log_n >= 4, 1 <= blowup <= 16, log_n + blowup <= 27)",
                      log_n, B);
   STWO_HIP_CHECK_ARG(ctx, config.pow_bits <= 32, "wide_fib_prove: pow_bits %u > 32", config.pow_bits);
+  STWO_HIP_CHECK_ARG(ctx, store || B == 1,
+                     "wide_fib_prove: log blowup %u without stored coefficients (extending the trace to the "
+                     "constraint domain needs them; set STWO_HIP_PROVE_STORE_COEFFS)", B);
   const uint32_t E = log_n + B;  // committed evaluation (and lifting) log size
   const uint32_t C = log_n + 1;  // composition (constraint) domain log size
   const size_t n = (size_t)1 << log_n, e = (size_t)1 << E, nc = (size_t)1 << C;
   const size_t tree_words = tree_layer_off(E + 1);
 
-  // Device layout: trace coefficients | trace evaluations | trace tree |
-  // trace on CanonicCoset(C) (blowup > 1 only) | composition coordinates |
-  // composition-half evaluations | composition tree | quotient coordinates.
+  // Device layout (prove_layout_words): trace coefficients | trace
+  // evaluations | trace tree | trace on CanonicCoset(C) (blowup > 1 only) |
+  // composition coordinates | composition-half evaluations | composition
+  // tree | quotient coordinates.
   size_t off = 0;
   const size_t o_coef = off;
   off += n_cols * n;
@@ -316,12 +427,30 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
     ch.mix_root(root.data());
   }
 
+  // split_at_mid of the composition coordinates: left halves (coefficients
+  // [0, n)), then right halves ([n, 2n)).
+  std::vector<uint32_t *> halves(8);
+  for (int k = 0; k < 4; k++) {
+    halves[k] = comp[k];
+    halves[4 + k] = comp[k] + n;
+  }
+  ProvePlans *plans = nullptr;
+  if (!ctx->job_base) {
+    ProvePlans key{mem, n_cols, log_n, B, tw->d_tw2, tw->root.initial_index, tw->root.log_size, store};
+    rc = prove_plans_get(ctx, key, coef, eval, tree1, comp, halves, ceval, tree2, tw, &plans);
+    if (rc) return rc;
+  }
+
   // 2. trace tree.
   const std::vector<uint32_t> logs_n(n_cols, log_n), logs_e(n_cols + 8, E), logs_8(8, log_n);
   rc = stwo_hip_wide_fib_trace(ctx, coef.data(), n_cols, log_n, a, b);
-  if (!rc) rc = stwo_hip_interpolate_columns(ctx, coef.data(), logs_n.data(), n_cols, tw);
-  if (!rc) rc = stwo_hip_evaluate_polynomials(ctx, coef.data(), logs_n.data(), n_cols, B, tw, eval.data());
-  if (!rc) rc = stwo_hip_merkle_lifted_commit(ctx, eval.data(), logs_e.data(), n_cols, E, tree1, STWO_HIP_HASH_BLAKE2S);
+  if (!rc && plans) {
+    rc = stwo_hip_batch_run(ctx, plans->trace);
+  } else if (!rc) {
+    rc = stwo_hip_extend_evals(ctx, coef.data(), logs_n.data(), n_cols, B, tw, eval.data(), store ? 1 : 0);
+    if (!rc)
+      rc = stwo_hip_merkle_lifted_commit(ctx, eval.data(), logs_e.data(), n_cols, E, tree1, STWO_HIP_HASH_BLAKE2S);
+  }
   std::array<uint32_t, 8> root1;
   if (!rc) rc = stwo_hip_download(ctx, root1.data(), tree1, 8);
   if (rc) return rc;
@@ -333,18 +462,17 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
   t0 = Clock::now();
   const qm31 comp_coeff = ch.draw_secure_felt();
   if (B > 1) rc = stwo_hip_evaluate_polynomials(ctx, coef.data(), logs_n.data(), n_cols, 1, tw, ext.data());
-  if (!rc) rc = stwo_hip_memset_zero(ctx, comp[0], 4 * nc);
-  if (!rc) rc = stwo_hip_wide_fib_constraint_quotients(ctx, ext.data(), n_cols, log_n, comp_coeff.v, comp);
+  if (!rc) rc = wide_fib_constraint_quotients_impl(ctx, ext.data(), n_cols, log_n, comp_coeff.v, comp, false);
   const std::vector<uint32_t> logs_c(4, C);
-  if (!rc) rc = stwo_hip_interpolate_columns(ctx, comp, logs_c.data(), 4, tw);
-  // split_at_mid: left halves (coefficients [0, n)), then right halves ([n, 2n)).
-  std::vector<const uint32_t *> halves(8);
-  for (int k = 0; k < 4; k++) {
-    halves[k] = comp[k];
-    halves[4 + k] = comp[k] + n;
+  if (!rc && plans) {
+    rc = stwo_hip_batch_run(ctx, plans->comp);
+  } else if (!rc) {
+    rc = stwo_hip_interpolate_columns(ctx, comp, logs_c.data(), 4, tw);
+    std::vector<const uint32_t *> hc(halves.begin(), halves.end());
+    if (!rc) rc = stwo_hip_evaluate_polynomials(ctx, hc.data(), logs_8.data(), 8, B, tw, ceval.data());
+    if (!rc)
+      rc = stwo_hip_merkle_lifted_commit(ctx, ceval.data(), logs_e.data(), 8, E, tree2, STWO_HIP_HASH_BLAKE2S);
   }
-  if (!rc) rc = stwo_hip_evaluate_polynomials(ctx, halves.data(), logs_8.data(), 8, B, tw, ceval.data());
-  if (!rc) rc = stwo_hip_merkle_lifted_commit(ctx, ceval.data(), logs_e.data(), 8, E, tree2, STWO_HIP_HASH_BLAKE2S);
   std::array<uint32_t, 8> root2;
   if (!rc) rc = stwo_hip_download(ctx, root2.data(), tree2, 8);
   if (rc) return rc;
@@ -359,11 +487,22 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
   memcpy(sp.x, oods.x.v, 16);
   memcpy(sp.y, oods.y.v, 16);
   std::vector<const uint32_t *> sampled(n_cols + 8);
-  std::vector<uint32_t> sample_logs(n_cols + 8, log_n);
-  for (uint32_t k = 0; k < n_cols; k++) sampled[k] = coef[k];
-  for (int k = 0; k < 8; k++) sampled[n_cols + k] = halves[k];
   std::vector<uint32_t> values(4 * (n_cols + 8));
-  rc = stwo_hip_eval_at_point(ctx, sampled.data(), sample_logs.data(), n_cols + 8, &sp, values.data());
+  if (store) {
+    // from the coefficients (log_n)
+    std::vector<uint32_t> sample_logs(n_cols + 8, log_n);
+    for (uint32_t k = 0; k < n_cols; k++) sampled[k] = coef[k];
+    for (int k = 0; k < 8; k++) sampled[n_cols + k] = halves[k];
+    rc = stwo_hip_eval_at_point(ctx, sampled.data(), sample_logs.data(), n_cols + 8, &sp, values.data());
+  } else {
+    // from the committed evaluations (log E, the lifting size: no doubling of
+    // the point), one weight vector for every column; the FRI quotient
+    // columns, written later, hold the weights meanwhile
+    for (uint32_t k = 0; k < n_cols; k++) sampled[k] = eval[k];
+    for (int k = 0; k < 8; k++) sampled[n_cols + k] = ceval[k];
+    rc = stwo_hip_barycentric_weights(ctx, E, &sp, quot);
+    if (!rc) rc = stwo_hip_barycentric_eval_at_point(ctx, sampled.data(), n_cols + 8, E, quot, values.data());
+  }
   if (rc) return rc;
   proof->sampled_values.resize(3);
   proof->sampled_values[1].resize(n_cols);
@@ -475,6 +614,13 @@ int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t
   *channel_n_draws = ch.n_draws;
   *out = proof.release();
   return STWO_HIP_OK;
+}
+
+int stwo_hip_wide_fib_prove(stwo_hip_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n_cols,
+                            uint32_t log_n, stwo_hip_pcs_config config, const stwo_hip_twiddles *tw,
+                            uint8_t channel_digest[32], uint32_t *channel_n_draws, stwo_hip_stark_proof **out) {
+  return stwo_hip_wide_fib_prove_ex(ctx, a, b, n_cols, log_n, config, tw, STWO_HIP_PROVE_STORE_COEFFS,
+                                    channel_digest, channel_n_draws, out);
 }
 
 int stwo_hip_stark_proof_bytes(const stwo_hip_stark_proof *proof, uint8_t *buf, size_t cap, size_t *len) {

@@ -202,7 +202,69 @@ int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr) 
   return upload_tables(ctx, 1, &src, &bytes, dptr);
 }
 
+static int job_carve(stwo_hip_ctx *ctx, size_t bytes, void **dptr, const char *what) {
+  const size_t off = (ctx->job_used + 255) & ~(size_t)255;
+  if (off + bytes > ctx->job_cap)
+    return set_error(ctx, STWO_HIP_ENOMEM, "%s of %zu bytes exceeds the job's memory (%zu of %zu used)", what, bytes,
+                     ctx->job_used, ctx->job_cap);
+  *dptr = ctx->job_base + off;
+  ctx->job_used = off + bytes;
+  return STWO_HIP_OK;
+}
+
+void job_mem_begin(stwo_hip_ctx *ctx, void *base, size_t cap) {
+  ctx->job_base = (uint8_t *)base;
+  ctx->job_cap = cap;
+  ctx->job_used = 0;
+  ctx->job_keys.clear();
+}
+
+void job_mem_end(stwo_hip_ctx *ctx) {
+  for (const std::string &k : ctx->job_keys) ctx->scratch.erase(k);
+  ctx->job_keys.clear();
+  ctx->job_base = nullptr;
+  ctx->job_cap = ctx->job_used = 0;
+}
+
+int device_alloc(stwo_hip_ctx *ctx, size_t bytes, void **dptr, bool *from_job) {
+  *from_job = ctx->job_base != nullptr;
+  if (*from_job) return job_carve(ctx, bytes, dptr, "device table");
+  if (hipMalloc(dptr, bytes ? bytes : 256) != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    return set_error(ctx, STWO_HIP_ENOMEM, "device allocation of %zu bytes failed", bytes);
+  }
+  return STWO_HIP_OK;
+}
+
+void device_free(void *p, bool from_job) {
+  if (p && !from_job) (void)hipFree(p);
+}
+
 int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr) {
+  if (ctx->job_base) {
+    // Within a job: entries carved from the job block (a larger request for
+    // a key carves a new piece; the block is released with the job).
+    auto it = ctx->scratch.find(key);
+    const bool carved = std::find(ctx->job_keys.begin(), ctx->job_keys.end(), key) != ctx->job_keys.end();
+    if (it != ctx->scratch.end() && carved && it->second.second >= bytes) {
+      *dptr = it->second.first;
+      return STWO_HIP_OK;
+    }
+    void *p = nullptr;
+    int rc = job_carve(ctx, bytes ? bytes : 256, &p, key);
+    if (rc) return rc;
+    if (it != ctx->scratch.end() && !carved) {
+      // an entry from before the job: freed, the job's piece takes its key
+      STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      (void)hipFree(it->second.first);
+      ctx->scratch.erase(it);
+    }
+    ctx->scratch[key] = {p, bytes};
+    if (!carved) ctx->job_keys.push_back(key);
+    *dptr = p;
+    return STWO_HIP_OK;
+  }
   auto it = ctx->scratch.find(key);
   if (it != ctx->scratch.end() && it->second.second >= bytes) {
     *dptr = it->second.first;
@@ -291,6 +353,7 @@ int stwo_hip_ctx_destroy(stwo_hip_ctx *ctx) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
   }
+  if (ctx->prove_plans_free) ctx->prove_plans_free(ctx, ctx->prove_plans);
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   for (auto &kv : ctx->scratch) (void)hipFree(kv.second.first);
   if (ctx->h_arena) (void)hipHostFree(ctx->h_arena);

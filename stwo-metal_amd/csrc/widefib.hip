@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t 
                                                                  uint32_t eval_log, uint32_t log_n,
                                                                  const uint32_t *weights, const uint32_t *denom_inv,
                                                                  uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
-                                                                 uint32_t *acc3) {
+                                                                 uint32_t *acc3, uint32_t accumulate) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >> eval_log) return;
   uint64_t a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0};
@@ -91,34 +91,22 @@ __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t 
 #pragma unroll
   for (int k = 0; k < 4; k++) res.v[k] = m31_add(m31_reduce64(a1[k]), m31_reduce64(a2[k]));
   res = qm31_mul_m31(res, denom_inv[r >> log_n]);
-  acc0[r] = m31_add(acc0[r], res.v[0]);
-  acc1[r] = m31_add(acc1[r], res.v[1]);
-  acc2[r] = m31_add(acc2[r], res.v[2]);
-  acc3[r] = m31_add(acc3[r], res.v[3]);
+  if (accumulate) {
+    acc0[r] = m31_add(acc0[r], res.v[0]);
+    acc1[r] = m31_add(acc1[r], res.v[1]);
+    acc2[r] = m31_add(acc2[r], res.v[2]);
+    acc3[r] = m31_add(acc3[r], res.v[3]);
+  } else {
+    acc0[r] = res.v[0];
+    acc1[r] = res.v[1];
+    acc2[r] = res.v[2];
+    acc3[r] = res.v[3];
+  }
 }
 
-}  // namespace stwo_hip
-
-using namespace stwo_hip;
-
-extern "C" {
-
-int stwo_hip_wide_fib_trace(stwo_hip_ctx *ctx, uint32_t *const *cols, uint32_t n_cols, uint32_t log_n,
-                            const uint32_t *a, const uint32_t *b) {
-  if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, cols && a && b && n_cols >= 2 && n_cols <= 4096 && log_n >= 1 && log_n <= 28,
-                     "wide_fib_trace: bad arguments");
-  for (uint32_t k = 0; k < n_cols; k++) STWO_HIP_CHECK_ARG(ctx, cols[k] != nullptr, "wide_fib_trace: NULL column");
-  void *d_tab = nullptr;
-  int rc = upload_table(ctx, cols, n_cols * sizeof(uint32_t *), &d_tab);
-  if (rc) return rc;
-  const uint64_t n = 1ull << log_n;
-  return launch(ctx, "wide_fib_trace", wide_fib_trace_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
-                (uint32_t *const *)d_tab, n_cols, log_n, a, b);
-}
-
-int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
-                                           uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4]) {
+int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
+                                       uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4],
+                                       bool accumulate) {
   if (!ctx) return STWO_HIP_EINVAL;
   STWO_HIP_CHECK_ARG(ctx, cols && random_coeff && accum && n_cols >= 3 && n_cols <= 4096 && log_n >= 1 &&
                               log_n <= 27,
@@ -177,7 +165,32 @@ int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *co
   const uint64_t n = 1ull << eval_log;
   return launch(ctx, "wide_fib_constraint_quotients", wide_fib_quotients_kernel, dim3((uint32_t)((n + 255) / 256)),
                 dim3(256), 0, (const uint32_t *const *)d_cols, n_cols, eval_log, log_n, (const uint32_t *)d_pw,
-                (const uint32_t *)d_dinv, accum[0], accum[1], accum[2], accum[3]);
+                (const uint32_t *)d_dinv, accum[0], accum[1], accum[2], accum[3], (uint32_t)accumulate);
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_wide_fib_trace(stwo_hip_ctx *ctx, uint32_t *const *cols, uint32_t n_cols, uint32_t log_n,
+                            const uint32_t *a, const uint32_t *b) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, cols && a && b && n_cols >= 2 && n_cols <= 4096 && log_n >= 1 && log_n <= 28,
+                     "wide_fib_trace: bad arguments");
+  for (uint32_t k = 0; k < n_cols; k++) STWO_HIP_CHECK_ARG(ctx, cols[k] != nullptr, "wide_fib_trace: NULL column");
+  void *d_tab = nullptr;
+  int rc = upload_table(ctx, cols, n_cols * sizeof(uint32_t *), &d_tab);
+  if (rc) return rc;
+  const uint64_t n = 1ull << log_n;
+  return launch(ctx, "wide_fib_trace", wide_fib_trace_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+                (uint32_t *const *)d_tab, n_cols, log_n, a, b);
+}
+
+int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
+                                           uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4]) {
+  return wide_fib_constraint_quotients_impl(ctx, cols, n_cols, log_n, random_coeff, accum, true);
 }
 
 }  // extern "C"

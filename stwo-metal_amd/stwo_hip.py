@@ -56,22 +56,37 @@ PROVE_PHASES = ("trace_commit", "composition_commit", "oods_samples", "fri_quoti
                 "proof_of_work", "decommit")
 
 
+JOB_COMMIT_FRI, JOB_PROVE = 0, 1
+
+
+class BatchSet(ctypes.Structure):
+    _fields_ = [("cols", ctypes.c_void_p), ("log_sizes", ctypes.c_void_p), ("n_cols", ctypes.c_size_t),
+                ("log_blowup", ctypes.c_uint32), ("outs", ctypes.c_void_p), ("layers", ctypes.c_void_p),
+                ("tree_kind", ctypes.c_int), ("hash_mode", ctypes.c_int), ("stage", ctypes.c_int)]
+
+
+BATCH_LDE, BATCH_IFFT, BATCH_RFFT = 0, 1, 2
+
+
 class JobDesc(ctypes.Structure):
-    _fields_ = [("n_cols", ctypes.c_uint32), ("log_size", ctypes.c_uint32), ("log_blowup", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("fri", FriConfig)]
+    _fields_ = [("kind", ctypes.c_uint32), ("n_cols", ctypes.c_uint32), ("log_size", ctypes.c_uint32),
+                ("log_blowup", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("cols", ctypes.c_void_p),
+                ("host", ctypes.c_void_p), ("fri", FriConfig), ("pcs", PcsConfig), ("proof_out", ctypes.c_void_p),
+                ("proof_cap", ctypes.c_size_t)]
 
 
 class JobResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("trace_root", ctypes.c_uint32 * 8), ("n_fri_roots", ctypes.c_uint32),
                 ("fri_roots", ctypes.c_uint32 * 256), ("last_layer_poly", ctypes.c_uint32 * 64),
-                ("final_digest", ctypes.c_uint8 * 32), ("t_submit_ms", ctypes.c_double),
-                ("t_admit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double), ("t_done_ms", ctypes.c_double)]
+                ("final_digest", ctypes.c_uint8 * 32), ("proof_len", ctypes.c_size_t), ("group_size", ctypes.c_uint32),
+                ("t_submit_ms", ctypes.c_double), ("t_admit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double),
+                ("t_done_ms", ctypes.c_double)]
 
     def key(self):
         """Everything the job computed (for alone-vs-mix comparisons)."""
         n = self.n_fri_roots
         return (self.status, tuple(self.trace_root), tuple(self.fri_roots[: 8 * n]), tuple(self.last_layer_poly),
-                bytes(self.final_digest))
+                bytes(self.final_digest), self.proof_len)
 
 
 def half_odds(n):
@@ -121,6 +136,8 @@ def lib():
             "stwo_hip_eval_at_point_by_folding": [_vp, _vp, ctypes.c_uint32, _u32p, _vp, _u32p],
             "stwo_hip_split_at_mid": [_vp, _vp, ctypes.c_uint32, _vp, _vp],
             "stwo_hip_fri_decompose": [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.POINTER(_vp), _u32p],
+            "stwo_hip_lde_commit_per_column": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32,
+                                               _vp, ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int],
             "stwo_hip_extend_evals": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
                                       ctypes.POINTER(_vp), ctypes.c_int],
             "stwo_hip_merkle_commit_on_layer": [_vp, ctypes.c_uint32, _vp, ctypes.POINTER(_vp), ctypes.c_size_t, _vp,
@@ -138,10 +155,14 @@ def lib():
             "stwo_hip_batch_create": [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_size_t, ctypes.c_uint32, _vp,
                                       ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(_vp)],
+            "stwo_hip_batch_create_multi": [_vp, _vp, ctypes.c_size_t, _vp, ctypes.c_int, ctypes.POINTER(_vp)],
             "stwo_hip_batch_run": [_vp, _vp],
             "stwo_hip_batch_destroy": [_vp, _vp],
             "stwo_hip_server_create": [ctypes.c_int, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                        ctypes.POINTER(_vp)],
+            "stwo_hip_server_set_max_group": [_vp, ctypes.c_uint32],
+            "stwo_hip_server_hold": [_vp, ctypes.c_int],
+            "stwo_hip_server_job_bytes": [_vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_size_t)],
             "stwo_hip_server_submit": [_vp, ctypes.POINTER(JobDesc), ctypes.POINTER(ctypes.c_uint64)],
             "stwo_hip_server_poll": [_vp, ctypes.c_uint64, ctypes.POINTER(JobResult)],
             "stwo_hip_server_wait": [_vp, ctypes.c_uint64, ctypes.POINTER(JobResult)],
@@ -177,6 +198,8 @@ def lib():
             "stwo_hip_grind": [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)],
             "stwo_hip_wide_fib_prove": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, PcsConfig, _vp,
                                         ctypes.c_char_p, _u32p, ctypes.POINTER(_vp)],
+            "stwo_hip_wide_fib_prove_ex": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, PcsConfig, _vp, ctypes.c_int,
+                                           ctypes.c_char_p, _u32p, ctypes.POINTER(_vp)],
             "stwo_hip_stark_proof_bytes": [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
             "stwo_hip_stark_proof_info": [_vp, ctypes.POINTER(ctypes.c_double), _u32p, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_size_t)],
@@ -540,18 +563,25 @@ class Context:
         self._check(lib().stwo_hip_grind(self.h, bytes(digest), pow_bits, ctypes.byref(n)))
         return n.value
 
-    def wide_fib_prove(self, a, b, n_cols, log_n, tw, config=None, digest=bytes(32), n_draws=0):
+    def wide_fib_prove(self, a, b, n_cols, log_n, tw, config=None, digest=bytes(32), n_draws=0, store_coeffs=True):
         """The wide-Fibonacci prove (empty preprocessed tree, trace tree, prove)
-        from the device inputs a, b (2^log_n words each).  Returns (proof
-        bytes, phase times in ms by name, sorted query positions, channel
-        digest, channel n_draws)."""
+        from the device inputs a, b (2^log_n words each).  store_coeffs=True:
+        stwo_hip_wide_fib_prove (coefficients kept, as the reference's
+        benchmark sets); False: stwo_hip_wide_fib_prove_ex without stored
+        coefficients (the PCS default).  Returns (proof bytes, phase times in
+        ms by name, sorted query positions, channel digest, channel n_draws)."""
         cfg = config if config is not None else PcsConfig.default()
         d = ctypes.create_string_buffer(bytes(digest), 32)
         nd = np.array([n_draws], dtype=np.uint32)
         h = _vp()
-        self._check(lib().stwo_hip_wide_fib_prove(self.h, a.ptr if isinstance(a, DeviceBuf) else a,
-                                                  b.ptr if isinstance(b, DeviceBuf) else b, n_cols, log_n, cfg,
-                                                  tw.h, d, nd.ctypes.data_as(_u32p), ctypes.byref(h)))
+        pa = a.ptr if isinstance(a, DeviceBuf) else a
+        pb = b.ptr if isinstance(b, DeviceBuf) else b
+        if store_coeffs:
+            self._check(lib().stwo_hip_wide_fib_prove(self.h, pa, pb, n_cols, log_n, cfg, tw.h, d,
+                                                      nd.ctypes.data_as(_u32p), ctypes.byref(h)))
+        else:
+            self._check(lib().stwo_hip_wide_fib_prove_ex(self.h, pa, pb, n_cols, log_n, cfg, tw.h, 0, d,
+                                                         nd.ctypes.data_as(_u32p), ctypes.byref(h)))
         try:
             n = ctypes.c_size_t()
             self._check(lib().stwo_hip_stark_proof_bytes(h, None, 0, ctypes.byref(n)))
@@ -606,6 +636,36 @@ class Context:
                                                 layers.ptr if isinstance(layers, DeviceBuf) else layers,
                                                 tree_kind, mode, flags, ctypes.byref(h)))
         return Batch(self, h.value)
+
+    def batch_create_multi(self, sets, tw, use_graph=True, drop_coeffs=False):
+        """stwo_hip_batch_create_multi; sets: [(cols, log_sizes, log_blowup, outs, layers, tree_kind, mode[,
+        stage])] (outs / layers None where the stage has none)."""
+        keep, arr = [], (BatchSet * max(1, len(sets)))()
+        for i, st in enumerate(sets):
+            cols, logs, b, outs, layers, kind, mode = st[:7]
+            stage = st[7] if len(st) > 7 else BATCH_LDE
+            cp = _ptrs(cols)
+            op = _ptrs(outs) if outs is not None else None
+            ls = np.ascontiguousarray(logs, dtype=np.uint32)
+            keep += [cp, op, ls]
+            arr[i] = BatchSet(ctypes.cast(cp, ctypes.c_void_p), ls.ctypes.data, len(cols), b,
+                              ctypes.cast(op, ctypes.c_void_p) if op is not None else None,
+                              layers.ptr if isinstance(layers, DeviceBuf) else layers, kind, mode, stage)
+        h = _vp()
+        flags = (BATCH_GRAPH if use_graph else 0) | (BATCH_DROP_COEFFS if drop_coeffs else 0)
+        self._check(lib().stwo_hip_batch_create_multi(self.h, ctypes.addressof(arr), len(sets), tw.h, flags,
+                                                      ctypes.byref(h)))
+        del keep
+        return Batch(self, h.value)
+
+    def lde_commit_per_column(self, cols, log_sizes, log_blowup, tw, outs, layers, tree_kind=TREE_MIXED,
+                              mode=HASH_BLAKE2S):
+        """stwo_hip_lde_commit_per_column: the plain per-column form of a batch plan's work."""
+        ls = np.ascontiguousarray(log_sizes, dtype=np.uint32)
+        self._check(lib().stwo_hip_lde_commit_per_column(self.h, _ptrs(cols), ls.ctypes.data_as(_u32p), len(cols),
+                                                         log_blowup, tw.h, _ptrs(outs),
+                                                         layers.ptr if isinstance(layers, DeviceBuf) else layers,
+                                                         tree_kind, mode))
 
     # -- profiling --------------------------------------------------------------
     def prof_enable(self, on=True):
@@ -701,22 +761,79 @@ def tree_words(max_log):
 
 
 class Server:
-    """Continuous-batching commit + FRI server (stwo_hip_server)."""
+    """Continuous-batching server of commit + FRI and prove jobs (stwo_hip_server)."""
 
-    def __init__(self, device=0, hbm_budget_bytes=8 << 30, max_log_size=24, n_slots=4):
+    def __init__(self, device=0, hbm_budget_bytes=8 << 30, max_log_size=24, n_slots=4, max_group=None):
         h = _vp()
         rc = lib().stwo_hip_server_create(device, hbm_budget_bytes, max_log_size, n_slots, ctypes.byref(h))
         if rc != 0:
             raise StwoHipError(rc, "stwo_hip_server_create failed")
         self.h = h.value
+        self._keep = {}  # caller data of pending jobs (kept alive until wait)
+        if max_group is not None:
+            rc = lib().stwo_hip_server_set_max_group(self.h, max_group)
+            if rc != 0:
+                raise StwoHipError(rc, "set_max_group rejected")
 
-    def submit(self, n_cols, log_size, log_blowup, seed, fri):
-        d = JobDesc(n_cols, log_size, log_blowup, 0, seed, fri)
+    def _desc(self, kind, n_cols, log_size, log_blowup, seed, fri=None, pcs=None, cols=None, host=None,
+              proof_cap=0):
+        d = JobDesc()
+        d.kind, d.n_cols, d.log_size, d.log_blowup, d.seed = kind, n_cols, log_size, log_blowup, seed
+        keep = []
+        if cols is not None:
+            arr = (ctypes.c_void_p * len(cols))(*[c.ptr if isinstance(c, DeviceBuf) else c for c in cols])
+            keep.append(arr)
+            d.cols = ctypes.cast(arr, ctypes.c_void_p)
+        if host is not None:
+            h = np.ascontiguousarray(host, dtype=np.uint32)
+            keep.append(h)
+            d.host = h.ctypes.data
+        if fri is not None:
+            d.fri = fri
+        if pcs is not None:
+            d.pcs = pcs
+        if proof_cap:
+            buf = (ctypes.c_uint8 * proof_cap)()
+            keep.append(buf)
+            d.proof_out = ctypes.cast(buf, ctypes.c_void_p)
+            d.proof_cap = proof_cap
+        return d, keep
+
+    def _submit(self, d, keep):
         jid = ctypes.c_uint64()
         rc = lib().stwo_hip_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid))
         if rc != 0:
             raise StwoHipError(rc, "submit rejected")
+        self._keep[jid.value] = keep
         return jid.value
+
+    def hold(self, on=True):
+        """stwo_hip_server_hold: while held, submitted jobs queue without being admitted."""
+        rc = lib().stwo_hip_server_hold(self.h, int(bool(on)))
+        if rc != 0:
+            raise StwoHipError(rc, "hold rejected")
+
+    def submit(self, n_cols, log_size, log_blowup, seed, fri, cols=None, host=None):
+        """A commit + FRI job: seeded columns, or the caller's device columns
+        `cols` / host data `host` (n_cols x 2^log_size words)."""
+        d, keep = self._desc(JOB_COMMIT_FRI, n_cols, log_size, log_blowup, seed, fri=fri, cols=cols, host=host)
+        return self._submit(d, keep)
+
+    def submit_prove(self, n_cols, log_size, seed, pcs=None, cols=None, host=None, proof_cap=1 << 20):
+        """A wide-Fibonacci prove job; wait() then returns (result, proof bytes)."""
+        pcs = pcs or PcsConfig.default()
+        d, keep = self._desc(JOB_PROVE, n_cols, log_size, pcs.fri.log_blowup_factor, seed, pcs=pcs, cols=cols,
+                             host=host, proof_cap=proof_cap)
+        return self._submit(d, keep)
+
+    def job_bytes(self, n_cols, log_size, log_blowup, fri=None, pcs=None, kind=JOB_COMMIT_FRI):
+        d, keep = self._desc(kind, n_cols, log_size, log_blowup, 1, fri=fri, pcs=pcs,
+                             proof_cap=1 if kind == JOB_PROVE else 0)
+        b = ctypes.c_size_t()
+        rc = lib().stwo_hip_server_job_bytes(self.h, ctypes.byref(d), ctypes.byref(b))
+        if rc != 0:
+            raise StwoHipError(rc, "job_bytes rejected")
+        return b.value
 
     def poll(self, jid):
         r = JobResult()
@@ -726,8 +843,12 @@ class Server:
     def wait(self, jid):
         r = JobResult()
         rc = lib().stwo_hip_server_wait(self.h, jid, ctypes.byref(r))
+        keep = self._keep.pop(jid, [])
         if rc != 0:
             raise StwoHipError(rc, "wait failed")
+        for k in keep:
+            if isinstance(k, ctypes.Array) and k._type_ is ctypes.c_uint8:
+                return r, bytes(k[: r.proof_len])
         return r
 
     def stats(self):

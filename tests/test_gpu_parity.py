@@ -290,11 +290,14 @@ def test_merkle_golden(ctx, golden):
 
 @pytest.mark.parametrize("mode", [sh.HASH_BLAKE2S, sh.HASH_BLAKE2S_M31])
 def test_commit_on_layer_vs_oracle(ctx, mode):
-    # > 16 columns up to 2^14 nodes: a quad per node; at 2^15: a lane per node
-    # with the loads one block ahead; otherwise a lane per node.
+    # > 16 columns up to 2^14 nodes: a quad per node, the columns staged
+    # through LDS 240 at a time (one, two and three chunks, partial last
+    # chunks, fewer than 64 nodes); at 2^15: a lane per node with the loads
+    # one block ahead; otherwise a lane per node.
     for log_size, n_cols, with_prev in [(10, 0, True), (10, 1, False), (11, 16, False), (9, 17, True),
                                         (12, 33, True), (3, 5, True), (0, 2, True), (14, 40, False),
-                                        (1, 18, False), (15, 20, True), (16, 17, False)]:
+                                        (1, 18, False), (15, 20, True), (16, 17, False), (11, 241, True),
+                                        (6, 480, False), (13, 250, False), (5, 300, True), (8, 721, True)]:
         cols = [rand(1 << log_size, 13, 100 * log_size + c) for c in range(n_cols)]
         prev = rand(16 << log_size, 17, log_size) if with_prev else None
         dcols = [ctx.to_device(c) for c in cols]
@@ -662,6 +665,131 @@ def test_batch_lde_commit_vs_oracle(ctx, tree_kind, use_graph, mode):
         assert np.array_equal(layers.download(), exp_tree), ("tree", rep)
     plan.destroy()
     for x in cols + outs + [layers]:
+        x.free()
+    tw.destroy()
+
+
+def _lde_expect(evs, logs, b, tw_root, tree_kind, mode):
+    et, eit = oracle.precompute_twiddles(tw_root.initial_index, tw_root.log_size)
+    coeffs = [e.copy() for e in evs]
+    oracle.interpolate_columns(coeffs, eit)
+    exp_evals = oracle.evaluate_polynomials(coeffs, b, et)
+    if tree_kind == sh.TREE_MIXED:
+        tree = oracle.merkle_commit(exp_evals, m31_output=bool(mode))
+    else:
+        tree = oracle.lifted_commit(exp_evals, max(logs) + b, m31_output=bool(mode))
+    return coeffs, exp_evals, tree
+
+
+@pytest.mark.parametrize("tree_kind,mode", [(sh.TREE_MIXED, sh.HASH_BLAKE2S), (sh.TREE_LIFTED, sh.HASH_BLAKE2S_M31)])
+def test_lde_commit_per_column_vs_oracle(ctx, tree_kind, mode):
+    """The plain per-column entry point: coefficients, evaluations and tree
+    bit-exact against the oracle (uneven columns log 4..16, blowup 2)."""
+    logs = _geometric_logs(40, 4, 16, 9) + [16, 4]
+    b = 1
+    root = sh.half_odds(max(logs) + b - 1)
+    tw = ctx.precompute_twiddles(root)
+    evs = [rand(1 << l, 91, i) for i, l in enumerate(logs)]
+    cols = [ctx.to_device(e) for e in evs]
+    outs = [ctx.alloc(1 << (l + b)) for l in logs]
+    layers = ctx.alloc(sh.tree_words(max(logs) + b))
+    ctx.lde_commit_per_column(cols, logs, b, tw, outs, layers, tree_kind, mode)
+    ctx.sync()
+    coeffs, exp_evals, tree = _lde_expect(evs, logs, b, root, tree_kind, mode)
+    for c, e in zip(cols, coeffs):
+        assert np.array_equal(c.download(), e)
+    for o, e in zip(outs, exp_evals):
+        assert np.array_equal(o.download(), e)
+    assert np.array_equal(layers.download(), tree)
+    for x in cols + outs + [layers]:
+        x.free()
+    tw.destroy()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_multi_sets_vs_oracle(ctx, use_graph):
+    """Three column sets in one plan -- blowups 2, 4 and 2, mixed and lifted
+    trees, shared log sizes across sets (their FFT launches are shared) --
+    coefficients dropped; every set's evaluations and tree bit-exact against
+    the oracle on two replays."""
+    specs = [(_geometric_logs(20, 5, 14, 21) + [14], 1, sh.TREE_MIXED, sh.HASH_BLAKE2S),
+             (_geometric_logs(12, 5, 12, 22) + [12, 5], 2, sh.TREE_LIFTED, sh.HASH_BLAKE2S),
+             ([14, 9, 9, 6], 1, sh.TREE_MIXED, sh.HASH_BLAKE2S_M31)]
+    max_log = max(max(l) + b for l, b, _, _ in specs)
+    root = sh.half_odds(max_log - 1)
+    tw = ctx.precompute_twiddles(root)
+    sets, bufs, expect = [], [], []
+    for si, (logs, b, kind, mode) in enumerate(specs):
+        evs = [rand(1 << l, 300 + si, i) for i, l in enumerate(logs)]
+        cols = [ctx.to_device(e) for e in evs]
+        outs = [ctx.alloc(1 << (l + b)) for l in logs]
+        layers = ctx.alloc(sh.tree_words(max(logs) + b))
+        sets.append((cols, logs, b, outs, layers, kind, mode))
+        bufs += cols + outs + [layers]
+        expect.append((evs, _lde_expect(evs, logs, b, root, kind, mode)))
+    plan = ctx.batch_create_multi(sets, tw, use_graph=use_graph, drop_coeffs=True)
+    for rep in range(2):
+        for (cols, logs, b, outs, layers, kind, mode), (evs, _) in zip(sets, expect):
+            for c, e in zip(cols, evs):
+                c.upload(e)
+            layers.upload(np.zeros(sh.tree_words(max(logs) + b), dtype=U))
+        plan.run()
+        ctx.sync()
+        for si, ((cols, logs, b, outs, layers, kind, mode), (_, (_, exp_evals, tree))) in enumerate(zip(sets, expect)):
+            for o, e in zip(outs, exp_evals):
+                assert np.array_equal(o.download(), e), ("evals", si, rep)
+            assert np.array_equal(layers.download(), tree), ("tree", si, rep)
+    plan.destroy()
+    for x in bufs:
+        x.free()
+    tw.destroy()
+
+
+@pytest.mark.parametrize("b", [1, 2])
+def test_batch_ifft_then_rfft_sets(ctx, b):
+    """A composition-polynomial plan: an IFFT-only set (4 columns of log 13,
+    interpolated in place) and an RFFT-only set evaluating their halves
+    (coefficients [0, 2^12) and [2^12, 2^13) of each, 8 columns of log 12) at
+    the blowup into a lifted tree; plus an LDE set of two other columns in the
+    same plan.  Coefficients, evaluations and both trees bit-exact against the
+    oracle, on two graph replays."""
+    C = 13
+    n = C - 1
+    root = sh.half_odds(C + b - 1)
+    tw = ctx.precompute_twiddles(root)
+    et, eit = oracle.precompute_twiddles(root.initial_index, root.log_size)
+    comp_ev = [rand(1 << C, 400, k) for k in range(4)]
+    comp = [ctx.to_device(e) for e in comp_ev]
+    halves = [c.at(0) for c in comp] + [c.at(1 << n) for c in comp]
+    hout = [ctx.alloc(1 << (n + b)) for _ in range(8)]
+    htree = ctx.alloc(sh.tree_words(n + b))
+    lde_ev = [rand(1 << 11, 401, k) for k in range(2)]
+    lde = [ctx.to_device(e) for e in lde_ev]
+    lout = [ctx.alloc(1 << (11 + b)) for _ in range(2)]
+    ltree = ctx.alloc(sh.tree_words(11 + b))
+    sets = [(comp, [C] * 4, 0, None, None, 0, 0, sh.BATCH_IFFT),
+            (halves, [n] * 8, b, hout, htree, sh.TREE_LIFTED, sh.HASH_BLAKE2S, sh.BATCH_RFFT),
+            (lde, [11, 11], b, lout, ltree, sh.TREE_MIXED, sh.HASH_BLAKE2S, sh.BATCH_LDE)]
+    plan = ctx.batch_create_multi(sets, tw, use_graph=True)
+    coeffs = [e.copy() for e in comp_ev]
+    oracle.interpolate_columns(coeffs, eit)
+    hc = [c[: 1 << n].copy() for c in coeffs] + [c[1 << n:].copy() for c in coeffs]
+    exp_h = oracle.evaluate_polynomials(hc, b, et)
+    exp_htree = oracle.lifted_commit(exp_h, n + b)
+    _, exp_l, exp_ltree = _lde_expect(lde_ev, [11, 11], b, root, sh.TREE_MIXED, sh.HASH_BLAKE2S)
+    for rep in range(2):
+        for d, e in zip(comp + lde, comp_ev + lde_ev):
+            d.upload(e)
+        plan.run()
+        ctx.sync()
+        for d, e in zip(comp, coeffs):
+            assert np.array_equal(d.download(), e), ("coeffs", rep)
+        for o, e in zip(hout + lout, exp_h + exp_l):
+            assert np.array_equal(o.download(), e), ("evals", rep)
+        assert np.array_equal(htree.download(), exp_htree), ("halves tree", rep)
+        assert np.array_equal(ltree.download(), exp_ltree), ("lde tree", rep)
+    plan.destroy()
+    for x in comp + hout + [htree] + lde + lout + [ltree]:
         x.free()
     tw.destroy()
 

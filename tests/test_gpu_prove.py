@@ -36,11 +36,11 @@ def twiddles(ctx, E):
     return _tws[E]
 
 
-def lib_prove(ctx, a, b, n_cols, log_n, cfg, digest=bytes(32), n_draws=0):
+def lib_prove(ctx, a, b, n_cols, log_n, cfg, digest=bytes(32), n_draws=0, store_coeffs=True):
     da, db = ctx.to_device(a), ctx.to_device(b)
     try:
         return ctx.wide_fib_prove(da, db, n_cols, log_n, twiddles(ctx, log_n + cfg.fri.log_blowup_factor), cfg,
-                                  digest, n_draws)
+                                  digest, n_draws, store_coeffs=store_coeffs)
     finally:
         da.free()
         db.free()
@@ -86,6 +86,45 @@ def test_prove_bytes_equal_oracle(ctx, case):
     assert set(phases) == set(sh.PROVE_PHASES) and all(v >= 0 for v in phases.values())
 
 
+@pytest.mark.parametrize("case", [c for c in range(len(CASES)) if CASES[c][2].fri.log_blowup_factor == 1])
+def test_prove_without_stored_coefficients_equals_oracle(ctx, case):
+    """The PCS default (store_polynomials_coefficients = false): the trace LDE
+    drops the coefficients and the OODS samples come from the committed
+    evaluations through barycentric weights; the proof bytes and the channel
+    are the oracle's (which samples the coefficients)."""
+    log_n, n_cols, cfg, kind = CASES[case]
+    a, b = inputs(log_n, kind, 3)
+    data, _, qpos, digest, n_draws = lib_prove(ctx, a, b, n_cols, log_n, cfg, store_coeffs=False)
+    ch = O.Channel()
+    want = PR.wide_fib_prove(a, b, n_cols, log_n, oracle_cfg(cfg), ch)
+    assert qpos == want["query_positions"]
+    assert data == PR.proof_bytes(want)
+    assert digest == ch.digest_bytes() and n_draws == ch.n_draws
+
+
+def test_prove_plans_follow_the_shape(ctx):
+    """The context keeps the last prove shape's launch plans: proves of
+    alternating shapes and both sampling paths, each against the oracle."""
+    seq = [(5, 100, True), (4, 8, False), (5, 100, False), (5, 100, True), (8, 64, True), (5, 100, True)]
+    for i, (log_n, n_cols, store) in enumerate(seq):
+        a, b = inputs(log_n, "rand", 20 + i)
+        data, _, _, _, _ = lib_prove(ctx, a, b, n_cols, log_n, pcs(), store_coeffs=store)
+        want = PR.wide_fib_prove(a, b, n_cols, log_n, PR.PcsConfig(), O.Channel())
+        assert data == PR.proof_bytes(want), (i, log_n, n_cols, store)
+
+
+def test_prove_without_stored_coefficients_needs_blowup_1(ctx):
+    a, b = inputs(5, "ref")
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    try:
+        with pytest.raises(sh.StwoHipError) as e:
+            ctx.wide_fib_prove(da, db, 8, 5, twiddles(ctx, 7), pcs(blowup=2), store_coeffs=False)
+        assert e.value.code == -1
+    finally:
+        da.free()
+        db.free()
+
+
 def test_prove_continues_a_channel(ctx):
     """A non-default starting channel state (a prover that mixed other data first)."""
     log_n, n_cols = 5, 10
@@ -113,6 +152,9 @@ def test_prove_full_size_verifies(ctx):
     assert digest == ch.digest_bytes() and n_draws == ch.n_draws
     proof = PR.parse_proof(data)
     PR.verify(proof, n_cols, log_n)
+    # the PCS default path (no stored coefficients) gives the same bytes
+    data2, _, _, _, _ = lib_prove(ctx, a, b, n_cols, log_n, pcs(), store_coeffs=False)
+    assert data2 == data
     bad = PR.parse_proof(data)
     bad["queried_values"][1][100][0] = (bad["queried_values"][1][100][0] + 1) % O.P
     with pytest.raises(PR.VerificationError):
