@@ -613,6 +613,23 @@ def main():
     }
     log("side rates: %s" % json.dumps(side))
 
+    # The same workload in the reference's plain per-column form (one
+    # interpolate and one evaluate launch sequence per column, then the
+    # commit: stwo_hip_lde_commit_per_column), timed beside the headline.
+    pc = []
+    for i in range(3):
+        for w, p in zip(work, pristine):
+            ctx._check(sh.lib().stwo_hip_copy(ctx.h, w.ptr, p.ptr, w.n))
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.lde_commit_per_column(work, logs, LOG_BLOWUP, tw, outs, layers, sh.TREE_MIXED)
+        ctx.sync()
+        pc.append(time.perf_counter() - t0)
+    per_column = {"form": "per column: stwo_hip_interpolate + stwo_hip_evaluate per column, then the commit",
+                  "ms": round(min(pc) * 1e3, 3), "melem_s": round(elems / min(pc) / 1e6, 1),
+                  "root_equal": bool(np.array_equal(layers.download(8), gpu_root))}
+    log("per-column form: %s" % json.dumps(per_column))
+
     serve = None
     batched = None
     if not args.no_batched:
@@ -684,6 +701,7 @@ def main():
         },
         "valu_roofline": valu_roofline,
         "side_rates": side,
+        "per_column_form": per_column,
         "hbm_kernels": hbm,
         "cpu_baseline": cpu,
         "kernels": kernels,

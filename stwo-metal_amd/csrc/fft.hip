@@ -299,22 +299,6 @@ __device__ __forceinline__ uint32_t gld(const uint32_t *base, uint32_t byte_off)
 __device__ __forceinline__ void gst(uint32_t *base, uint32_t byte_off, uint32_t v) {
   *(g_u32 *)((g_u8 *)as_global(base) + byte_off) = v;
 }
-// The same with a constant part `imm`: the 32-bit lane offset is widened
-// before the constant is added, so the constant (below 4096) folds into the
-// instruction's immediate offset instead of taking a VALU add per load (a
-// 32-bit sum could wrap, which keeps the compiler from folding it).
-__device__ __forceinline__ uint32_t gld_imm(const uint32_t *base, uint32_t lane_off, uint32_t imm) {
-  return *(gconst_u32 *)((gconst_u8 *)as_global(base) + (uint64_t)lane_off + imm);
-}
-__device__ __forceinline__ void gst_imm(uint32_t *base, uint32_t lane_off, uint32_t imm, uint32_t v) {
-  *(g_u32 *)((g_u8 *)as_global(base) + (uint64_t)lane_off + imm) = v;
-}
-__device__ __forceinline__ u32x4 gld4_imm(const uint32_t *base, uint32_t lane_off, uint32_t imm) {
-  return *(gconst_u4 *)((gconst_u8 *)as_global(base) + (uint64_t)lane_off + imm);
-}
-__device__ __forceinline__ void gst4_imm(uint32_t *base, uint32_t lane_off, uint32_t imm, u32x4 v) {
-  *(g_u4 *)((g_u8 *)as_global(base) + (uint64_t)lane_off + imm) = v;
-}
 __device__ __forceinline__ u32x4 gld4(const uint32_t *base, uint32_t byte_off) {
   return *(gconst_u4 *)((gconst_u8 *)as_global(base) + byte_off);
 }
@@ -571,7 +555,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         const uint32_t vo = (gidx(local << 4) & in_mask) << 2;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-          const u32x4 v = gld4_imm(in, vo, 16 * k);
+          const u32x4 v = gld4(in, vo + 16 * k);
           x[4 * k] = v.x;
           x[4 * k + 1] = v.y;
           x[4 * k + 2] = v.z;
@@ -596,8 +580,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         const uint32_t vo = (gb & in_mask) << 2, jmask = in_mask >> sh;
         if (NS == 3) {
 #pragma unroll
-          for (int j = 0; j < 16; j++)
-            x[j] = gld_imm(in + ((j >> 2) << (T_FIXED12 - 2)), vo, (uint32_t)(j & 3) << (T_FIXED12 - 4 + 2));
+          for (int j = 0; j < 16; j++) x[j] = gld(in, vo + ((uint32_t)j << (T_FIXED12 - 4 + 2)));
         } else {
 #pragma unroll
           for (int j = 0; j < 16; j++) x[j] = gld(in, vo + ((((uint32_t)j) & jmask) << (sh + 2)));
@@ -644,7 +627,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           const u32x4 v = {x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]};
-          gst4_imm(out, vo, 16 * k, v);
+          gst4(out, vo + 16 * k, v);
         }
       }
       continue;
@@ -664,8 +647,7 @@ __device__ __forceinline__ void fft_pass_body(const FftPass2 &p, uint32_t wg, ui
         // fixed 12-layer passes end in the top layout (B0C is 0 or 8; 0 was
         // moved there above) with w = lo = 0: sh = 8
 #pragma unroll
-        for (int j = 0; j < 16; j++)
-          gst_imm(out + ((j >> 2) << (T_FIXED12 - 2)), vo, (uint32_t)(j & 3) << (T_FIXED12 - 4 + 2), x[j]);
+        for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (T_FIXED12 - 4 + 2)), x[j]);
       } else {
 #pragma unroll
         for (int j = 0; j < 16; j++) gst(out, vo + ((uint32_t)j << (sh + 2)), x[j]);
@@ -713,37 +695,31 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
 // TreeBuilder::extend_evals (STWO/prover/pcs/mod.rs:718-735) drops them
 // unless store_polynomials_coefficients is set (:347).
 //
-// NS = 2: the 8-layer strided pair on a 2^13 tile (inverse stages at tile
-// bits 5 then 9, forward 9 then 5: the inverse ends in the layout the
-// forward starts in, so the blocks need one LDS exchange each); NS = 0: the
-// stage lists of both descriptors at run time.  Both passes tile the
-// coefficient domain (pf.rep_log = b), and pi.T >= 10 (wave-uniform columns).
-template <bool INV>
-__device__ __forceinline__ void runtime_stages(const FftPass2 &p, uint32_t (&x)[16], uint32_t *L, uint32_t &cur,
-                                               uint32_t local, uint32_t tile_gb) {
-  const uint32_t wmask = (1u << p.w) - 1;
-  auto gidx = [&](uint32_t b) { return tile_gb | ((b >> p.w) << p.lo) | (b & wmask); };
-  auto group_base = [&](uint32_t b0) { return ((local >> b0) << (b0 + 4)) | (local & ((1u << b0) - 1)); };
-  for (uint32_t si = 0; si < p.n_stages; si++) {
-    const FftStage S = p.st[si];
-    const uint32_t base = group_base(S.b0);
-    const uint32_t sh = S.b0 - p.w + p.lo;
-    StageTw t;
-    load_stage_twiddles(t, p.tw_end, p.N, sh, gidx(base), S.r_lo, S.r_hi);
-    if (S.b0 != cur) {
-      __syncthreads();
-      lds_put(L, cur, group_base(cur), x);
-      __syncthreads();
-      lds_get(L, S.b0, base, x);
-      cur = S.b0;
-    }
-    stage_dispatch<INV>(x, t, S.r_lo, S.r_hi);
-  }
-}
+// K is the compile-time schedule both passes share (fixed_schedule's kind):
+// 2 = the 8-layer pair on a 2^13 tile (inverse stages at tile bits 5 then 9,
+// forward 9 then 5); 3 / 4 / 5 = 5 / 6 / 7 strided layers (inverse: the
+// partial group at tile bit 6, then the full one at T - 4; forward the
+// reverse); 6..12 = one group at tile bit 6 with register bits kShort[K - 6].
+// The inverse stages end in the group the forward stages start in, so each
+// output block needs no LDS exchange before its first stage.  Both passes
+// tile the coefficient domain (pf.rep_log = b), and pi.T >= 10 (wave-uniform
+// columns).  A pass pair without such a schedule is not fused (the run-time
+// stage lists made the fused pass slower than the two passes it replaces).
+template <int K>
+struct LdeSchedule {
+  // full group, partial group, partial register bits [0, PART) (kinds 2..5);
+  // short kinds: register bits [RLO, RHI) of the group at tile bit 6
+  static constexpr int FULL = K == 2 ? 9 : K == 3 ? 7 : K == 4 ? 8 : K == 5 ? 9 : 6;
+  static constexpr int PARTB0 = K == 2 ? 5 : 6;
+  static constexpr int PART = K == 2 ? 4 : K == 3 ? 1 : K == 4 ? 2 : K == 5 ? 3 : 0;
+  static constexpr int RLO = K >= 6 ? (K == 7 ? 1 : K == 8 ? 2 : K == 9 ? 3 : 0) : 0;
+  static constexpr int RHI = K >= 6 ? (K <= 9 ? 4 : K == 10 ? 1 : K == 11 ? 2 : 3) : 4;
+};
 
-template <int WG_LOG, int NS>
+template <int WG_LOG, int K>
 __device__ __forceinline__ void lde_pass_body(const FftPass2 &pi, const FftPass2 &pf, uint32_t store_coeffs,
                                               uint32_t wg, uint32_t *lds) {
+  using S = LdeSchedule<K>;
   const uint32_t tpt_log = pi.T - 4;
   const uint32_t local = threadIdx.x & ((1u << tpt_log) - 1);
   const uint32_t slot = threadIdx.x >> tpt_log;
@@ -777,11 +753,11 @@ __device__ __forceinline__ void lde_pass_body(const FftPass2 &pi, const FftPass2
     for (int j = 0; j < 16; j++) x[j] = gld(ev, vo + ((uint32_t)j << (sh_top + 2)));
   }
   uint32_t cur = top_b0;
-  if constexpr (NS == 2) {
-    fixed_stage<true, 5, -1>(pi, x, L, cur, local, group_base, gidx0);
-    fixed_stage<true, 9, 5>(pi, x, L, cur, local, group_base, gidx0);
+  if constexpr (K >= 6) {
+    fixed_stage<true, 6, -1, S::RHI, S::RLO>(pi, x, L, cur, local, group_base, gidx0);
   } else {
-    runtime_stages<true>(pi, x, L, cur, local, tile_gb0);
+    fixed_stage<true, S::PARTB0, -1, S::PART>(pi, x, L, cur, local, group_base, gidx0);
+    fixed_stage<true, S::FULL, S::PARTB0>(pi, x, L, cur, local, group_base, gidx0);
   }
 #pragma unroll
   for (int j = 0; j < 16; j++) x[j] = m31_mul(x[j], pi.scale);
@@ -811,11 +787,11 @@ __device__ __forceinline__ void lde_pass_body(const FftPass2 &pi, const FftPass2
 #pragma unroll
     for (int j = 0; j < 16; j++) y[j] = x[j];
     cur = last_inv;
-    if constexpr (NS == 2) {
-      fixed_stage<false, 9, 9>(pf, y, L, cur, local, group_base, gidx);
-      fixed_stage<false, 5, 9>(pf, y, L, cur, local, group_base, gidx);
+    if constexpr (K >= 6) {
+      fixed_stage<false, 6, 6, S::RHI, S::RLO>(pf, y, L, cur, local, group_base, gidx);
     } else {
-      runtime_stages<false>(pf, y, L, cur, local, tile_gb);
+      fixed_stage<false, S::FULL, S::FULL>(pf, y, L, cur, local, group_base, gidx);
+      fixed_stage<false, S::PARTB0, S::FULL, S::PART>(pf, y, L, cur, local, group_base, gidx);
     }
     if (cur != top_b0 && cur < 5) {
       __syncthreads();
@@ -832,17 +808,17 @@ __device__ __forceinline__ void lde_pass_body(const FftPass2 &pi, const FftPass2
   }
 }
 
-template <int WG_LOG, int NS>
+template <int WG_LOG, int K>
 __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(4)))
 lde_pass_kernel(FftPass2 pi, FftPass2 pf, uint32_t store_coeffs) {
   __shared__ uint32_t lds[(16u << WG_LOG) + (1u << WG_LOG)];
-  lde_pass_body<WG_LOG, NS>(pi, pf, store_coeffs, blockIdx.x, lds);
+  lde_pass_body<WG_LOG, K>(pi, pf, store_coeffs, blockIdx.x, lds);
 }
 
 // The fused pass of several column groups in one launch (the batched path):
 // group g owns workgroups [wg_start[g], wg_start[g + 1]) and the pair
-// (pis[g], pfs[g]); every group matches the kernel's schedule (NS).
-template <int WG_LOG, int NS>
+// (pis[g], pfs[g]); every group matches the kernel's schedule (K).
+template <int WG_LOG, int K>
 __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_eu(4)))
 lde_pass_multi_kernel(const FftPass2 *pis, const FftPass2 *pfs, const uint32_t *wg_start, uint32_t n_groups,
                       uint32_t store_coeffs) {
@@ -853,30 +829,47 @@ lde_pass_multi_kernel(const FftPass2 *pis, const FftPass2 *pfs, const uint32_t *
     if (wg_start[mid] <= blockIdx.x) g = mid; else hi = mid;
   }
   const FftPass2 pi = pis[g], pf = pfs[g];
-  lde_pass_body<WG_LOG, NS>(pi, pf, store_coeffs, blockIdx.x - wg_start[g], lds);
+  lde_pass_body<WG_LOG, K>(pi, pf, store_coeffs, blockIdx.x - wg_start[g], lds);
+}
+
+// Launches of the fused pass by schedule kind (wg9: kinds 2 and 5; wg8: 3, 4
+// and the short kinds).  Profiling names: lde_pass_wg9 / lde_pass_wg8
+// (lde_pass_batched for the multi-group form).
+#define STWO_LDE_KINDS(X) X(9, 2) X(8, 3) X(8, 4) X(9, 5) X(8, 6) X(8, 7) X(8, 8) X(8, 9) X(8, 10) X(8, 11) X(8, 12)
+
+static int launch_lde(stwo_hip_ctx *ctx, int kind, const FftPass2 &pi, const FftPass2 &pf, uint32_t wgs,
+                      uint32_t store_coeffs) {
+#define STWO_LDE_ONE(WG, K) \
+  if (kind == K)            \
+    return launch(ctx, "lde_pass_wg" #WG, lde_pass_kernel<WG, K>, dim3(wgs), dim3(1u << WG), 0, pi, pf, store_coeffs);
+  STWO_LDE_KINDS(STWO_LDE_ONE)
+#undef STWO_LDE_ONE
+  return set_error(ctx, STWO_HIP_EINVAL, "internal: no fused schedule of kind %d", kind);
 }
 
 int launch_lde_multi(stwo_hip_ctx *ctx, const FftPass2 *d_pis, const FftPass2 *d_pfs, const uint32_t *d_wg_start,
                      uint32_t n_groups, uint32_t total_wgs, uint32_t wg_log, int kind, uint32_t store_coeffs) {
-  if (wg_log == 9 && kind == 2)
-    return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<9, 2>, dim3(total_wgs), dim3(512), 0, d_pis, d_pfs,
-                  d_wg_start, n_groups, store_coeffs);
-  if (wg_log == 9)
-    return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<9, 0>, dim3(total_wgs), dim3(512), 0, d_pis, d_pfs,
-                  d_wg_start, n_groups, store_coeffs);
-  return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<8, 0>, dim3(total_wgs), dim3(256), 0, d_pis, d_pfs,
-                d_wg_start, n_groups, store_coeffs);
+  (void)wg_log;  // implied by the kind
+#define STWO_LDE_MULTI(WG, K)                                                                                    \
+  if (kind == K)                                                                                                 \
+    return launch(ctx, "lde_pass_batched", lde_pass_multi_kernel<WG, K>, dim3(total_wgs), dim3(1u << WG), 0, d_pis, \
+                  d_pfs, d_wg_start, n_groups, store_coeffs);
+  STWO_LDE_KINDS(STWO_LDE_MULTI)
+#undef STWO_LDE_MULTI
+  return set_error(ctx, STWO_HIP_EINVAL, "internal: no fused schedule of kind %d", kind);
+}
+
+int lde_kind(const PassLaunch &last_inv, const PassLaunch &first_fwd) {
+  const uint32_t k = last_inv.p.fixed;
+  if (k < 2 || k > 12 || first_fwd.p.fixed != k || last_inv.wg_log != first_fwd.wg_log) return 0;
+  const uint32_t wg = (k == 2 || k == 5) ? 9 : 8;
+  return last_inv.wg_log == wg ? (int)k : 0;
 }
 
 bool lde_fusable(const PassLaunch &last_inv, const PassLaunch &first_fwd, uint32_t log_blowup) {
   const FftPass2 &li = last_inv.p, &ff = first_fwd.p;
   return li.T == ff.T && li.lo == ff.lo && li.w == ff.w && li.m == ff.m && li.lo > 0 && li.T >= 10 &&
-         ff.rep_log == log_blowup && last_inv.wg_log == first_fwd.wg_log && last_inv.wgs == first_fwd.wgs &&
-         (last_inv.wg_log == 8 || last_inv.wg_log == 9);
-}
-
-int lde_kind(const PassLaunch &last_inv, const PassLaunch &first_fwd) {
-  return (last_inv.wg_log == 9 && last_inv.p.fixed == 2 && first_fwd.p.fixed == 2) ? 2 : 0;
+         ff.rep_log == log_blowup && last_inv.wgs == first_fwd.wgs && lde_kind(last_inv, first_fwd) != 0;
 }
 
 // Stages covering layer tile bits [A, B) of a tile of T bits, forward order.
@@ -1216,14 +1209,9 @@ static int run_lde_group(stwo_hip_ctx *ctx, const std::vector<FftCol> &cols, con
     if (rc) return rc;
   }
   if (fuse) {
-    const uint32_t wgs = inv.back().wgs, wg_log = inv.back().wg_log;
+    const uint32_t wgs = inv.back().wgs;
     const uint32_t sc = store_coeffs ? 1 : 0;
-    if (lde_kind(inv.back(), fwd.front()) == 2)
-      rc = launch(ctx, "lde_pass_wg9", lde_pass_kernel<9, 2>, dim3(wgs), dim3(512), 0, li, ff, sc);
-    else if (wg_log == 9)
-      rc = launch(ctx, "lde_pass_wg9", lde_pass_kernel<9, 0>, dim3(wgs), dim3(512), 0, li, ff, sc);
-    else
-      rc = launch(ctx, "lde_pass_wg8", lde_pass_kernel<8, 0>, dim3(wgs), dim3(256), 0, li, ff, sc);
+    rc = launch_lde(ctx, lde_kind(inv.back(), fwd.front()), li, ff, wgs, sc);
     if (rc) return rc;
   }
   for (size_t k = fuse ? 1 : 0; k < fwd.size(); k++) {

@@ -16,9 +16,11 @@
 // the formula (the reference's batch inversion included).
 //
 // barycentric_eval_at_point (CPUB/circle.rs:134-141): sum_i evals[i] * w_i,
-// batched over columns of one size: a workgroup takes a 2^11 block of up to
-// 8 columns, each lane 8 consecutive rows, with 64-bit accumulation per
-// limb; a second kernel sums each column's block partials.
+// batched over columns of one size: a workgroup takes a 2^11-row block of
+// up to 32 columns, each lane 8 consecutive rows (16-byte loads when the
+// columns allow); it loads the block's weights once and runs the columns 8 at
+// a time, with 64-bit accumulation per limb and an LDS reduction per column;
+// a second kernel sums each column's block partials.
 #include <string.h>
 
 #include <algorithm>
@@ -30,7 +32,8 @@
 namespace stwo_hip {
 
 constexpr uint32_t kBaryBlockLog = 11;
-constexpr uint32_t kBaryCols = 8;
+constexpr uint32_t kBaryCols = 8;     // columns per inner step
+constexpr uint32_t kBaryWgCols = 32;  // columns per workgroup (the weights are loaded once for them)
 
 struct BaryPointArgs {
   uint32_t n;              // log size of the domain
@@ -77,7 +80,10 @@ struct BaryWork {
   uint32_t col0, n, block;
 };
 
-// Block `w.block` (2^B rows, B = min(log, 11)) of columns col0 .. col0 + n - 1.
+// Block `w.block` (2^B rows, B = min(log, 11)) of columns col0 .. col0 + n - 1
+// (n <= kBaryWgCols).  VEC: every column and weight array is 16-byte aligned
+// and B >= 3, so a lane's 8 rows are two 16-byte loads.
+template <bool VEC>
 __global__ void __launch_bounds__(256) barycentric_dot_kernel(const uint32_t *const *cols, const BaryWork *work,
                                                               uint32_t log_size, const uint32_t *w0,
                                                               const uint32_t *w1, const uint32_t *w2,
@@ -90,47 +96,70 @@ __global__ void __launch_bounds__(256) barycentric_dot_kernel(const uint32_t *co
   const uint32_t nb = 1u << B;
   const uint32_t r0 = (wk.block << B) + threadIdx.x * 8;
   const uint32_t valid = threadIdx.x * 8 < nb ? min(8u, nb - threadIdx.x * 8) : 0u;
+  auto load8 = [&](const uint32_t *src, uint32_t (&v)[8]) {
+    if (VEC) {
+      if (valid) {
+        const uint4 a = *(const uint4 *)(src + r0), b = *(const uint4 *)(src + r0 + 4);
+        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = 0;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) v[e] = (uint32_t)e < valid ? src[r0 + e] : 0u;
+    }
+  };
   uint32_t w[4][8];
-  const uint32_t *ws[4] = {w0, w1, w2, w3};
+  load8(w0, w[0]);
+  load8(w1, w[1]);
+  load8(w2, w[2]);
+  load8(w3, w[3]);
+  for (uint32_t g0 = 0; g0 < wk.n; g0 += kBaryCols) {
+    const uint32_t gn = min(kBaryCols, wk.n - g0);
+    uint32_t cv[kBaryCols][8];
 #pragma unroll
-  for (int k = 0; k < 4; k++)
+    for (uint32_t j = 0; j < kBaryCols; j++) {
+      if (j < gn) {
+        load8(cols[wk.col0 + g0 + j], cv[j]);
+      } else {
 #pragma unroll
-    for (int e = 0; e < 8; e++) w[k][e] = (uint32_t)e < valid ? ws[k][r0 + e] : 0u;
-  uint32_t cv[kBaryCols][8];
-#pragma unroll
-  for (uint32_t j = 0; j < kBaryCols; j++)
-#pragma unroll
-    for (int e = 0; e < 8; e++) cv[j][e] = (j < wk.n && (uint32_t)e < valid) ? cols[wk.col0 + j][r0 + e] : 0u;
-#pragma unroll
-  for (uint32_t j = 0; j < kBaryCols; j++) {
-    if (j >= wk.n) break;
-    uint64_t acc[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-#pragma unroll
-      for (int k = 0; k < 4; k++) acc[k] += (uint64_t)cv[j][e] * w[k][e];
-      if (e == 2 || e == 5)
-#pragma unroll
-        for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k]);
+        for (int e = 0; e < 8; e++) cv[j][e] = 0;
+      }
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) red[4 * j + k][threadIdx.x] = m31_reduce64(acc[k]);
-  }
-  __syncthreads();
-  const uint32_t r = threadIdx.x >> 3, seg = threadIdx.x & 7;
-  if (r < 4 * wk.n) {
-    uint64_t s = 0;
+    for (uint32_t j = 0; j < kBaryCols; j++) {
+      if (j >= gn) break;
+      uint64_t acc[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < 32; i++) s += red[r][seg * 32 + i];
-    uint32_t v = m31_reduce64(s);
+      for (int e = 0; e < 8; e++) {
 #pragma unroll
-    for (int o = 1; o < 8; o <<= 1) v = m31_add(v, __shfl_xor(v, o, 8));
-    if (seg == 0) sums[r] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < 4 * wk.n) {
-    const uint32_t j = threadIdx.x >> 2, k = threadIdx.x & 3;
-    partials[4 * ((size_t)(wk.col0 + j) * n_blocks + wk.block) + k] = sums[threadIdx.x];
+        for (int k = 0; k < 4; k++) acc[k] += (uint64_t)cv[j][e] * w[k][e];
+        if (e == 2 || e == 5)
+#pragma unroll
+          for (int k = 0; k < 4; k++) acc[k] = m31_fold64(acc[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) red[4 * j + k][threadIdx.x] = m31_reduce64(acc[k]);
+    }
+    __syncthreads();
+    const uint32_t r = threadIdx.x >> 3, seg = threadIdx.x & 7;
+    if (r < 4 * gn) {
+      uint64_t sacc = 0;
+#pragma unroll
+      for (int i = 0; i < 32; i++) sacc += red[r][seg * 32 + i];
+      uint32_t v = m31_reduce64(sacc);
+#pragma unroll
+      for (int o = 1; o < 8; o <<= 1) v = m31_add(v, __shfl_xor(v, o, 8));
+      if (seg == 0) sums[r] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 * gn) {
+      const uint32_t j = threadIdx.x >> 2, k = threadIdx.x & 3;
+      partials[4 * ((size_t)(wk.col0 + g0 + j) * n_blocks + wk.block) + k] = sums[threadIdx.x];
+    }
+    // the next group rewrites red and sums only after every lane passed here
+    __syncthreads();
   }
 }
 
@@ -271,8 +300,12 @@ int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const 
   for (size_t c = 0; c < n_cols; c++) STWO_HIP_CHECK_ARG(ctx, evals[c], "barycentric_eval_at_point: NULL column");
   const uint32_t n_blocks = 1u << (log_size > kBaryBlockLog ? log_size - kBaryBlockLog : 0);
   std::vector<BaryWork> work;
-  for (uint32_t c = 0; c < n_cols; c += kBaryCols)
-    for (uint32_t b = 0; b < n_blocks; b++) work.push_back({c, (uint32_t)std::min<size_t>(kBaryCols, n_cols - c), b});
+  for (uint32_t c = 0; c < n_cols; c += kBaryWgCols)
+    for (uint32_t b = 0; b < n_blocks; b++)
+      work.push_back({c, (uint32_t)std::min<size_t>(kBaryWgCols, n_cols - c), b});
+  bool vec = log_size >= 3;
+  for (int k = 0; k < 4; k++) vec = vec && ((uintptr_t)weights[k] & 15) == 0;
+  for (size_t c = 0; c < n_cols; c++) vec = vec && ((uintptr_t)evals[c] & 15) == 0;
   void *d[2];
   const void *srcs[2] = {evals, work.data()};
   const size_t bytes[2] = {n_cols * sizeof(uint32_t *), work.size() * sizeof(BaryWork)};
@@ -283,7 +316,8 @@ int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const 
   if (rc) return rc;
   rc = get_scratch(ctx, "bary_out", n_cols * 16, &out);
   if (rc) return rc;
-  rc = launch(ctx, "barycentric_dot", barycentric_dot_kernel, dim3((uint32_t)work.size()), dim3(256), 0,
+  rc = launch(ctx, "barycentric_dot", vec ? barycentric_dot_kernel<true> : barycentric_dot_kernel<false>,
+              dim3((uint32_t)work.size()), dim3(256), 0,
               (const uint32_t *const *)d[0], (const BaryWork *)d[1], log_size, weights[0], weights[1], weights[2],
               weights[3], n_blocks, (uint32_t *)part);
   if (rc) return rc;

@@ -1,11 +1,12 @@
 """stwo_hip_extend_evals (interpolate_columns + evaluate_polynomials with
 the IFFT's last pass and the RFFT's first pass fused into one kernel)
 against the oracle, bit-exact, over every pass geometry the fusion meets:
-no fusion (n <= 12: the low pass alone), the run-time-schedule fused pass
-(n = 13..19: one strided pass of 1..7 layers), the compile-time 8-layer
-fused pass (n = 20; n >= 21, where the fused pair is the top 8-layer pass),
-blowups 0..3, with and without keeping the coefficients; and against the
-library's own two-call path on the config-2 shape."""
+no fusion (n <= 12: the low pass alone), the fused pass on each
+compile-time strided schedule (n = 13..16: one strided pass of 1..4 layers
+on a 2^10 tile; 17, 18, 19: 5, 6, 7 layers; 20: the 8-layer pair; n >= 21,
+where the fused pair is the top 8-layer pass), blowups 0..3, with and
+without keeping the coefficients; and against the library's own two-call
+path on the config-2 shape."""
 import os
 import sys
 
@@ -27,7 +28,7 @@ def ctx():
     c.close()
 
 
-SHAPES = [(4, 1, 3), (10, 2, 5), (12, 1, 4), (13, 1, 3), (14, 2, 2), (16, 1, 4), (16, 0, 2), (17, 3, 2),
+SHAPES = [(4, 1, 3), (10, 2, 5), (12, 1, 4), (13, 1, 3), (14, 2, 2), (15, 1, 3), (16, 1, 4), (16, 0, 2), (17, 3, 2),
           (18, 1, 2), (19, 2, 2), (20, 2, 3), (21, 1, 2), (22, 1, 1), (23, 2, 1)]
 
 

@@ -48,17 +48,23 @@ def test_barycentric_weights_vs_oracle(ctx, n):
         o.free()
 
 
-@pytest.mark.parametrize("n,C", [(1, 1), (3, 5), (10, 9), (11, 3), (12, 17), (16, 2), (20, 3)])
-def test_barycentric_eval_matches_eval_at_point(ctx, n, C):
+@pytest.mark.parametrize("n,C,offset", [(1, 1, 0), (3, 5, 0), (10, 9, 0), (11, 3, 0), (12, 17, 0), (16, 2, 0),
+                                         (20, 3, 0), (13, 70, 0), (12, 11, 1)])
+def test_barycentric_eval_matches_eval_at_point(ctx, n, C, offset):
     """Weights and dot products on the device; the values against the C
-    oracle's eval_at_point of each column's coefficients (columns grouped
-    8 to a workgroup, with a remainder; one column all P - 1)."""
+    oracle's eval_at_point of each column's coefficients (columns 32 to a
+    workgroup in steps of 8, with remainders; one column all P - 1; with
+    offset 1 the columns start one word into their buffers, off the 16-byte
+    loads)."""
     rng = np.random.default_rng(n * 31 + C)
     coeffs = [rng.integers(0, P, 1 << n, dtype=np.uint64).astype(np.uint32) for _ in range(C)]
     if C > 2:
         coeffs[2][:] = P - 1
     evs = [evals_of(c, n) for c in coeffs]
-    d = [ctx.to_device(e) for e in evs]
+    bufs = [ctx.alloc((1 << n) + offset) for _ in evs]
+    for b_, e in zip(bufs, evs):
+        b_.upload(e, offset=offset)
+    d = [b_.at(offset) for b_ in bufs]
     w = [ctx.alloc(1 << n) for _ in range(4)]
     for k in (5, 0xBEEF):
         pt = secure_point(k)
@@ -66,7 +72,7 @@ def test_barycentric_eval_matches_eval_at_point(ctx, n, C):
         got = ctx.barycentric_eval_at_point(d, n, w)
         for c, g in zip(coeffs, got):
             assert tuple(int(v) for v in g) == O.eval_at_point(c, pt)
-    for x in d + w:
+    for x in bufs + w:
         x.free()
 
 

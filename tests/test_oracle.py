@@ -107,6 +107,19 @@ def test_twiddles_golden(golden):
         assert itw.tolist() == case["itwiddles"], case["kind"]
 
 
+@pytest.mark.parametrize("log_size", [0, 1, 2, 3, 7, 12])
+def test_bit_reverse_closed_form(log_size):
+    """bit_reverse (STWO/core/utils.rs:91-101) is the permutation
+    v'[i] = v[rev(i)] with rev reversing log_size bits (written here from
+    the binary digits), and an involution."""
+    n = 1 << log_size
+    v = np.arange(1000, 1000 + n, dtype=np.uint32)
+    got = oracle.bit_reverse(v.copy())
+    rev = [int(format(i, "0%db" % log_size)[::-1], 2) if log_size else 0 for i in range(n)]
+    assert got.tolist() == [int(v[r]) for r in rev]
+    assert np.array_equal(oracle.bit_reverse(got.copy()), v)
+
+
 def test_twiddle_inverse_invariant():
     tw, itw = oracle.precompute_twiddles(oracle.half_odds_initial(13), 13)
     assert np.all((tw.astype(np.uint64) * itw.astype(np.uint64)) % P == 1)
