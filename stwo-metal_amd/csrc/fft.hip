@@ -679,7 +679,9 @@ __global__ void __launch_bounds__(1 << WG_LOG) __attribute__((amdgpu_waves_per_e
     const uint32_t mid = (g + hi) >> 1;
     if (wg_start[mid] <= blockIdx.x) g = mid; else hi = mid;
   }
-  const FftPass2 p = groups[g];
+  // Read in place (wave-uniform scalar loads): a private copy would put the
+  // run-time stage list in scratch memory.
+  const FftPass2 &p = groups[g];
   fft_pass_body<INV, WG_LOG, NS, B0A, B0B, B0C>(p, blockIdx.x - wg_start[g], lds);
 }
 
@@ -828,7 +830,7 @@ lde_pass_multi_kernel(const FftPass2 *pis, const FftPass2 *pfs, const uint32_t *
     const uint32_t mid = (g + hi) >> 1;
     if (wg_start[mid] <= blockIdx.x) g = mid; else hi = mid;
   }
-  const FftPass2 pi = pis[g], pf = pfs[g];
+  const FftPass2 &pi = pis[g], &pf = pfs[g];
   lde_pass_body<WG_LOG, K>(pi, pf, store_coeffs, blockIdx.x - wg_start[g], lds);
 }
 

@@ -10,6 +10,7 @@ boundary.  There is no CPU fallback: if the library is missing or no GPU is
 visible, loading fails with an exception.
 """
 import ctypes
+import time
 import os
 
 import numpy as np
@@ -576,12 +577,16 @@ class Context:
         h = _vp()
         pa = a.ptr if isinstance(a, DeviceBuf) else a
         pb = b.ptr if isinstance(b, DeviceBuf) else b
+        t0 = time.perf_counter()
         if store_coeffs:
             self._check(lib().stwo_hip_wide_fib_prove(self.h, pa, pb, n_cols, log_n, cfg, tw.h, d,
                                                       nd.ctypes.data_as(_u32p), ctypes.byref(h)))
         else:
             self._check(lib().stwo_hip_wide_fib_prove_ex(self.h, pa, pb, n_cols, log_n, cfg, tw.h, 0, d,
                                                          nd.ctypes.data_as(_u32p), ctypes.byref(h)))
+        # wall time of the prove call itself (the proof object on the host,
+        # before serialization)
+        self.last_prove_ms = (time.perf_counter() - t0) * 1e3
         try:
             n = ctypes.c_size_t()
             self._check(lib().stwo_hip_stark_proof_bytes(h, None, 0, ctypes.byref(n)))
