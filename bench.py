@@ -548,6 +548,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-prove", action="store_true")
+    ap.add_argument("--no-percol", action="store_true", help="skip the per-column form of the headline workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
@@ -625,7 +626,7 @@ def main():
     # interpolate and one evaluate launch sequence per column, then the
     # commit: stwo_hip_lde_commit_per_column), timed beside the headline.
     pc = []
-    for i in range(3):
+    for i in range(0 if args.no_percol else 3):
         for w, p in zip(work, pristine):
             ctx._check(sh.lib().stwo_hip_copy(ctx.h, w.ptr, p.ptr, w.n))
         ctx.sync()
@@ -633,10 +634,12 @@ def main():
         ctx.lde_commit_per_column(work, logs, LOG_BLOWUP, tw, outs, layers, sh.TREE_MIXED)
         ctx.sync()
         pc.append(time.perf_counter() - t0)
-    per_column = {"form": "per column: stwo_hip_interpolate + stwo_hip_evaluate per column, then the commit",
-                  "ms": round(min(pc) * 1e3, 3), "melem_s": round(elems / min(pc) / 1e6, 1),
-                  "root_equal": bool(np.array_equal(layers.download(8), gpu_root))}
-    log("per-column form: %s" % json.dumps(per_column))
+    per_column = None
+    if pc:
+        per_column = {"form": "per column: stwo_hip_interpolate + stwo_hip_evaluate per column, then the commit",
+                      "ms": round(min(pc) * 1e3, 3), "melem_s": round(elems / min(pc) / 1e6, 1),
+                      "root_equal": bool(np.array_equal(layers.download(8), gpu_root))}
+        log("per-column form: %s" % json.dumps(per_column))
 
     serve = None
     batched = None
