@@ -90,7 +90,8 @@ def batched_vs_plain(ctx, steps=3):
     interpolate and one evaluate per column, then the commit) against one
     batched plan replayed as a hipGraph (coefficients not stored, the PCS
     default), with the planner's defaults and after stwo_hip_fft_autotune
-    timed every candidate plan of the set's 26 FFT shapes on the device."""
+    timed every candidate plan of the set's 26 FFT shapes on the device; the
+    tuned plan also issued launch by launch.  batched_ms is the fastest."""
     logs = uneven_logs()
     b = 1
     max_log = max(logs) + b
@@ -140,6 +141,11 @@ def batched_vs_plain(ctx, steps=3):
                             drop_coeffs=True)
     tt, root_tuned = timed(plan.run)
     plan.destroy()
+    # the same plan issued launch by launch instead of replayed as a graph
+    plan = ctx.batch_create(cols, logs, b, tw, out_cols, layers, sh.TREE_MIXED, sh.HASH_BLAKE2S, False,
+                            drop_coeffs=True)
+    te, root_eager = timed(plan.run)
+    plan.destroy()
     ctx.fft_tune_clear()
     for x in (pristine, work, outs, layers):
         x.free()
@@ -147,7 +153,7 @@ def batched_vs_plain(ctx, steps=3):
     hist = {str(l): logs.count(l) for l in sorted(set(logs))}
     changed = [{"n": t[0], "N": t[1], "inverse": t[2], "t0": t[3], "smax": t[4], "wg": t[5], "best_us": t[6],
                 "default_us": t[7]} for t in tuned if (t[3], t[4], t[5]) != (12, 8, 0)]
-    best = min(tb, tt)
+    best = min(tb, tt, te)
     return {
         "workload": "1,024 M31 columns, log_size 10..22 (P(10+j) ~ 0.7^j, seed 3), LDE blowup 2, one "
                     "mixed-degree Blake2s tree",
@@ -158,13 +164,15 @@ def batched_vs_plain(ctx, steps=3):
         "batched_ms": round(best * 1e3, 3),
         "batched_default_ms": round(tb * 1e3, 3),
         "batched_tuned_ms": round(tt * 1e3, 3),
+        "batched_tuned_eager_ms": round(te * 1e3, 3),
         "autotune_s": round(tune_s, 2),
         "autotune_shapes": len(tuned),
         "autotune_changed": changed,
         "plain_melem_s": round(elems / tp / 1e6, 1),
         "batched_melem_s": round(elems / best / 1e6, 1),
         "batched_over_plain": round(tp / best, 2),
-        "roots_equal": bool(np.array_equal(root_plain, root_batched) and np.array_equal(root_plain, root_tuned)),
+        "roots_equal": bool(np.array_equal(root_plain, root_batched) and np.array_equal(root_plain, root_tuned)
+                            and np.array_equal(root_plain, root_eager)),
     }
 
 

@@ -130,6 +130,87 @@ __global__ void __launch_bounds__(256) quotient_kernel(const QuotPoint *pts, uin
   o3[row] = q.v[3];
 }
 
+// Every column at the lifting size (no lifting) and 8-byte aligned: a lane
+// takes rows 2t and 2t + 1, one 8-byte load per entry.  Their bit-reversed
+// indices are i and i + 2^(L-1), so the second point is the conjugate
+// (p.x, -p.y) of the first: one point computation, and one M31 inversion per
+// sample point for both denominators (1/a = b/(ab), 1/b = a/(ab)).
+__global__ void __launch_bounds__(256) quotient_pair_kernel(const QuotPoint *pts, uint32_t n_pts,
+                                                            const QuotEntry *ents, uint32_t L, GenTable gen,
+                                                            uint32_t *o0, uint32_t *o1, uint32_t *o2, uint32_t *o3) {
+  const uint32_t row = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (row >> L) return;
+  const uint32_t i = bitrev32(row, L);  // below 2^(L-1): row is even
+  const uint32_t half = 1u << (L - 1);
+  const uint32_t hstep = coset_step(L - 1);
+  const uint32_t idx = (half_odds_initial(L - 1) + hstep * (i & (half - 1))) & IDX_MASK;
+  cpoint p = {1, 0};
+  for (uint32_t k = 0; k < 31; k++)
+    if (idx >> k & 1) p = cp_add(p, gen.pw[k]);
+  const uint32_t py[2] = {p.y, m31_neg(p.y)};
+  qm31 q[2] = {qm31_make(0, 0, 0, 0), qm31_make(0, 0, 0, 0)};
+  for (uint32_t s = 0; s < n_pts; s++) {
+    const QuotPoint z = pts[s];
+    uint32_t d0[2], d1[2], nrm[2];
+#pragma unroll
+    for (int rr = 0; rr < 2; rr++) {
+      const uint32_t dx0 = m31_sub(z.prx[0], p.x), dx1 = z.prx[1];
+      const uint32_t dy0 = m31_sub(z.pry[0], py[rr]), dy1 = z.pry[1];
+      d0[rr] = m31_sub(cm_re_mul(dx0, dx1, z.piy[0], z.piy[1]), cm_re_mul(dy0, dy1, z.pix[0], z.pix[1]));
+      d1[rr] = m31_sub(m31_add(m31_mul(dx0, z.piy[1]), m31_mul(dx1, z.piy[0])),
+                       m31_add(m31_mul(dy0, z.pix[1]), m31_mul(dy1, z.pix[0])));
+      nrm[rr] = m31_add(m31_mul(d0[rr], d0[rr]), m31_mul(d1[rr], d1[rr]));
+    }
+    const uint32_t inv01 = m31_inv(m31_mul(nrm[0], nrm[1]));
+    const uint32_t ninv[2] = {m31_mul(inv01, nrm[1]), m31_mul(inv01, nrm[0])};
+    uint64_t acc[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    uint32_t e = z.e_begin;
+    for (; e + 6 <= z.e_end; e += 6) {
+      uint2 v[6];
+#pragma unroll
+      for (int u = 0; u < 6; u++) v[u] = *reinterpret_cast<const uint2 *>(ents[e + u].col + row);
+#pragma unroll
+      for (int u = 0; u < 6; u++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          acc[0][k] += (uint64_t)v[u].x * ents[e + u].c[k];
+          acc[1][k] += (uint64_t)v[u].y * ents[e + u].c[k];
+        }
+        if (u == 2 || u == 5)
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            acc[0][k] = m31_fold64(acc[0][k]);
+            acc[1][k] = m31_fold64(acc[1][k]);
+          }
+      }
+    }
+    for (; e < z.e_end; e++) {
+      const uint2 v = *reinterpret_cast<const uint2 *>(ents[e].col + row);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        acc[0][k] = m31_fold64(acc[0][k] + (uint64_t)v.x * ents[e].c[k]);
+        acc[1][k] = m31_fold64(acc[1][k] + (uint64_t)v.y * ents[e].c[k]);
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 2; rr++) {
+      qm31 num =
+          qm31_make(m31_reduce64(acc[rr][0]), m31_reduce64(acc[rr][1]), m31_reduce64(acc[rr][2]), m31_reduce64(acc[rr][3]));
+      num = qm31_sub(num, qm31_make(z.bsum[0], z.bsum[1], z.bsum[2], z.bsum[3]));
+      num = qm31_sub(num, qm31_mul_m31(qm31_make(z.first[0], z.first[1], z.first[2], z.first[3]), py[rr]));
+      const uint32_t i0 = m31_mul(d0[rr], ninv[rr]), i1 = m31_mul(m31_neg(d1[rr]), ninv[rr]);
+      uint32_t r0, r1, r2, r3;
+      cm31_mul(num.v[0], num.v[1], i0, i1, r0, r1);
+      cm31_mul(num.v[2], num.v[3], i0, i1, r2, r3);
+      q[rr] = qm31_add(q[rr], qm31_make(r0, r1, r2, r3));
+    }
+  }
+  *reinterpret_cast<uint2 *>(o0 + row) = make_uint2(q[0].v[0], q[1].v[0]);
+  *reinterpret_cast<uint2 *>(o1 + row) = make_uint2(q[0].v[1], q[1].v[1]);
+  *reinterpret_cast<uint2 *>(o2 + row) = make_uint2(q[0].v[2], q[1].v[2]);
+  *reinterpret_cast<uint2 *>(o3 + row) = make_uint2(q[0].v[3], q[1].v[3]);
+}
+
 static qm31 conj(const qm31 &v) { return qm31_make(v.v[0], v.v[1], m31_neg(v.v[2]), m31_neg(v.v[3])); }
 static bool eq(const qm31 &a, const qm31 &b) { return memcmp(a.v, b.v, 16) == 0; }
 static qm31 from(const uint32_t v[4]) { return qm31_make(v[0], v[1], v[2], v[3]); }
@@ -253,10 +334,15 @@ int stwo_hip_compute_fri_quotients(stwo_hip_ctx *ctx, const uint32_t *const *col
     gen.pw[k] = cur;
     cur = cp_add(cur, cur);
   }
-  const uint64_t n = 1ull << L;
-  return launch(ctx, "fri_quotients", quotient_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
-                (const QuotPoint *)d_pts, (uint32_t)pts.size(), (const QuotEntry *)d_ents, L, gen, out[0], out[1],
-                out[2], out[3]);
+  // Two rows per lane when no column is lifted and every column and output is
+  // 8-byte aligned.
+  bool pairs = L >= 1;
+  for (const QuotEntry &q : ents) pairs = pairs && q.shift == 0 && ((uintptr_t)q.col & 7) == 0;
+  for (int k = 0; k < 4; k++) pairs = pairs && ((uintptr_t)out[k] & 7) == 0;
+  const uint64_t n = (1ull << L) >> (pairs ? 1 : 0);
+  return launch(ctx, "fri_quotients", pairs ? quotient_pair_kernel : quotient_kernel,
+                dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (const QuotPoint *)d_pts, (uint32_t)pts.size(),
+                (const QuotEntry *)d_ents, L, gen, out[0], out[1], out[2], out[3]);
 }
 
 }  // extern "C"

@@ -64,43 +64,71 @@ __device__ __forceinline__ void wf_fold(uint64_t (&a1)[4], uint64_t (&a2)[4]) {
     a2[k] = m31_fold64(a2[k]);
   }
 }
+// R consecutive rows per lane (R = 2: one 8-byte load per column, which
+// doubles the bytes each load instruction carries).
+template <int R>
 __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t *const *cols, uint32_t n_cols,
                                                                  uint32_t eval_log, uint32_t log_n,
                                                                  const uint32_t *weights, const uint32_t *denom_inv,
                                                                  uint32_t *acc0, uint32_t *acc1, uint32_t *acc2,
                                                                  uint32_t *acc3, uint32_t accumulate) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >> eval_log) return;
-  uint64_t a1[4] = {0, 0, 0, 0}, a2[4] = {0, 0, 0, 0};
+  const uint32_t r0 = (blockIdx.x * blockDim.x + threadIdx.x) * R;
+  if (r0 >> eval_log) return;
+  uint64_t a1[R][4], a2[R][4];
+#pragma unroll
+  for (int rr = 0; rr < R; rr++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) a1[rr][k] = a2[rr][k] = 0;
+  auto load = [&](const uint32_t *c, uint32_t (&x)[R]) {
+    if constexpr (R == 2) {
+      const uint2 v = *reinterpret_cast<const uint2 *>(c + r0);
+      x[0] = v.x;
+      x[1] = v.y;
+    } else {
+      x[0] = as_global(c)[r0];
+    }
+  };
   uint32_t i = 0;
   for (; i + 8 <= n_cols; i += 8) {
-    uint32_t x[8];
+    uint32_t x[8][R];
 #pragma unroll
-    for (int u = 0; u < 8; u++) x[u] = as_global(cols[i + u])[r];
+    for (int u = 0; u < 8; u++) load(cols[i + u], x[u]);
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      wf_column(a1, a2, x[u], weights + 8 * (i + u));
-      if (u == 2 || u == 5 || u == 7) wf_fold(a1, a2);
+#pragma unroll
+      for (int rr = 0; rr < R; rr++) {
+        wf_column(a1[rr], a2[rr], x[u][rr], weights + 8 * (i + u));
+        if (u == 2 || u == 5 || u == 7) wf_fold(a1[rr], a2[rr]);
+      }
     }
   }
   for (; i < n_cols; i++) {
-    wf_column(a1, a2, as_global(cols[i])[r], weights + 8 * i);
-    wf_fold(a1, a2);
-  }
-  qm31 res;
+    uint32_t x[R];
+    load(cols[i], x);
 #pragma unroll
-  for (int k = 0; k < 4; k++) res.v[k] = m31_add(m31_reduce64(a1[k]), m31_reduce64(a2[k]));
-  res = qm31_mul_m31(res, denom_inv[r >> log_n]);
-  if (accumulate) {
-    acc0[r] = m31_add(acc0[r], res.v[0]);
-    acc1[r] = m31_add(acc1[r], res.v[1]);
-    acc2[r] = m31_add(acc2[r], res.v[2]);
-    acc3[r] = m31_add(acc3[r], res.v[3]);
-  } else {
-    acc0[r] = res.v[0];
-    acc1[r] = res.v[1];
-    acc2[r] = res.v[2];
-    acc3[r] = res.v[3];
+    for (int rr = 0; rr < R; rr++) {
+      wf_column(a1[rr], a2[rr], x[rr], weights + 8 * i);
+      wf_fold(a1[rr], a2[rr]);
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < R; rr++) {
+    const uint32_t r = r0 + rr;
+    qm31 res;
+#pragma unroll
+    for (int k = 0; k < 4; k++) res.v[k] = m31_add(m31_reduce64(a1[rr][k]), m31_reduce64(a2[rr][k]));
+    res = qm31_mul_m31(res, denom_inv[r >> log_n]);
+    if (accumulate) {
+      acc0[r] = m31_add(acc0[r], res.v[0]);
+      acc1[r] = m31_add(acc1[r], res.v[1]);
+      acc2[r] = m31_add(acc2[r], res.v[2]);
+      acc3[r] = m31_add(acc3[r], res.v[3]);
+    } else {
+      acc0[r] = res.v[0];
+      acc1[r] = res.v[1];
+      acc2[r] = res.v[2];
+      acc3[r] = res.v[3];
+    }
   }
 }
 
@@ -162,10 +190,15 @@ int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const 
     d_pw = d[1];
     d_dinv = d[2];
   }
-  const uint64_t n = 1ull << eval_log;
-  return launch(ctx, "wide_fib_constraint_quotients", wide_fib_quotients_kernel, dim3((uint32_t)((n + 255) / 256)),
-                dim3(256), 0, (const uint32_t *const *)d_cols, n_cols, eval_log, log_n, (const uint32_t *)d_pw,
-                (const uint32_t *)d_dinv, accum[0], accum[1], accum[2], accum[3], (uint32_t)accumulate);
+  // Two rows per lane when every column is 8-byte aligned (eval_log >= 2).
+  bool pairs = true;
+  for (uint32_t k = 0; k < n_cols; k++) pairs = pairs && ((uintptr_t)cols[k] & 7) == 0;
+  const uint64_t n = (1ull << eval_log) >> (pairs ? 1 : 0);
+  return launch(ctx, "wide_fib_constraint_quotients",
+                pairs ? wide_fib_quotients_kernel<2> : wide_fib_quotients_kernel<1>,
+                dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (const uint32_t *const *)d_cols, n_cols, eval_log,
+                log_n, (const uint32_t *)d_pw, (const uint32_t *)d_dinv, accum[0], accum[1], accum[2], accum[3],
+                (uint32_t)accumulate);
 }
 
 }  // namespace stwo_hip

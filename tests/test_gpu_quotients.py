@@ -160,10 +160,12 @@ def test_eval_at_point_unaligned_columns(ctx):
         b.free()
 
 
-@pytest.mark.parametrize("log_n,n_cols", [(5, 8), (12, 100), (16, 256)])
-def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols):
+@pytest.mark.parametrize("log_n,n_cols,offset", [(5, 8, 0), (12, 100, 0), (16, 256, 0), (9, 13, 1)])
+def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols, offset):
     """Trace generation and the constraint quotients, bit-exact against the
-    oracle; the GPU's quotients are low degree (checked at log 5)."""
+    oracle (two rows per lane; with offset 1 the extended columns start one
+    word into their buffers, one row per lane); the GPU's quotients are low
+    degree (checked at log 5)."""
     rng = np.random.default_rng(log_n)
     a = rng.integers(0, P, 1 << log_n, dtype=np.uint64).astype(U)
     b = rng.integers(0, P, 1 << log_n, dtype=np.uint64).astype(U)
@@ -178,7 +180,10 @@ def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols):
     coeffs = [t.copy() for t in tr]
     O.interpolate_columns(coeffs, itw)
     ext = O.evaluate_polynomials(coeffs, 1, tw)
-    dext = [ctx.to_device(e) for e in ext]
+    ebufs = [ctx.alloc(len(e) + offset) for e in ext]
+    for bf, e in zip(ebufs, ext):
+        bf.upload(e, offset=offset)
+    dext = [bf.at(offset) for bf in ebufs]
     base = [rng.integers(0, P, 2 << log_n, dtype=np.uint64).astype(U) for _ in range(4)]
     acc = [ctx.to_device(x) for x in base]
     ctx.wide_fib_constraint_quotients(dext, log_n, ALPHA, acc)
@@ -192,5 +197,5 @@ def test_wide_fib_trace_and_constraints_vs_oracle(ctx, log_n, n_cols):
         for z in zero:
             assert not interp(z.download(), log_n + 1)[(1 << log_n) + 1:].any()
             z.free()
-    for x in [da, db] + cols + dext + acc:
+    for x in [da, db] + cols + ebufs + acc:
         x.free()
