@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
@@ -36,16 +37,17 @@ struct EvalCol {
 struct EvalFactors {
   uint32_t g[32][4];  // g_0 = y, g_1 = x, g_k = pi^(k-1)(x)
 };
+static_assert(sizeof(EvalFactors) == 4 * kOodsFactorWords, "the device OODS point's factor layout");
 
-__device__ __forceinline__ qm31 ld_factor(const EvalFactors &f, uint32_t k) {
-  return qm31_make(f.g[k][0], f.g[k][1], f.g[k][2], f.g[k][3]);
+__device__ __forceinline__ qm31 ld_factor(const EvalFactors *f, uint32_t k) {
+  return qm31_make(f->g[k][0], f->g[k][1], f->g[k][2], f->g[k][3]);
 }
 
 // Basis values of the low kEvalBlockLog bits: basis[t] = prod of g_k over the
 // set bits k of t (k < kEvalBlockLog), 4 words each.  The basis of index
 // j = (b << B) + t is basis[t] times the product of g_k over the set bits of
 // b at positions k >= B, so one table serves every column and block.
-__global__ void __launch_bounds__(256) eval_basis_kernel(EvalFactors f, uint32_t *basis, uint32_t n) {
+__global__ void __launch_bounds__(256) eval_basis_kernel(const EvalFactors *f, uint32_t *basis, uint32_t n) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   qm31 r = qm31_make(1, 0, 0, 0);
@@ -68,7 +70,8 @@ struct EvalWork {
 };
 
 
-__global__ void __launch_bounds__(256) eval_point_dot_kernel(const EvalCol *cols, const EvalWork *work, EvalFactors f,
+__global__ void __launch_bounds__(256) eval_point_dot_kernel(const EvalCol *cols, const EvalWork *work,
+                                                             const EvalFactors *f,
                                                              const uint32_t *basis, uint32_t *partials) {
   __shared__ uint32_t red[kDotCols * 4][257];
   __shared__ uint32_t sums[kDotCols * 4];
@@ -210,12 +213,17 @@ int eval_point_scratch_setup(stwo_hip_ctx *ctx) {
 
 // `na_scratch`: NULL = buffers from the context's scratch pool (allocated on
 // first use or growth); otherwise the fixed per-device array, with every
-// size checked against it and no allocating call made.
+// size checked against it and no allocating call made.  `d_factors`: the
+// point's factors already on the device (EvalFactors layout), `point` unused;
+// `d_values`: the values stay on the device there (4 words per column),
+// `values` unused.
 static int eval_at_point_impl(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes,
                               size_t n_cols, const stwo_hip_secure_point *point, uint32_t *values,
-                              uint32_t *na_scratch) {
+                              uint32_t *na_scratch, const uint32_t *d_factors = nullptr,
+                              uint32_t *d_values = nullptr) {
   if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, point && values && (coeffs || !n_cols) && (log_sizes || !n_cols),
+  STWO_HIP_CHECK_ARG(ctx, (point || d_factors) && (values || d_values) && (coeffs || !n_cols) &&
+                              (log_sizes || !n_cols),
                      "eval_at_point: NULL argument");
   if (n_cols == 0) return STWO_HIP_OK;
   STWO_HIP_CHECK_ARG(ctx, n_cols <= 65536, "eval_at_point: %zu columns in one call (at most 65536)", n_cols);
@@ -233,12 +241,14 @@ static int eval_at_point_impl(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, 
   STWO_HIP_CHECK_ARG(ctx, total < (1ull << 31), "eval_at_point: too many blocks");
   // Factors g_0 = y, g_1 = x, g_k = pi^(k-1)(x) (CPUB/circle.rs:82-99).
   EvalFactors f{};
-  qm31 x = qm31_make(point->x[0], point->x[1], point->x[2], point->x[3]);
-  memcpy(f.g[0], point->y, 16);
-  for (uint32_t k = 1; k < 32 && k < max_log + 1; k++) {
-    memcpy(f.g[k], x.v, 16);
-    qm31 sq = qm31_mul(x, x);
-    x = qm31_sub(qm31_add(sq, sq), qm31_make(1, 0, 0, 0));
+  if (!d_factors) {
+    qm31 x = qm31_make(point->x[0], point->x[1], point->x[2], point->x[3]);
+    memcpy(f.g[0], point->y, 16);
+    for (uint32_t k = 1; k < 32 && k < max_log + 1; k++) {
+      memcpy(f.g[k], x.v, 16);
+      qm31 sq = qm31_mul(x, x);
+      x = qm31_sub(qm31_add(sq, sq), qm31_make(1, 0, 0, 0));
+    }
   }
   // Work items: runs of up to kDotCols consecutive columns of one log size, per block.
   std::vector<EvalWork> work;
@@ -259,19 +269,23 @@ static int eval_at_point_impl(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, 
                        "scratch, more than the %zu available",
                        n_cols, (unsigned long long)total,
                        (unsigned long long)(kNaBasisWords + 4 * ((uint64_t)n_cols + total)), kNaScratchWords);
-    STWO_HIP_CHECK_ARG(ctx, tab.size() * sizeof(EvalCol) + work.size() * sizeof(EvalWork) + 512 <= ctx->arena_cap / 2,
+    STWO_HIP_CHECK_ARG(ctx,
+                       tab.size() * sizeof(EvalCol) + work.size() * sizeof(EvalWork) + sizeof(EvalFactors) + 512 <=
+                           ctx->arena_cap / 2,
                        "eval_at_point_noalloc: %zu work items exceed the table arena", work.size());
   }
   void *d_tab = nullptr, *d_work = nullptr, *d_basis = nullptr, *d_part = nullptr, *d_out = nullptr;
+  const EvalFactors *d_f = (const EvalFactors *)d_factors;
   int rc;
   {
-    const void *srcs[2] = {tab.data(), work.data()};
-    const size_t bytes[2] = {tab.size() * sizeof(EvalCol), work.size() * sizeof(EvalWork)};
-    void *d[2];
-    rc = upload_tables(ctx, 2, srcs, bytes, d);
+    const void *srcs[3] = {tab.data(), work.data(), &f};
+    const size_t bytes[3] = {tab.size() * sizeof(EvalCol), work.size() * sizeof(EvalWork), sizeof f};
+    void *d[3];
+    rc = upload_tables(ctx, d_factors ? 2 : 3, srcs, bytes, d);
     if (rc) return rc;
     d_tab = d[0];
     d_work = d[1];
+    if (!d_factors) d_f = (const EvalFactors *)d[2];
   }
   if (na_scratch) {
     d_basis = na_scratch;
@@ -285,16 +299,25 @@ static int eval_at_point_impl(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, 
     rc = get_scratch(ctx, "eval_point_out", n_cols * 16, &d_out);
     if (rc) return rc;
   }
-  rc = launch(ctx, "eval_at_point_basis", eval_basis_kernel, dim3((n_basis + 255) / 256), dim3(256), 0, f,
+  if (d_values) d_out = d_values;
+  rc = launch(ctx, "eval_at_point_basis", eval_basis_kernel, dim3((n_basis + 255) / 256), dim3(256), 0, d_f,
               (uint32_t *)d_basis, n_basis);
   if (rc) return rc;
   rc = launch(ctx, "eval_at_point_dot", eval_point_dot_kernel, dim3((uint32_t)work.size()), dim3(256), 0,
-              (const EvalCol *)d_tab, (const EvalWork *)d_work, f, (const uint32_t *)d_basis, (uint32_t *)d_part);
+              (const EvalCol *)d_tab, (const EvalWork *)d_work, d_f, (const uint32_t *)d_basis, (uint32_t *)d_part);
   if (rc) return rc;
   rc = launch(ctx, "eval_at_point_sum", eval_point_sum_kernel, dim3((uint32_t)n_cols), dim3(256), 0,
               (const EvalCol *)d_tab, (uint32_t)n_cols, (uint32_t)total, (const uint32_t *)d_part, (uint32_t *)d_out);
   if (rc) return rc;
+  if (d_values) return STWO_HIP_OK;
   return stwo_hip_download(ctx, values, (const uint32_t *)d_out, 4 * n_cols);
+}
+
+int eval_at_point_dev(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
+                      const uint32_t *d_factors, uint32_t *d_values) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, d_factors && d_values, "eval_at_point: NULL device factors or values");
+  return eval_at_point_impl(ctx, coeffs, log_sizes, n_cols, nullptr, nullptr, nullptr, d_factors, d_values);
 }
 
 }  // namespace stwo_hip

@@ -394,6 +394,67 @@ __global__ void __launch_bounds__(kTailThreads) fri_top_kernel(uint32_t *tree, u
   if (threadIdx.x < 4) channel_step(chan, root, root_out, alpha_out);
 }
 
+// chan = [digest, n_draws, 0] (the device channel's starting state).
+struct ChanInit {
+  uint32_t w[10];
+};
+__global__ void chan_init_kernel(ChanInit init, uint32_t *chan) {
+  if (threadIdx.x < 10) chan[threadIdx.x] = init.w[threadIdx.x];
+}
+
+// One Fiat-Shamir step of the prove on the device: chan (optionally first
+// set to [init digest, 0, 0]) mixes `root` and draws a secure felt t into
+// draw_out (channel_step).  When point_out is set, the draw is the OODS
+// point's parameter (CirclePoint::get_random_point, STWO/core/circle.rs:169-181):
+// p = ((1 - t^2) / (1 + t^2), 2t / (1 + t^2)); point_out receives the point's
+// factors for eval_at_point, g_0 = p.y, g_1 = p.x, g_k = pi^(k-1)(p.x) for
+// k < 32 (pi(x) = 2x^2 - 1; 128 words), then p.x, p.y and
+// vn = pi^(point_log - 1)(p.x) (12 words) for the barycentric weights.
+// One wave; the step runs on lanes 0-3.
+__global__ void __launch_bounds__(64) prove_channel_kernel(uint32_t *chan, ChanInit init, uint32_t has_init,
+                                                           const uint32_t *root, uint32_t *root_out,
+                                                           uint32_t *draw_out, uint32_t *point_out,
+                                                           uint32_t point_log) {
+  const uint32_t tid = threadIdx.x;
+  if (has_init) {
+    if (tid < 10) chan[tid] = init.w[tid];
+    __syncthreads();
+  }
+  if (tid < 4) channel_step(chan, root, root_out, draw_out);
+  if (!point_out) return;
+  __syncthreads();
+  if (tid != 0) return;
+  const qm31 t = load_qm31(draw_out);
+  const qm31 one = qm31_make(1, 0, 0, 0);
+  const qm31 t2 = qm31_mul(t, t);
+  const qm31 inv = qm31_inv(qm31_add(t2, one));
+  const qm31 px = qm31_mul(qm31_sub(one, t2), inv);
+  const qm31 py = qm31_mul(qm31_add(t, t), inv);
+  qm31 vn = px;
+  for (int k = 0; k < 4; k++) point_out[k] = py.v[k];
+  qm31 x = px;
+  for (uint32_t g = 1; g < 32; g++) {
+    if (g == point_log) vn = x;
+    for (int k = 0; k < 4; k++) point_out[4 * g + k] = x.v[k];
+    const qm31 sq = qm31_mul(x, x);
+    x = qm31_sub(qm31_add(sq, sq), one);
+  }
+  for (int k = 0; k < 4; k++) {
+    point_out[128 + k] = px.v[k];
+    point_out[132 + k] = py.v[k];
+    point_out[136 + k] = vn.v[k];
+  }
+}
+
+int prove_channel_step(stwo_hip_ctx *ctx, uint32_t *chan, const uint8_t *init_digest, const uint32_t *root,
+                       uint32_t *root_out, uint32_t *draw_out, uint32_t *point_out, uint32_t point_log) {
+  STWO_HIP_CHECK_ARG(ctx, point_log >= 1 && point_log < 32, "internal: OODS point log %u", point_log);
+  ChanInit init{};
+  if (init_digest) memcpy(init.w, init_digest, 32);
+  return launch(ctx, "prove_channel", prove_channel_kernel, dim3(1), dim3(64), 0, chan, init,
+                (uint32_t)(init_digest != nullptr), root, root_out, draw_out, point_out, point_log);
+}
+
 static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
   STWO_HIP_CHECK_ARG(ctx, tw != nullptr && tw->device == ctx->device, "twiddles missing or from another device");
   STWO_HIP_CHECK_ARG(ctx, dom.log_size <= tw->root.log_size, "domain log %u exceeds twiddle root log %u",
@@ -613,11 +674,12 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
   uint32_t *state = (uint32_t *)state_p;
   uint32_t *d_chan = state, *d_alpha = state + 10, *d_alpha_tmp = state + 14, *d_roots = state + kRootsOff;
   {
-    uint32_t init[10];
-    memcpy(init, channel_digest, 32);
-    init[8] = *channel_n_draws;
-    init[9] = 0;
-    rc = stwo_hip_upload(ctx, d_chan, init, 10);
+    // set by a launch, in stream order: no synchronization
+    ChanInit init{};
+    memcpy(init.w, channel_digest, 32);
+    init.w[8] = *channel_n_draws;
+    init.w[9] = 0;
+    rc = launch(ctx, "fri_chan_init", chan_init_kernel, dim3(1), dim3(64), 0, init, d_chan);
     if (rc) return rc;
   }
   uint32_t nr = 0;
@@ -917,9 +979,33 @@ int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uin
                                  uint32_t *channel_n_draws, uint32_t *query_positions, uint32_t *n_positions,
                                  uint32_t *unsorted_queries) {
   if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, pr && channel_digest && channel_n_draws && query_positions && n_positions &&
-                              unsorted_queries,
-                     "fri_prover_decommit: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, query_positions && n_positions && unsorted_queries, "fri_prover_decommit: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws, "fri_prover_decommit: NULL argument");
+  // The channel is written back only on success.
+  uint8_t digest[32];
+  memcpy(digest, channel_digest, 32);
+  uint32_t n_draws = *channel_n_draws;
+  Gatherer g;
+  FriDecommitPlan plan;
+  int rc = fri_decommit_plan(ctx, pr, digest, &n_draws, g, plan);
+  if (rc) return rc;
+  rc = g.run(ctx);
+  if (rc) return rc;
+  memcpy(channel_digest, digest, 32);
+  *channel_n_draws = n_draws;
+  memcpy(query_positions, plan.queries.data(), plan.queries.size() * 4);
+  *n_positions = (uint32_t)plan.queries.size();
+  memcpy(unsorted_queries, plan.raw.data(), plan.raw.size() * 4);
+  fri_decommit_finish(pr, g, plan);
+  return STWO_HIP_OK;
+}
+
+}  // extern "C"
+
+int stwo_hip::fri_decommit_plan(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uint8_t channel_digest[32],
+                                uint32_t *channel_n_draws, Gatherer &g, FriDecommitPlan &plan) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, pr && channel_digest && channel_n_draws, "fri_prover_decommit: NULL argument");
   STWO_HIP_CHECK_ARG(ctx, pr->device == ctx->device, "fri_prover_decommit: prover of another device");
   STWO_HIP_CHECK_ARG(ctx, !pr->decommitted, "fri_prover_decommit: already decommitted");
   const uint32_t n_queries = pr->cfg.n_queries;
@@ -930,29 +1016,24 @@ int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uin
   ch.n_draws = *channel_n_draws;
   const uint32_t N = pr->log_size;
   const uint32_t mask = (uint32_t)((1ull << N) - 1);
-  std::vector<uint32_t> raw;
+  std::vector<uint32_t> &raw = plan.raw;
+  raw.clear();
   while (raw.size() < n_queries) {
     uint32_t w[8];
     ch.draw_u32s(w);
     for (int k = 0; k < 8 && raw.size() < n_queries; k++) raw.push_back(w[k] & mask);
   }
-  std::vector<uint32_t> queries = raw;
+  std::vector<uint32_t> &queries = plan.queries;
+  queries = raw;
   std::sort(queries.begin(), queries.end());
   queries.erase(std::unique(queries.begin(), queries.end()), queries.end());
 
-  struct LayerPlan {
-    std::vector<uint32_t> positions;
-    std::vector<size_t> val_off;   // gather offset of each coordinate read (4 per position)
-    std::vector<uint8_t> witness;  // 1 if the position is not a query
-    MerkleDecommitPlan merkle;
-  };
-  Gatherer g;
-  std::vector<LayerPlan> plans(pr->layers.size());
+  plan.layers.assign(pr->layers.size(), FriDecommitPlan::Layer());
   std::vector<uint32_t> lq = queries;
   for (size_t k = 0; k < pr->layers.size(); k++) {
     const FriLayerKeep &layer = pr->layers[k];
     const uint32_t step = layer.shape.fold_step;
-    LayerPlan &pl = plans[k];
+    FriDecommitPlan::Layer &pl = plan.layers[k];
     for (size_t i = 0; i < lq.size();) {
       size_t j = i;
       while (j < lq.size() && (lq[j] >> step) == (lq[i] >> step)) j++;
@@ -975,11 +1056,15 @@ int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uin
     }
     lq.swap(folded);
   }
-  int rc = g.run(ctx);
-  if (rc) return rc;
+  memcpy(channel_digest, ch.digest, 32);
+  *channel_n_draws = ch.n_draws;
+  return STWO_HIP_OK;
+}
+
+void stwo_hip::fri_decommit_finish(stwo_hip_fri_prover *pr, const Gatherer &g, FriDecommitPlan &plan) {
   pr->proofs.assign(pr->layers.size(), FriLayerDecommit());
-  for (size_t k = 0; k < plans.size(); k++) {
-    const LayerPlan &pl = plans[k];
+  for (size_t k = 0; k < plan.layers.size(); k++) {
+    const FriDecommitPlan::Layer &pl = plan.layers[k];
     FriLayerDecommit &d = pr->proofs[k];
     d.positions = pl.positions;
     for (size_t i = 0; i < pl.positions.size(); i++) {
@@ -989,16 +1074,12 @@ int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uin
     }
     finish_merkle_decommit(g, pl.merkle, d.hash_witness, nullptr);
   }
-  pr->query_positions = queries;
-  pr->unsorted_queries = raw;
+  pr->query_positions = plan.queries;
+  pr->unsorted_queries = plan.raw;
   pr->decommitted = true;
-  memcpy(query_positions, queries.data(), queries.size() * 4);
-  *n_positions = (uint32_t)queries.size();
-  memcpy(unsorted_queries, raw.data(), raw.size() * 4);
-  memcpy(channel_digest, ch.digest, 32);
-  *channel_n_draws = ch.n_draws;
-  return STWO_HIP_OK;
 }
+
+extern "C" {
 
 int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *pr, uint32_t layer,
                                     uint32_t commitment[8], uint32_t *fri_witness, size_t fri_cap, size_t *n_fri,

@@ -7,8 +7,10 @@
 // le32(pow_bits)) (STWO/core/channel/blake2s.rs:122-139).  Here one lane
 // hashes one nonce (a single 40-byte Blake2s block) and a launch covers
 // 2^hb consecutive hi words; valid lanes take the minimum with a 64-bit
-// atomic.  Launches go in hi order, so the first launch that finds a nonce
-// holds the smallest one.
+// atomic.  Launches go in nonce order, so the first launch that finds a
+// nonce holds the smallest one.  The first launch covers only the lowest
+// 2^(pow_bits + 6) lo words of hi = 0 (it misses with probability about
+// e^-64), so the usual search hashes a few thousand nonces, not 2^20.
 #include <string.h>
 
 #include <algorithm>
@@ -22,15 +24,16 @@ namespace stwo_hip {
 constexpr uint32_t kPowPrefix = 0x12345678u;
 constexpr uint32_t kGrindLowBits = 20;
 
+// Lane gid hashes nonce (hi0 + (gid >> 20), lo0 + (gid & (2^20 - 1))).
 struct GrindArgs {
   uint32_t prefixed[8];
-  uint32_t hi0;
+  uint32_t hi0, lo0;
   uint32_t pow_bits;
 };
 
 __global__ void __launch_bounds__(256) grind_kernel(GrindArgs a, unsigned long long *best) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t lo = gid & ((1u << kGrindLowBits) - 1);
+  const uint32_t lo = a.lo0 + (gid & ((1u << kGrindLowBits) - 1));
   const uint32_t hi = a.hi0 + (gid >> kGrindLowBits);
   uint32_t m[16];
 #pragma unroll
@@ -78,24 +81,39 @@ int stwo_hip_grind(stwo_hip_ctx *ctx, const uint8_t channel_digest[32], uint32_t
   void *d_best = nullptr;
   int rc = get_scratch(ctx, "grind_best", 8, &d_best);
   if (rc) return rc;
+  // Searches `lanes` nonces from (hi0, lo0) on; 1 when one was found.
+  auto search = [&](uint32_t hi0, uint32_t lo0, uint32_t lanes) -> int {
+    int r = launch(ctx, "grind_reset", grind_reset_kernel, dim3(1), dim3(64), 0, (unsigned long long *)d_best);
+    if (r) return r;
+    a.hi0 = hi0;
+    a.lo0 = lo0;
+    r = launch(ctx, "grind", grind_kernel, dim3(lanes / 256), dim3(256), 0, a, (unsigned long long *)d_best);
+    if (r) return r;
+    uint64_t best = 0;
+    r = stwo_hip_download(ctx, (uint32_t *)&best, (const uint32_t *)d_best, 2);
+    if (r) return r;
+    if (best == ~0ull) return 0;
+    // The reference asserts both halves are below P (SIMD/grind.rs:50-57).
+    STWO_HIP_CHECK_ARG(ctx, (best >> 32) < P && (uint32_t)best < P, "grind: nonce halves not below P");
+    *nonce = best;
+    return 1;
+  };
+  uint64_t h_start = 0;
+  // lo < 2^k0, then the rest of hi = 0 (both multiples of 256 lanes).
+  const uint32_t k0 = std::max<uint32_t>(8, pow_bits + 6);
+  if (k0 < kGrindLowBits) {
+    rc = search(0, 0, 1u << k0);
+    if (rc) return rc == 1 ? STWO_HIP_OK : rc;
+    rc = search(0, 1u << k0, (1u << kGrindLowBits) - (1u << k0));
+    if (rc) return rc == 1 ? STWO_HIP_OK : rc;
+    h_start = 1;
+  }
   // About 2^(pow_bits + 1) attempts per launch (at least one hi word, at most 64).
   const uint32_t hb = pow_bits + 1 > kGrindLowBits ? std::min<uint32_t>(pow_bits + 1 - kGrindLowBits, 6) : 0;
   const uint32_t lanes = 1u << (kGrindLowBits + hb);
-  for (uint64_t h0 = 0; h0 < P; h0 += (1u << hb)) {
-    rc = launch(ctx, "grind_reset", grind_reset_kernel, dim3(1), dim3(64), 0, (unsigned long long *)d_best);
-    if (rc) return rc;
-    a.hi0 = (uint32_t)h0;
-    rc = launch(ctx, "grind", grind_kernel, dim3(lanes / 256), dim3(256), 0, a, (unsigned long long *)d_best);
-    if (rc) return rc;
-    uint64_t best = 0;
-    rc = stwo_hip_download(ctx, (uint32_t *)&best, (const uint32_t *)d_best, 2);
-    if (rc) return rc;
-    if (best != ~0ull) {
-      // The reference asserts both halves are below P (SIMD/grind.rs:50-57).
-      STWO_HIP_CHECK_ARG(ctx, (best >> 32) < P && (uint32_t)best < P, "grind: nonce halves not below P");
-      *nonce = best;
-      return STWO_HIP_OK;
-    }
+  for (uint64_t h0 = h_start; h0 < P; h0 += (1u << hb)) {
+    rc = search((uint32_t)h0, 0, lanes);
+    if (rc) return rc == 1 ? STWO_HIP_OK : rc;
   }
   return set_error(ctx, STWO_HIP_EDEVICE, "Grind failed to find a solution.");
 }

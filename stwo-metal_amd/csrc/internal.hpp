@@ -102,10 +102,37 @@ int batch_create_trees(stwo_hip_ctx *ctx, const BatchTree *trees, size_t n_trees
 
 // stwo_hip_wide_fib_constraint_quotients, writing the quotients over accum
 // instead of adding them when !accumulate (a single component needs no zeroed
-// accumulator).
+// accumulator); the random coefficient from the host (random_coeff) or
+// already on the device (alpha_dev, 4 words, random_coeff NULL).
 int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
                                        uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4],
-                                       bool accumulate);
+                                       bool accumulate, const uint32_t *alpha_dev);
+
+// ---- the prove's Fiat-Shamir steps on the device ---------------------------
+
+// One wave: chan (10 words: digest, n_draws, error flag; first set to
+// [init_digest, 0, 0] when init_digest is not NULL) mixes the Merkle root at
+// `root` (copied to root_out) and draws a secure felt into draw_out.  With
+// point_out, the draw is turned into the OODS point: point_out gets
+// kOodsFactorWords of eval_at_point factors (g_0 = y, g_1 = x,
+// g_k = pi^(k-1)(x), k < 32), then x, y and pi^(point_log - 1)(x).
+constexpr uint32_t kOodsFactorWords = 128;
+constexpr uint32_t kOodsPointWords = kOodsFactorWords + 12;
+int prove_channel_step(stwo_hip_ctx *ctx, uint32_t *chan, const uint8_t *init_digest, const uint32_t *root,
+                       uint32_t *root_out, uint32_t *draw_out, uint32_t *point_out, uint32_t point_log);
+
+// stwo_hip_eval_at_point with the point's factors on the device (the first
+// kOodsFactorWords words of prove_channel_step's point_out) and the values
+// left on the device (4 words per column).
+int eval_at_point_dev(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, const uint32_t *log_sizes, size_t n_cols,
+                      const uint32_t *d_factors, uint32_t *d_values);
+
+// stwo_hip_barycentric_weights with the point on the device (x, y and
+// pi^(log_size - 1)(x), 12 words), and stwo_hip_barycentric_eval_at_point
+// with the values left on the device.
+int barycentric_weights_dev(stwo_hip_ctx *ctx, uint32_t log_size, const uint32_t *d_point, uint32_t *const out[4]);
+int barycentric_eval_dev(stwo_hip_ctx *ctx, const uint32_t *const *evals, size_t n_cols, uint32_t log_size,
+                         const uint32_t *const weights[4], uint32_t *d_values);
 
 // ---- decommitment: batched device gather -----------------------------------
 
@@ -194,6 +221,25 @@ int fri_prover_commit_mem(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_
                           stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
                           const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly,
                           const char *scratch_key, stwo_hip_fri_prover **out);
+
+// stwo_hip_fri_prover_decommit in two halves around a caller's gather, so
+// that other reads (the trace trees' decommitments) share its launch and
+// synchronisation: fri_decommit_plan draws the queries from the channel (the
+// digest and draw count are updated) and adds every FRI read to g;
+// fri_decommit_finish, after g.run, fills the prover's layer proofs.
+struct FriDecommitPlan {
+  struct Layer {
+    std::vector<uint32_t> positions;
+    std::vector<size_t> val_off;   // gather offset of each coordinate read (4 per position)
+    std::vector<uint8_t> witness;  // 1 if the position is not a query
+    MerkleDecommitPlan merkle;
+  };
+  std::vector<Layer> layers;
+  std::vector<uint32_t> queries, raw;  // sorted and deduplicated; as drawn
+};
+int fri_decommit_plan(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uint8_t channel_digest[32],
+                      uint32_t *channel_n_draws, Gatherer &g, FriDecommitPlan &plan);
+void fri_decommit_finish(stwo_hip_fri_prover *pr, const Gatherer &g, FriDecommitPlan &plan);
 
 // Merkle column reference (pointer + lifting shift).
 struct ColRef {

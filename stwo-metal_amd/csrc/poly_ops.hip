@@ -28,6 +28,7 @@
 
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
@@ -44,8 +45,11 @@ struct BaryPointArgs {
   uint32_t vn[4];          // vn(p)
 };
 
-__global__ void __launch_bounds__(256) barycentric_weights_kernel(BaryPointArgs a, uint32_t *w0, uint32_t *w1,
-                                                                  uint32_t *w2, uint32_t *w3) {
+// d_pt: the sample point x, y and vn(p) on the device (12 words), in place
+// of a.px, a.py and a.vn, when not NULL.
+__global__ void __launch_bounds__(256) barycentric_weights_kernel(BaryPointArgs a, const uint32_t *d_pt,
+                                                                  uint32_t *w0, uint32_t *w1, uint32_t *w2,
+                                                                  uint32_t *w3) {
   const uint32_t n = a.n;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (1u << n)) return;
@@ -63,12 +67,14 @@ __global__ void __launch_bounds__(256) barycentric_weights_kernel(BaryPointArgs 
     x = m31_sub(m31_add(sq, sq), 1);
   }
   const uint32_t s = m31_mul(m31_mul(m31_neg(m31_add(q.y, q.y)), a.four_pow), prod);
-  const qm31 px = qm31_make(a.px[0], a.px[1], a.px[2], a.px[3]);
-  const qm31 py = qm31_make(a.py[0], a.py[1], a.py[2], a.py[3]);
+  const uint32_t *pp = d_pt ? d_pt : a.px;  // a.px, a.py, a.vn are adjacent
+  const qm31 px = qm31_make(pp[0], pp[1], pp[2], pp[3]);
+  const qm31 py = qm31_make(pp[4], pp[5], pp[6], pp[7]);
+  const qm31 vn = qm31_make(pp[8], pp[9], pp[10], pp[11]);
   // h = p - q = p + (q.x, -q.y)
   const qm31 hx = qm31_add(qm31_mul_m31(px, q.x), qm31_mul_m31(py, q.y));
   const qm31 hy = qm31_sub(qm31_mul_m31(py, q.x), qm31_mul_m31(px, q.y));
-  const qm31 num = qm31_mul(qm31_make(a.vn[0], a.vn[1], a.vn[2], a.vn[3]), qm31_add(qm31_make(1, 0, 0, 0), hx));
+  const qm31 num = qm31_mul(vn, qm31_add(qm31_make(1, 0, 0, 0), hx));
   const qm31 w = qm31_mul(num, qm31_inv(qm31_mul_m31(hy, s)));
   w0[i] = w.v[0];
   w1[i] = w.v[1];
@@ -257,16 +263,12 @@ __global__ void __launch_bounds__(256) decompose_apply_kernel(const uint32_t *c0
   }
 }
 
-}  // namespace stwo_hip
-
-using namespace stwo_hip;
-
-extern "C" {
-
-int stwo_hip_barycentric_weights(stwo_hip_ctx *ctx, uint32_t log_size, const stwo_hip_secure_point *p,
-                                 uint32_t *const out[4]) {
+// The point from the host (p) or the device (d_pt, 12 words: x, y, vn).
+static int barycentric_weights_impl(stwo_hip_ctx *ctx, uint32_t log_size, const stwo_hip_secure_point *p,
+                                    const uint32_t *d_pt, uint32_t *const out[4]) {
   if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, p && out && out[0] && out[1] && out[2] && out[3], "barycentric_weights: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, (p || d_pt) && out && out[0] && out[1] && out[2] && out[3],
+                     "barycentric_weights: NULL argument");
   STWO_HIP_CHECK_ARG(ctx, log_size >= 1 && log_size <= 30, "barycentric_weights: log size %u outside [1, 30]",
                      log_size);
   BaryPointArgs a{};
@@ -276,23 +278,33 @@ int stwo_hip_barycentric_weights(stwo_hip_ctx *ctx, uint32_t log_size, const stw
   a.init = host_index_to_point(init);
   for (uint32_t j = 0; j + 1 < log_size; j++) a.step[j] = host_index_to_point((uint32_t)(((uint64_t)step << j) & IDX_MASK));
   a.four_pow = m31_pow(4, log_size - 1);
-  memcpy(a.px, p->x, 16);
-  memcpy(a.py, p->y, 16);
-  qm31 x = qm31_make(p->x[0], p->x[1], p->x[2], p->x[3]);
-  for (uint32_t j = 1; j < log_size; j++) {
-    const qm31 sq = qm31_mul(x, x);
-    x = qm31_sub(qm31_add(sq, sq), qm31_make(1, 0, 0, 0));
+  if (p) {
+    memcpy(a.px, p->x, 16);
+    memcpy(a.py, p->y, 16);
+    qm31 x = qm31_make(p->x[0], p->x[1], p->x[2], p->x[3]);
+    for (uint32_t j = 1; j < log_size; j++) {
+      const qm31 sq = qm31_mul(x, x);
+      x = qm31_sub(qm31_add(sq, sq), qm31_make(1, 0, 0, 0));
+    }
+    memcpy(a.vn, x.v, 16);
   }
-  memcpy(a.vn, x.v, 16);
   const uint64_t n = 1ull << log_size;
   return launch(ctx, "barycentric_weights", barycentric_weights_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256),
-                0, a, out[0], out[1], out[2], out[3]);
+                0, a, d_pt, out[0], out[1], out[2], out[3]);
 }
 
-int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *evals, size_t n_cols,
-                                       uint32_t log_size, const uint32_t *const weights[4], uint32_t *values) {
+int barycentric_weights_dev(stwo_hip_ctx *ctx, uint32_t log_size, const uint32_t *d_point, uint32_t *const out[4]) {
   if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, (evals || !n_cols) && weights && values, "barycentric_eval_at_point: NULL argument");
+  STWO_HIP_CHECK_ARG(ctx, d_point, "barycentric_weights: NULL device point");
+  return barycentric_weights_impl(ctx, log_size, nullptr, d_point, out);
+}
+
+// The values to the host (values) or left on the device (d_values).
+static int barycentric_eval_impl(stwo_hip_ctx *ctx, const uint32_t *const *evals, size_t n_cols, uint32_t log_size,
+                                 const uint32_t *const weights[4], uint32_t *values, uint32_t *d_values) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, (evals || !n_cols) && weights && (values || d_values),
+                     "barycentric_eval_at_point: NULL argument");
   if (n_cols == 0) return STWO_HIP_OK;
   STWO_HIP_CHECK_ARG(ctx, log_size >= 1 && log_size <= 30 && n_cols <= 65536,
                      "barycentric_eval_at_point: log size %u or %zu columns out of range", log_size, n_cols);
@@ -311,11 +323,13 @@ int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const 
   const size_t bytes[2] = {n_cols * sizeof(uint32_t *), work.size() * sizeof(BaryWork)};
   int rc = upload_tables(ctx, 2, srcs, bytes, d);
   if (rc) return rc;
-  void *part = nullptr, *out = nullptr;
+  void *part = nullptr, *out = d_values;
   rc = get_scratch(ctx, "bary_partials", n_cols * (size_t)n_blocks * 16, &part);
   if (rc) return rc;
-  rc = get_scratch(ctx, "bary_out", n_cols * 16, &out);
-  if (rc) return rc;
+  if (!out) {
+    rc = get_scratch(ctx, "bary_out", n_cols * 16, &out);
+    if (rc) return rc;
+  }
   rc = launch(ctx, "barycentric_dot", vec ? barycentric_dot_kernel<true> : barycentric_dot_kernel<false>,
               dim3((uint32_t)work.size()), dim3(256), 0,
               (const uint32_t *const *)d[0], (const BaryWork *)d[1], log_size, weights[0], weights[1], weights[2],
@@ -324,7 +338,35 @@ int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const 
   rc = launch(ctx, "barycentric_sum", barycentric_sum_kernel, dim3((uint32_t)n_cols), dim3(256), 0,
               (const uint32_t *)part, n_blocks, (uint32_t *)out);
   if (rc) return rc;
+  if (d_values) return STWO_HIP_OK;
   return stwo_hip_download(ctx, values, (const uint32_t *)out, 4 * n_cols);
+}
+
+int barycentric_eval_dev(stwo_hip_ctx *ctx, const uint32_t *const *evals, size_t n_cols, uint32_t log_size,
+                         const uint32_t *const weights[4], uint32_t *d_values) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, d_values, "barycentric_eval_at_point: NULL device values");
+  return barycentric_eval_impl(ctx, evals, n_cols, log_size, weights, nullptr, d_values);
+}
+
+}  // namespace stwo_hip
+
+using namespace stwo_hip;
+
+extern "C" {
+
+int stwo_hip_barycentric_weights(stwo_hip_ctx *ctx, uint32_t log_size, const stwo_hip_secure_point *p,
+                                 uint32_t *const out[4]) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, p, "barycentric_weights: NULL argument");
+  return barycentric_weights_impl(ctx, log_size, p, nullptr, out);
+}
+
+int stwo_hip_barycentric_eval_at_point(stwo_hip_ctx *ctx, const uint32_t *const *evals, size_t n_cols,
+                                       uint32_t log_size, const uint32_t *const weights[4], uint32_t *values) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, values, "barycentric_eval_at_point: NULL argument");
+  return barycentric_eval_impl(ctx, evals, n_cols, log_size, weights, values, nullptr);
 }
 
 int stwo_hip_eval_at_point_by_folding(stwo_hip_ctx *ctx, const uint32_t *evals, uint32_t log_size,

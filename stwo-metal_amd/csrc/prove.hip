@@ -283,42 +283,52 @@ static int fetch_layer(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *fri, uint32
 }
 
 // MerkleProverLifted::decommit of several trees (all columns of log size L)
-// at the same query positions, with one device gather for all of them.
+// at the same query positions: plan_trees adds their reads to a gather,
+// finish_trees reads them back after it ran.
 struct TreeDecommit {
   const uint32_t *tree;
   std::vector<const uint32_t *> cols;
   std::vector<std::vector<uint32_t>> *values;  // [column][query]
   std::vector<uint32_t> *witness;              // 8 words per hash
+  std::vector<size_t> val_off;
+  MerkleDecommitPlan plan;
 };
-static int decommit_trees(stwo_hip_ctx *ctx, uint32_t L, const std::vector<uint32_t> &queries,
-                          std::vector<TreeDecommit> &trees) {
-  Gatherer g;
-  std::vector<std::vector<size_t>> val_off(trees.size());
-  std::vector<MerkleDecommitPlan> plans(trees.size());
-  for (size_t t = 0; t < trees.size(); t++) {
-    for (const uint32_t *c : trees[t].cols)
-      for (uint32_t pos : queries) val_off[t].push_back(g.add(c + pos, 1));  // log size L: no lifting
-    plan_merkle_decommit(g, trees[t].tree, L, queries, plans[t]);
+static void plan_trees(Gatherer &g, uint32_t L, const std::vector<uint32_t> &queries,
+                       std::vector<TreeDecommit> &trees) {
+  for (TreeDecommit &t : trees) {
+    t.val_off.clear();
+    for (const uint32_t *c : t.cols)
+      for (uint32_t pos : queries) t.val_off.push_back(g.add(c + pos, 1));  // log size L: no lifting
+    plan_merkle_decommit(g, t.tree, L, queries, t.plan);
   }
-  int rc = g.run(ctx);
-  if (rc) return rc;
-  const size_t nq = queries.size();
-  for (size_t t = 0; t < trees.size(); t++) {
-    const size_t nc = trees[t].cols.size();
-    trees[t].values->assign(nc, std::vector<uint32_t>(nq));
-    for (size_t c = 0; c < nc; c++)
-      for (size_t q = 0; q < nq; q++) (*trees[t].values)[c][q] = g.out[val_off[t][c * nq + q]];
-    finish_merkle_decommit(g, plans[t], *trees[t].witness, nullptr);
-  }
-  return STWO_HIP_OK;
 }
+static void finish_trees(const Gatherer &g, size_t nq, std::vector<TreeDecommit> &trees) {
+  for (TreeDecommit &t : trees) {
+    const size_t nc = t.cols.size();
+    t.values->assign(nc, std::vector<uint32_t>(nq));
+    for (size_t c = 0; c < nc; c++)
+      for (size_t q = 0; q < nq; q++) (*t.values)[c][q] = g.out[t.val_off[c * nq + q]];
+    finish_merkle_decommit(g, t.plan, *t.witness, nullptr);
+  }
+}
+
+// The device Fiat-Shamir block at the end of the prove's main buffer: the
+// channel state, the composition coefficient, the OODS draw, copies of the
+// two roots, the OODS point (prove_channel_step's point_out), then the
+// sampled values (4 words per trace column and composition half).  One
+// download reads it all.
+constexpr size_t kFsChan = 0, kFsCompCoeff = 16, kFsOodsDraw = 20, kFsRoot1 = 24, kFsRoot2 = 32, kFsPoint = 64,
+                 kFsValues = 256;
+static_assert(kFsPoint + kOodsPointWords <= kFsValues, "Fiat-Shamir block layout");
+static size_t prove_fs_words(uint32_t n_cols) { return kFsValues + 4 * ((size_t)n_cols + 8); }
 
 // Device words of the prove's main buffer (the layout in stwo_hip_wide_fib_prove).
 static size_t prove_layout_words(uint32_t n_cols, uint32_t log_n, uint32_t B) {
   const uint32_t E = log_n + B, C = log_n + 1;
   const size_t n = (size_t)1 << log_n, e = (size_t)1 << E, nc = (size_t)1 << C;
   const size_t tree_words = tree_layer_off(E + 1);
-  return n_cols * n + n_cols * e + tree_words + (B > 1 ? n_cols * nc : 0) + 4 * nc + 8 * e + tree_words + 4 * e;
+  return n_cols * n + n_cols * e + tree_words + (B > 1 ? n_cols * nc : 0) + 4 * nc + 8 * e + tree_words + 4 * e +
+         prove_fs_words(n_cols);
 }
 
 size_t wide_fib_prove_device_bytes(uint32_t n_cols, uint32_t log_n, const stwo_hip_pcs_config &config) {
@@ -333,9 +343,10 @@ size_t wide_fib_prove_device_bytes(uint32_t n_cols, uint32_t log_n, const stwo_h
     words += (size_t)4 << lg;
   }
   words += 32 + 8 * 32;  // the FRI channel state
-  // OODS samples (eval_at_point): basis table, block partials, sums
-  const size_t blocks = (size_t)(n_cols + 8) << (log_n > 11 ? log_n - 11 : 0);
-  words += ((size_t)4 << 11) + 4 * blocks + 4 * (n_cols + 8);
+  // OODS samples (eval_at_point from log_n, or barycentric from E): basis
+  // table, block partials, sums; the constraint weights
+  const size_t blocks = (size_t)(n_cols + 8) << (E > 11 ? E - 11 : 0);
+  words += ((size_t)4 << 11) + 4 * blocks + 4 * (n_cols + 8) + 8 * (size_t)n_cols;
   // proof of work, decommitment gathers and tables: bounded by a margin
   return words * 4 + ((size_t)16 << 20);
 }
@@ -371,7 +382,7 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
   // Device layout (prove_layout_words): trace coefficients | trace
   // evaluations | trace tree | trace on CanonicCoset(C) (blowup > 1 only) |
   // composition coordinates | composition-half evaluations | composition
-  // tree | quotient coordinates.
+  // tree | quotient coordinates | Fiat-Shamir block.
   size_t off = 0;
   const size_t o_coef = off;
   off += n_cols * n;
@@ -389,6 +400,8 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
   off += tree_words;
   const size_t o_quot = off;
   off += 4 * e;
+  const size_t o_fs = off;
+  off += prove_fs_words(n_cols);
 
   ProveMem pm;
   pm.ctx = ctx;
@@ -441,7 +454,15 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
     if (rc) return rc;
   }
 
-  // 2. trace tree.
+  // 2.-4a. Trace commit, composition commit and the OODS samples run on the
+  // stream without a host round trip: after each commit a one-wave kernel
+  // mixes the tree's root into the device copy of the channel and draws the
+  // next value (the composition coefficient, then the OODS point), which the
+  // following kernels read from device memory.  One download then brings the
+  // roots, the channel state and the sampled values; the host replays the
+  // same channel steps (two roots, two draws) and checks that it reaches the
+  // device's state.
+  uint32_t *fs = mem + o_fs;
   const std::vector<uint32_t> logs_n(n_cols, log_n), logs_e(n_cols + 8, E), logs_8(8, log_n);
   rc = stwo_hip_wide_fib_trace(ctx, coef.data(), n_cols, log_n, a, b);
   if (!rc && plans) {
@@ -451,18 +472,15 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
     if (!rc)
       rc = stwo_hip_merkle_lifted_commit(ctx, eval.data(), logs_e.data(), n_cols, E, tree1, STWO_HIP_HASH_BLAKE2S);
   }
-  std::array<uint32_t, 8> root1;
-  if (!rc) rc = stwo_hip_download(ctx, root1.data(), tree1, 8);
+  if (!rc) rc = prove_channel_step(ctx, fs + kFsChan, ch.digest, tree1, fs + kFsRoot1, fs + kFsCompCoeff, nullptr, E);
   if (rc) return rc;
-  proof->commitments.push_back(root1);
-  ch.mix_root(root1.data());
   proof->phase_ms[0] = ms_since(t0);
 
   // 3. composition polynomial.
   t0 = Clock::now();
-  const qm31 comp_coeff = ch.draw_secure_felt();
   if (B > 1) rc = stwo_hip_evaluate_polynomials(ctx, coef.data(), logs_n.data(), n_cols, 1, tw, ext.data());
-  if (!rc) rc = wide_fib_constraint_quotients_impl(ctx, ext.data(), n_cols, log_n, comp_coeff.v, comp, false);
+  if (!rc)
+    rc = wide_fib_constraint_quotients_impl(ctx, ext.data(), n_cols, log_n, nullptr, comp, false, fs + kFsCompCoeff);
   const std::vector<uint32_t> logs_c(4, C);
   if (!rc && plans) {
     rc = stwo_hip_batch_run(ctx, plans->comp);
@@ -473,37 +491,48 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
     if (!rc)
       rc = stwo_hip_merkle_lifted_commit(ctx, ceval.data(), logs_e.data(), 8, E, tree2, STWO_HIP_HASH_BLAKE2S);
   }
-  std::array<uint32_t, 8> root2;
-  if (!rc) rc = stwo_hip_download(ctx, root2.data(), tree2, 8);
+  if (!rc) rc = prove_channel_step(ctx, fs + kFsChan, nullptr, tree2, fs + kFsRoot2, fs + kFsOodsDraw, fs + kFsPoint, E);
   if (rc) return rc;
-  proof->commitments.push_back(root2);
-  ch.mix_root(root2.data());
   proof->phase_ms[1] = ms_since(t0);
 
   // 4a. out-of-domain samples: every trace column and composition half at the OODS point.
   t0 = Clock::now();
-  const SecurePoint oods = random_point(ch);
-  stwo_hip_secure_point sp;
-  memcpy(sp.x, oods.x.v, 16);
-  memcpy(sp.y, oods.y.v, 16);
   std::vector<const uint32_t *> sampled(n_cols + 8);
-  std::vector<uint32_t> values(4 * (n_cols + 8));
+  uint32_t *d_values = fs + kFsValues;
   if (store) {
     // from the coefficients (log_n)
     std::vector<uint32_t> sample_logs(n_cols + 8, log_n);
     for (uint32_t k = 0; k < n_cols; k++) sampled[k] = coef[k];
     for (int k = 0; k < 8; k++) sampled[n_cols + k] = halves[k];
-    rc = stwo_hip_eval_at_point(ctx, sampled.data(), sample_logs.data(), n_cols + 8, &sp, values.data());
+    rc = eval_at_point_dev(ctx, sampled.data(), sample_logs.data(), n_cols + 8, fs + kFsPoint, d_values);
   } else {
     // from the committed evaluations (log E, the lifting size: no doubling of
     // the point), one weight vector for every column; the FRI quotient
     // columns, written later, hold the weights meanwhile
     for (uint32_t k = 0; k < n_cols; k++) sampled[k] = eval[k];
     for (int k = 0; k < 8; k++) sampled[n_cols + k] = ceval[k];
-    rc = stwo_hip_barycentric_weights(ctx, E, &sp, quot);
-    if (!rc) rc = stwo_hip_barycentric_eval_at_point(ctx, sampled.data(), n_cols + 8, E, quot, values.data());
+    rc = barycentric_weights_dev(ctx, E, fs + kFsPoint + kOodsFactorWords, quot);
+    if (!rc) rc = barycentric_eval_dev(ctx, sampled.data(), n_cols + 8, E, quot, d_values);
   }
+  std::vector<uint32_t> fsh(prove_fs_words(n_cols));
+  if (!rc) rc = stwo_hip_download(ctx, fsh.data(), fs, fsh.size());
   if (rc) return rc;
+  std::array<uint32_t, 8> root1, root2;
+  memcpy(root1.data(), &fsh[kFsRoot1], 32);
+  memcpy(root2.data(), &fsh[kFsRoot2], 32);
+  proof->commitments.push_back(root1);
+  proof->commitments.push_back(root2);
+  ch.mix_root(root1.data());
+  const qm31 comp_coeff = ch.draw_secure_felt();
+  ch.mix_root(root2.data());
+  const SecurePoint oods = random_point(ch);
+  if (fsh[kFsChan + 9] || memcmp(&fsh[kFsChan], ch.digest, 32) != 0 || fsh[kFsChan + 8] != ch.n_draws ||
+      memcmp(&fsh[kFsCompCoeff], comp_coeff.v, 16) != 0)
+    return set_error(ctx, STWO_HIP_EDEVICE, "wide_fib_prove: the device channel did not reach the host's state");
+  stwo_hip_secure_point sp;
+  memcpy(sp.x, oods.x.v, 16);
+  memcpy(sp.y, oods.y.v, 16);
+  const std::vector<uint32_t> values(fsh.begin() + kFsValues, fsh.end());
   proof->sampled_values.resize(3);
   proof->sampled_values[1].resize(n_cols);
   proof->sampled_values[2].resize(8);
@@ -559,7 +588,6 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
   }
   rc = stwo_hip_compute_fri_quotients(ctx, qcols.data(), logs_e.data(), n_cols + 8, samples.data(), n_samples.data(),
                                       quot_coeff.v, E, quot);
-  if (!rc) rc = stwo_hip_sync(ctx);
   if (rc) return rc;
   proof->phase_ms[3] = ms_since(t0);
 
@@ -584,28 +612,35 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
 
   // 4e. FRI decommitment, then the trees at the query positions.
   t0 = Clock::now();
-  std::vector<uint32_t> qpos(config.fri.n_queries), unsorted(config.fri.n_queries);
-  uint32_t npos = 0;
-  rc = stwo_hip_fri_prover_decommit(ctx, pm.fri, ch.digest, &ch.n_draws, qpos.data(), &npos, unsorted.data());
-  if (rc) return rc;
-  qpos.resize(npos);
-  proof->query_positions = qpos;
+  // The FRI layers' and both trees' reads in one gather.
+  proof->decommitments.resize(3);
+  proof->queried_values.resize(3);
+  {
+    Gatherer g;
+    FriDecommitPlan fplan;
+    rc = fri_decommit_plan(ctx, pm.fri, ch.digest, &ch.n_draws, g, fplan);
+    if (rc) return rc;
+    proof->query_positions = fplan.queries;
+    std::vector<TreeDecommit> trees(2);
+    trees[0].tree = tree1;
+    trees[0].cols.assign(eval.begin(), eval.end());
+    trees[0].values = &proof->queried_values[1];
+    trees[0].witness = &proof->decommitments[1];
+    trees[1].tree = tree2;
+    trees[1].cols.assign(ceval.begin(), ceval.end());
+    trees[1].values = &proof->queried_values[2];
+    trees[1].witness = &proof->decommitments[2];
+    plan_trees(g, E, fplan.queries, trees);
+    rc = g.run(ctx);
+    if (rc) return rc;
+    fri_decommit_finish(pm.fri, g, fplan);
+    finish_trees(g, fplan.queries.size(), trees);
+  }
   rc = fetch_layer(ctx, pm.fri, 0, proof->first_layer);
   if (rc) return rc;
   proof->inner_layers.resize(n_roots - 1);
   for (uint32_t k = 1; k < n_roots; k++) {
     rc = fetch_layer(ctx, pm.fri, k, proof->inner_layers[k - 1]);
-    if (rc) return rc;
-  }
-  proof->decommitments.resize(3);
-  proof->queried_values.resize(3);
-  {
-    std::vector<TreeDecommit> trees(2);
-    trees[0] = {tree1, std::vector<const uint32_t *>(eval.begin(), eval.end()), &proof->queried_values[1],
-                &proof->decommitments[1]};
-    trees[1] = {tree2, std::vector<const uint32_t *>(ceval.begin(), ceval.end()), &proof->queried_values[2],
-                &proof->decommitments[2]};
-    rc = decommit_trees(ctx, E, qpos, trees);
     if (rc) return rc;
   }
   proof->phase_ms[6] = ms_since(t0);

@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
@@ -132,40 +133,50 @@ __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t 
   }
 }
 
+// The per-column weights from alpha on the device (one lane per column):
+// with w_j = alpha^(n_cons - 1 - j), W1_i = w_{i-2} (i >= 2) and
+// W2_i = -(w_i [i < n_cons] + w_{i-1} [1 <= i <= n_cons]); 8 words per
+// column (W1 then W2).
+__device__ __forceinline__ qm31 qm31_pow(qm31 a, uint32_t e) {
+  qm31 r = qm31_make(1, 0, 0, 0);
+  while (e) {
+    if (e & 1) r = qm31_mul(r, a);
+    a = qm31_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+__global__ void __launch_bounds__(256) wide_fib_weights_kernel(const uint32_t *alpha_dev, uint32_t n_cols,
+                                                               uint32_t *weights) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cols) return;
+  const qm31 alpha = qm31_make(alpha_dev[0], alpha_dev[1], alpha_dev[2], alpha_dev[3]);
+  const uint32_t n_cons = n_cols - 2;
+  const qm31 zero = qm31_make(0, 0, 0, 0);
+  // w_j for j = i - 2, i - 1, i: alpha^(n_cons + 1 - i), ^(n_cons - i), ^(n_cons - 1 - i)
+  const qm31 w1 = i >= 2 ? qm31_pow(alpha, n_cons + 1 - i) : zero;
+  qm31 w2 = zero;
+  if (i < n_cons) w2 = qm31_add(w2, qm31_pow(alpha, n_cons - 1 - i));
+  if (i >= 1 && i - 1 < n_cons) w2 = qm31_add(w2, qm31_pow(alpha, n_cons - i));
+  w2 = qm31_sub(zero, w2);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    weights[8 * (size_t)i + k] = w1.v[k];
+    weights[8 * (size_t)i + 4 + k] = w2.v[k];
+  }
+}
+
 int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
                                        uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4],
-                                       bool accumulate) {
+                                       bool accumulate, const uint32_t *alpha_dev) {
   if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, cols && random_coeff && accum && n_cols >= 3 && n_cols <= 4096 && log_n >= 1 &&
-                              log_n <= 27,
+  STWO_HIP_CHECK_ARG(ctx, cols && (random_coeff || alpha_dev) && accum && n_cols >= 3 && n_cols <= 4096 &&
+                              log_n >= 1 && log_n <= 27,
                      "wide_fib_constraint_quotients: bad arguments");
   for (uint32_t k = 0; k < n_cols; k++)
     STWO_HIP_CHECK_ARG(ctx, cols[k] != nullptr, "wide_fib_constraint_quotients: NULL column");
   for (int k = 0; k < 4; k++) STWO_HIP_CHECK_ARG(ctx, accum[k] != nullptr, "wide_fib_constraint_quotients: NULL accumulator");
   const uint32_t eval_log = log_n + 1;
-  const uint32_t n_cons = n_cols - 2;
-  // Powers alpha^0 .. alpha^(n_cons - 1), reversed: constraint j uses
-  // w_j = alpha^(n_cons - 1 - j); per column the weights W1, W2 of the kernel.
-  std::vector<qm31> pw(n_cons);
-  {
-    const qm31 alpha = qm31_make(random_coeff[0], random_coeff[1], random_coeff[2], random_coeff[3]);
-    qm31 p = qm31_make(1, 0, 0, 0);
-    for (uint32_t e = 0; e < n_cons; e++) {
-      pw[n_cons - 1 - e] = p;
-      p = qm31_mul(p, alpha);
-    }
-  }
-  std::vector<uint32_t> powers(8 * (size_t)n_cols);
-  for (uint32_t i = 0; i < n_cols; i++) {
-    const qm31 zero = qm31_make(0, 0, 0, 0);
-    const qm31 w1 = i >= 2 ? pw[i - 2] : zero;
-    qm31 w2 = zero;
-    if (i < n_cons) w2 = qm31_add(w2, pw[i]);
-    if (i >= 1 && i - 1 < n_cons) w2 = qm31_add(w2, pw[i - 1]);
-    w2 = qm31_sub(zero, w2);
-    memcpy(&powers[8 * i], w1.v, 16);
-    memcpy(&powers[8 * i + 4], w2.v, 16);
-  }
   // Vanishing-polynomial inverses at the first 2^(eval_log - log_n) = 2 points of
   // CanonicCoset(eval_log).circle_domain(), bit-reversed.  The trace coset is
   // canonic, so its shift (-initial + step / 2) is zero and v(p) = pi^(log_n - 1)(p.x).
@@ -179,17 +190,25 @@ int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const 
     STWO_HIP_CHECK_ARG(ctx, x != 0, "wide_fib_constraint_quotients: vanishing polynomial is zero");
     dinv[bitrev32(i, 1)] = m31_inv(x);
   }
-  void *d_cols = nullptr, *d_pw = nullptr, *d_dinv = nullptr;
+  void *d_cols = nullptr, *d_alpha = nullptr, *d_dinv = nullptr;
   {
-    const void *srcs[3] = {cols, powers.data(), dinv};
-    const size_t bytes[3] = {n_cols * sizeof(uint32_t *), powers.size() * 4, sizeof dinv};
+    const uint32_t unused[4] = {0, 0, 0, 0};
+    const void *srcs[3] = {cols, random_coeff ? (const void *)random_coeff : (const void *)unused, dinv};
+    const size_t bytes[3] = {n_cols * sizeof(uint32_t *), 16, sizeof dinv};
     void *d[3];
     int rc = upload_tables(ctx, 3, srcs, bytes, d);
     if (rc) return rc;
     d_cols = d[0];
-    d_pw = d[1];
+    d_alpha = d[1];
     d_dinv = d[2];
   }
+  // alpha: the caller's (uploaded) or one already on the device
+  void *d_pw = nullptr;
+  int rc = get_scratch(ctx, "wide_fib_weights", 32 * (size_t)n_cols, &d_pw);
+  if (rc) return rc;
+  rc = launch(ctx, "wide_fib_weights", wide_fib_weights_kernel, dim3((n_cols + 255) / 256), dim3(256), 0,
+              alpha_dev ? alpha_dev : (const uint32_t *)d_alpha, n_cols, (uint32_t *)d_pw);
+  if (rc) return rc;
   // Two rows per lane when every column is 8-byte aligned (eval_log >= 2).
   bool pairs = true;
   for (uint32_t k = 0; k < n_cols; k++) pairs = pairs && ((uintptr_t)cols[k] & 7) == 0;
@@ -223,7 +242,9 @@ int stwo_hip_wide_fib_trace(stwo_hip_ctx *ctx, uint32_t *const *cols, uint32_t n
 
 int stwo_hip_wide_fib_constraint_quotients(stwo_hip_ctx *ctx, const uint32_t *const *cols, uint32_t n_cols,
                                            uint32_t log_n, const uint32_t random_coeff[4], uint32_t *const accum[4]) {
-  return wide_fib_constraint_quotients_impl(ctx, cols, n_cols, log_n, random_coeff, accum, true);
+  if (!ctx) return STWO_HIP_EINVAL;
+  STWO_HIP_CHECK_ARG(ctx, random_coeff != nullptr, "wide_fib_constraint_quotients: NULL random coefficient");
+  return wide_fib_constraint_quotients_impl(ctx, cols, n_cols, log_n, random_coeff, accum, true, nullptr);
 }
 
 }  // extern "C"
