@@ -36,6 +36,8 @@ __global__ void __launch_bounds__(256) gather_kernel(const GatherOp *ops, uint32
   for (uint32_t w = 0; w < o.n; w++) out[o.dst + w] = o.src[w];
 }
 
+void Gatherer::reserve(size_t more_ops) { ops_.reserve(ops_.size() + more_ops); }
+
 size_t Gatherer::add(const uint32_t *src, uint32_t n) {
   size_t off = words;
   ops_.push_back({(uintptr_t)src, n, (uint32_t)off});
@@ -85,10 +87,14 @@ int Gatherer::run(stwo_hip_ctx *ctx) {
 void plan_merkle_decommit(Gatherer &g, const uint32_t *tree, uint32_t L, const std::vector<uint32_t> &queries,
                           MerkleDecommitPlan &plan) {
   plan.parents.clear();
-  std::vector<uint32_t> prev = queries;
+  std::vector<uint32_t> prev = queries, cur;
   prev.erase(std::unique(prev.begin(), prev.end()), prev.end());
+  // at most one parent per query and level: no reallocation below
+  plan.parents.reserve((size_t)L * prev.size());
+  cur.reserve(prev.size());
+  g.reserve((size_t)L * prev.size());
   for (uint32_t child_layer = L; child_layer >= 1; child_layer--) {
-    std::vector<uint32_t> cur;
+    cur.clear();
     const uint32_t *layer = tree + tree_layer_off(child_layer);
     for (size_t i = 0; i < prev.size();) {
       const uint32_t first = prev[i];

@@ -408,7 +408,7 @@ __global__ void chan_init_kernel(ChanInit init, uint32_t *chan) {
 // point's parameter (CirclePoint::get_random_point, STWO/core/circle.rs:169-181):
 // p = ((1 - t^2) / (1 + t^2), 2t / (1 + t^2)); point_out receives the point's
 // factors for eval_at_point, g_0 = p.y, g_1 = p.x, g_k = pi^(k-1)(p.x) for
-// k < 32 (pi(x) = 2x^2 - 1; 128 words), then p.x, p.y and
+// k <= point_log and 0 above (pi(x) = 2x^2 - 1; 128 words), then p.x, p.y and
 // vn = pi^(point_log - 1)(p.x) (12 words) for the barycentric weights.
 // One wave; the step runs on lanes 0-3.
 __global__ void __launch_bounds__(64) prove_channel_kernel(uint32_t *chan, ChanInit init, uint32_t has_init,
@@ -432,12 +432,16 @@ __global__ void __launch_bounds__(64) prove_channel_kernel(uint32_t *chan, ChanI
   const qm31 py = qm31_mul(qm31_add(t, t), inv);
   qm31 vn = px;
   for (int k = 0; k < 4; k++) point_out[k] = py.v[k];
+  // g_1 .. g_point_log (the samples read g_k only for k below their log
+  // size, which is at most point_log); zeros above
   qm31 x = px;
   for (uint32_t g = 1; g < 32; g++) {
     if (g == point_log) vn = x;
-    for (int k = 0; k < 4; k++) point_out[4 * g + k] = x.v[k];
-    const qm31 sq = qm31_mul(x, x);
-    x = qm31_sub(qm31_add(sq, sq), one);
+    for (int k = 0; k < 4; k++) point_out[4 * g + k] = g <= point_log ? x.v[k] : 0u;
+    if (g < point_log) {
+      const qm31 sq = qm31_mul(x, x);
+      x = qm31_sub(qm31_add(sq, sq), one);
+    }
   }
   for (int k = 0; k < 4; k++) {
     point_out[128 + k] = px.v[k];
@@ -1029,11 +1033,18 @@ int stwo_hip::fri_decommit_plan(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uint
   queries.erase(std::unique(queries.begin(), queries.end()), queries.end());
 
   plan.layers.assign(pr->layers.size(), FriDecommitPlan::Layer());
-  std::vector<uint32_t> lq = queries;
+  std::vector<uint32_t> lq = queries, folded;
+  folded.reserve(lq.size());
   for (size_t k = 0; k < pr->layers.size(); k++) {
     const FriLayerKeep &layer = pr->layers[k];
     const uint32_t step = layer.shape.fold_step;
     FriDecommitPlan::Layer &pl = plan.layers[k];
+    // at most 2^step positions per query
+    const size_t max_pos = lq.size() << step;
+    pl.positions.reserve(max_pos);
+    pl.val_off.reserve(4 * max_pos);
+    pl.witness.reserve(max_pos);
+    g.reserve(4 * max_pos);
     for (size_t i = 0; i < lq.size();) {
       size_t j = i;
       while (j < lq.size() && (lq[j] >> step) == (lq[i] >> step)) j++;
@@ -1049,7 +1060,7 @@ int stwo_hip::fri_decommit_plan(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uint
       i = j;
     }
     plan_merkle_decommit(g, layer.tree, layer.shape.log_size, pl.positions, pl.merkle);
-    std::vector<uint32_t> folded;
+    folded.clear();
     for (uint32_t q : lq) {
       uint32_t f = q >> step;
       if (folded.empty() || folded.back() != f) folded.push_back(f);

@@ -115,7 +115,9 @@ int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const 
 // `root` (copied to root_out) and draws a secure felt into draw_out.  With
 // point_out, the draw is turned into the OODS point: point_out gets
 // kOodsFactorWords of eval_at_point factors (g_0 = y, g_1 = x,
-// g_k = pi^(k-1)(x), k < 32), then x, y and pi^(point_log - 1)(x).
+// g_k = pi^(k-1)(x) for k <= point_log, zeros up to k = 31), then x, y and
+// pi^(point_log - 1)(x).  Columns sampled with these factors must have log
+// sizes of at most point_log.
 constexpr uint32_t kOodsFactorWords = 128;
 constexpr uint32_t kOodsPointWords = kOodsFactorWords + 12;
 int prove_channel_step(stwo_hip_ctx *ctx, uint32_t *chan, const uint8_t *init_digest, const uint32_t *root,
@@ -148,6 +150,7 @@ struct Gatherer {
   std::vector<uint32_t> out;
   // Returns the word offset of this read in `out`.
   size_t add(const uint32_t *src, uint32_t n);
+  void reserve(size_t more_ops);
   int run(stwo_hip_ctx *ctx);
 };
 

@@ -115,6 +115,19 @@ struct ByteWriter {
   }
 };
 
+// Per-column word runs of one tree, stored flat (column k's words follow
+// column k - 1's): the proof's Vec<Vec<_>> fields without an allocation per
+// column.
+struct ColumnRuns {
+  std::vector<uint32_t> len;   // words per column
+  std::vector<uint32_t> data;  // every column's words, in column order
+  // n columns of `words` words each, from src.
+  void assign_uniform(size_t n, uint32_t words, const uint32_t *src) {
+    len.assign(n, words);
+    data.assign(src, src + n * (size_t)words);
+  }
+};
+
 struct ProofLayer {
   std::vector<uint32_t> fri_witness;   // 4 words per QM31
   std::vector<uint32_t> hash_witness;  // 8 words per hash
@@ -126,10 +139,9 @@ struct ProofLayer {
 struct stwo_hip_stark_proof {
   stwo_hip_pcs_config config{};
   std::vector<std::array<uint32_t, 8>> commitments;
-  // [tree][column] -> 4 words per sample
-  std::vector<std::vector<std::vector<uint32_t>>> sampled_values;
-  std::vector<std::vector<uint32_t>> decommitments;  // [tree] -> 8 words per hash
-  std::vector<std::vector<std::vector<uint32_t>>> queried_values;  // [tree][column] -> values
+  std::vector<stwo_hip::ColumnRuns> sampled_values;  // [tree]: per column 4 words per sample
+  std::vector<std::vector<uint32_t>> decommitments;   // [tree] -> 8 words per hash
+  std::vector<stwo_hip::ColumnRuns> queried_values;  // [tree]: per column one word per query
   uint64_t proof_of_work = 0;
   stwo_hip::ProofLayer first_layer;
   std::vector<stwo_hip::ProofLayer> inner_layers;
@@ -155,11 +167,13 @@ static void write_proof(const stwo_hip_stark_proof &p, ByteWriter &w) {
   w.u64(p.commitments.size());
   for (const auto &c : p.commitments) w.words(c.data(), 8);
   w.u64(p.sampled_values.size());
-  for (const auto &tree : p.sampled_values) {
-    w.u64(tree.size());
-    for (const auto &col : tree) {
-      w.u64(col.size() / 4);
-      w.words(col.data(), col.size());
+  for (const ColumnRuns &tree : p.sampled_values) {
+    w.u64(tree.len.size());
+    size_t off = 0;
+    for (uint32_t n : tree.len) {
+      w.u64(n / 4);
+      w.words(tree.data.data() + off, n);
+      off += n;
     }
   }
   w.u64(p.decommitments.size());
@@ -168,11 +182,13 @@ static void write_proof(const stwo_hip_stark_proof &p, ByteWriter &w) {
     w.words(d.data(), d.size());
   }
   w.u64(p.queried_values.size());
-  for (const auto &tree : p.queried_values) {
-    w.u64(tree.size());
-    for (const auto &col : tree) {
-      w.u64(col.size());
-      w.words(col.data(), col.size());
+  for (const ColumnRuns &tree : p.queried_values) {
+    w.u64(tree.len.size());
+    size_t off = 0;
+    for (uint32_t n : tree.len) {
+      w.u64(n);
+      w.words(tree.data.data() + off, n);
+      off += n;
     }
   }
   w.u64(p.proof_of_work);
@@ -288,7 +304,7 @@ static int fetch_layer(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *fri, uint32
 struct TreeDecommit {
   const uint32_t *tree;
   std::vector<const uint32_t *> cols;
-  std::vector<std::vector<uint32_t>> *values;  // [column][query]
+  ColumnRuns *values;                          // [column][query]
   std::vector<uint32_t> *witness;              // 8 words per hash
   std::vector<size_t> val_off;
   MerkleDecommitPlan plan;
@@ -297,6 +313,8 @@ static void plan_trees(Gatherer &g, uint32_t L, const std::vector<uint32_t> &que
                        std::vector<TreeDecommit> &trees) {
   for (TreeDecommit &t : trees) {
     t.val_off.clear();
+    t.val_off.reserve(t.cols.size() * queries.size());
+    g.reserve(t.cols.size() * queries.size());
     for (const uint32_t *c : t.cols)
       for (uint32_t pos : queries) t.val_off.push_back(g.add(c + pos, 1));  // log size L: no lifting
     plan_merkle_decommit(g, t.tree, L, queries, t.plan);
@@ -305,9 +323,9 @@ static void plan_trees(Gatherer &g, uint32_t L, const std::vector<uint32_t> &que
 static void finish_trees(const Gatherer &g, size_t nq, std::vector<TreeDecommit> &trees) {
   for (TreeDecommit &t : trees) {
     const size_t nc = t.cols.size();
-    t.values->assign(nc, std::vector<uint32_t>(nq));
-    for (size_t c = 0; c < nc; c++)
-      for (size_t q = 0; q < nq; q++) (*t.values)[c][q] = g.out[t.val_off[c * nq + q]];
+    t.values->len.assign(nc, (uint32_t)nq);
+    t.values->data.resize(nc * nq);
+    for (size_t i = 0; i < nc * nq; i++) t.values->data[i] = g.out[t.val_off[i]];
     finish_merkle_decommit(g, t.plan, *t.witness, nullptr);
   }
 }
@@ -534,16 +552,30 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
   memcpy(sp.y, oods.y.v, 16);
   const std::vector<uint32_t> values(fsh.begin() + kFsValues, fsh.end());
   proof->sampled_values.resize(3);
-  proof->sampled_values[1].resize(n_cols);
-  proof->sampled_values[2].resize(8);
-  for (uint32_t k = 0; k < n_cols + 8; k++) {
-    auto &dst = k < n_cols ? proof->sampled_values[1][k] : proof->sampled_values[2][k - n_cols];
-    dst.assign(values.begin() + 4 * k, values.begin() + 4 * k + 4);
-  }
+  proof->sampled_values[1].assign_uniform(n_cols, 4, values.data());
+  proof->sampled_values[2].assign_uniform(8, 4, values.data() + 4 * n_cols);
   ch.mix_felts(values.data(), n_cols + 8);
 
+  proof->phase_ms[2] = ms_since(t0);
+
+  // 4b. FRI quotients over the flattened trees (trace columns, then composition halves).
+  t0 = Clock::now();
+  const qm31 quot_coeff = ch.draw_secure_felt();
+  std::vector<const uint32_t *> qcols(n_cols + 8);
+  for (uint32_t k = 0; k < n_cols; k++) qcols[k] = eval[k];
+  for (int k = 0; k < 8; k++) qcols[n_cols + k] = ceval[k];
+  std::vector<stwo_hip_point_sample> samples(n_cols + 8);
+  std::vector<uint32_t> n_samples(n_cols + 8, 1);
+  for (uint32_t k = 0; k < n_cols + 8; k++) {
+    samples[k].point = sp;
+    memcpy(samples[k].value, &values[4 * k], 16);
+  }
+  rc = stwo_hip_compute_fri_quotients(ctx, qcols.data(), logs_e.data(), n_cols + 8, samples.data(), n_samples.data(),
+                                      quot_coeff.v, E, quot);
+  if (rc) return rc;
   // prove_ex's self-check: composition value from the halves against the
-  // constraints at the OODS point.
+  // constraints at the OODS point (on the host while the quotient kernel runs;
+  // it changes no channel state, so the result is the reference's).
   {
     qm31 ev[8];
     for (int k = 0; k < 8; k++)
@@ -572,23 +604,6 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
                        "wide_fib_prove: constraints not satisfied (the composition value at the OODS point does "
                        "not match the trace samples)");
   }
-  proof->phase_ms[2] = ms_since(t0);
-
-  // 4b. FRI quotients over the flattened trees (trace columns, then composition halves).
-  t0 = Clock::now();
-  const qm31 quot_coeff = ch.draw_secure_felt();
-  std::vector<const uint32_t *> qcols(n_cols + 8);
-  for (uint32_t k = 0; k < n_cols; k++) qcols[k] = eval[k];
-  for (int k = 0; k < 8; k++) qcols[n_cols + k] = ceval[k];
-  std::vector<stwo_hip_point_sample> samples(n_cols + 8);
-  std::vector<uint32_t> n_samples(n_cols + 8, 1);
-  for (uint32_t k = 0; k < n_cols + 8; k++) {
-    samples[k].point = sp;
-    memcpy(samples[k].value, &values[4 * k], 16);
-  }
-  rc = stwo_hip_compute_fri_quotients(ctx, qcols.data(), logs_e.data(), n_cols + 8, samples.data(), n_samples.data(),
-                                      quot_coeff.v, E, quot);
-  if (rc) return rc;
   proof->phase_ms[3] = ms_since(t0);
 
   // 4c. FRI commit.
