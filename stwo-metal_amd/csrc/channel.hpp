@@ -9,6 +9,9 @@
 
 namespace stwo_hip {
 
+// POW_PREFIX of the proof-of-work hash (STWO/core/channel/blake2s.rs:122-139).
+constexpr uint32_t kPowPrefix = 0x12345678u;
+
 struct HostChannel {
   uint8_t digest[32];
   uint32_t n_draws;

@@ -84,6 +84,9 @@ struct stwo_hip_ctx {
 
   // Scratch device buffers kept across calls, keyed by purpose.
   std::map<std::string, std::pair<void *, size_t>> scratch;
+  // Host copies of the tables upload_tables_cached last placed in its
+  // device buffers, by call site.
+  std::map<std::string, std::vector<uint8_t>> table_shadow;
 
   // FFT planner choices by shape (fft_tune_key); absent shapes use the defaults.
   std::map<uint64_t, stwo_hip::FftTune> fft_tune;
@@ -130,6 +133,14 @@ int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr);
 // made before the sequence's first launch.  Sets larger than half the ring
 // go through a scratch buffer with a synchronous copy.
 int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const size_t *bytes, void **dptrs);
+// upload_tables for tables that repeat from call to call (column lists of
+// buffers a caller reuses): a call whose tables equal the previous call's
+// under `key` gets the device copy kept from then, with no copy on the
+// stream; otherwise the tables go through the ring and are copied into the
+// kept buffer (stream-ordered) for the next call.  Within a serving job it
+// is upload_tables.
+int upload_tables_cached(stwo_hip_ctx *ctx, const char *key, size_t n, const void *const *srcs, const size_t *bytes,
+                         void **dptrs);
 
 // Enqueues the copy of the pending tables (none: nothing to do).
 int flush_uploads(stwo_hip_ctx *ctx);

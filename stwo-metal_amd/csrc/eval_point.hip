@@ -281,7 +281,9 @@ static int eval_at_point_impl(stwo_hip_ctx *ctx, const uint32_t *const *coeffs, 
     const void *srcs[3] = {tab.data(), work.data(), &f};
     const size_t bytes[3] = {tab.size() * sizeof(EvalCol), work.size() * sizeof(EvalWork), sizeof f};
     void *d[3];
-    rc = upload_tables(ctx, d_factors ? 2 : 3, srcs, bytes, d);
+    // with the point on the device (the prove) the tables repeat from call to call
+    rc = (d_factors && !na_scratch) ? upload_tables_cached(ctx, "eval_at_point", 2, srcs, bytes, d)
+                                    : upload_tables(ctx, d_factors ? 2 : 3, srcs, bytes, d);
     if (rc) return rc;
     d_tab = d[0];
     d_work = d[1];

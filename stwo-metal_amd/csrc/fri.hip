@@ -459,6 +459,106 @@ int prove_channel_step(stwo_hip_ctx *ctx, uint32_t *chan, const uint8_t *init_di
                 (uint32_t)(init_digest != nullptr), root, root_out, draw_out, point_out, point_log);
 }
 
+// ---- the end of the FRI commit and the grind's prefix on the device ------
+//
+// For a last layer of at most 2^kDevLastMax values: LinePoly::interpolate of
+// the last evaluation (fri.rs:274-291; line.rs:131-160: into natural order,
+// the line IFFT with x^-1 twiddles of each doubling of the line domain,
+// 1/n), into_ordered_coefficients (bit reverse), the degree check (every
+// coefficient at or above 2^bound_log must be zero, else flags[0] = 1),
+// from_ordered_coefficients (bit reverse of the kept ones) into poly, then
+// mix_felts(poly) into the channel (blake2s.rs:58-68: H(digest || poly)),
+// then the proof-of-work prefix H(le32(POW_PREFIX) || 0^12 || digest ||
+// le32(pow_bits)) (blake2s.rs:122-139) into prefixed, and the grind's
+// best-nonce word reset.  One workgroup; the hashing on lanes 0-3.
+// The natural-order butterfly i of a line domain of log m uses the inverse x
+// of point i, held at itw_end - 2^m + bitrev_{m-1}(i) (the tree keeps each
+// doubling's x's of the first half in bit-reversed order).
+constexpr uint32_t kDevLastMax = 6;
+struct LastLayerArgs {
+  const uint32_t *src[4];
+  const uint32_t *itw_end;
+  uint32_t lg, bound_log, pow_bits, n_inv;
+  uint32_t *chan, *poly, *prefixed, *flags;
+  unsigned long long *best;
+};
+__global__ void __launch_bounds__(256) fri_last_layer_kernel(LastLayerArgs a) {
+  __shared__ uint32_t v[4][1u << kDevLastMax];
+  __shared__ uint32_t msg[8 + (4u << kDevLastMax) + 16];
+  const uint32_t tid = threadIdx.x, n = 1u << a.lg;
+  if (tid == 0) a.flags[0] = 0;
+  for (uint32_t j = tid; j < n; j += blockDim.x)
+    for (int c = 0; c < 4; c++) v[c][j] = a.src[c][bitrev32(j, a.lg)];
+  __syncthreads();
+  for (uint32_t m = a.lg; m >= 1; m--) {
+    const uint32_t half = 1u << (m - 1);
+    const uint32_t *lvl = a.itw_end - ((size_t)1 << m);
+    for (uint32_t b = tid; b < n / 2; b += blockDim.x) {
+      const uint32_t i = b & (half - 1), base = (b >> (m - 1)) << m;
+      const uint32_t xinv = lvl[bitrev32(i, m - 1)];
+      for (int c = 0; c < 4; c++) {
+        const uint32_t x = v[c][base + i], y = v[c][base + i + half];
+        v[c][base + i] = m31_add(x, y);
+        v[c][base + i + half] = m31_mul(m31_sub(x, y), xinv);
+      }
+    }
+    __syncthreads();
+  }
+  const uint32_t bound = 1u << a.bound_log;
+  if (tid < 8) msg[tid] = a.chan[tid];
+  for (uint32_t j = tid; j < n; j += blockDim.x) {
+    // ordered coefficient j = natural-order IFFT output at bitrev(j), times 1/n
+    uint32_t cf[4];
+    bool nz = false;
+    for (int c = 0; c < 4; c++) {
+      cf[c] = m31_mul(v[c][bitrev32(j, a.lg)], a.n_inv);
+      nz = nz || cf[c] != 0;
+    }
+    if (j >= bound) {
+      if (nz) a.flags[0] = 1;
+    } else {
+      const uint32_t o = 4 * bitrev32(j, a.bound_log);
+      for (int c = 0; c < 4; c++) {
+        a.poly[o + c] = cf[c];
+        msg[8 + o + c] = cf[c];
+      }
+    }
+  }
+  __syncthreads();
+  if (tid >= 4) return;
+  const uint32_t j = tid;
+  // digest = H(digest || poly): 32 + 16 bound bytes, streamed as the blake2
+  // crate buffers it (the last block, full or partial, is the finalized one)
+  const uint32_t total = 32 + 16 * bound, n_blocks = (total + 63) / 64;
+  uint32_t lo = b2_iv_lo(j), hi = b2_iv_hi(j);
+  for (uint32_t k = 0; k < n_blocks; k++) {
+    uint32_t m[16];
+    for (uint32_t w = 0; w < 16; w++) {
+      const uint32_t idx = 16 * k + w;
+      m[w] = idx < total / 4 ? msg[idx] : 0u;
+    }
+    const bool last = k + 1 == n_blocks;
+    b2_compress_quad(lo, hi, m, last ? total : 64 * (k + 1), last);
+  }
+  uint32_t d[8];
+  b2_quad_gather(lo, hi, d);
+  a.chan[j] = d[j];
+  a.chan[4 + j] = d[4 + j];
+  if (j == 0) a.chan[8] = 0;  // mixing resets the draw counter
+  uint32_t m[16];
+  m[0] = kPowPrefix;
+  m[1] = m[2] = m[3] = 0;
+  for (int w = 0; w < 8; w++) m[4 + w] = d[w];
+  m[12] = a.pow_bits;
+  m[13] = m[14] = m[15] = 0;
+  lo = b2_iv_lo(j);
+  hi = b2_iv_hi(j);
+  b2_compress_quad(lo, hi, m, 52, true);
+  a.prefixed[j] = lo;
+  a.prefixed[4 + j] = hi;
+  if (j == 0) *a.best = ~0ull;
+}
+
 static int check_line_domain(stwo_hip_ctx *ctx, stwo_hip_coset dom, const stwo_hip_twiddles *tw) {
   STWO_HIP_CHECK_ARG(ctx, tw != nullptr && tw->device == ctx->device, "twiddles missing or from another device");
   STWO_HIP_CHECK_ARG(ctx, dom.log_size <= tw->root.log_size, "domain log %u exceeds twiddle root log %u",
@@ -643,10 +743,19 @@ static int check_fri_args(stwo_hip_ctx *ctx, const stwo_hip_fri_config &cfg, uin
 // folded evaluation stays in `keep->mem` (laid out by the caller from
 // fri_schedule); without, one scratch tree and two scratch layer buffers
 // are reused.
+// grind: NULL, or (prove) the end of the commit and the proof of work's first
+// search range run on the device when the last layer allows
+// (fri_last_layer_kernel): grind->found then says whether that range held
+// the nonce.
+struct FriGrind {
+  uint32_t pow_bits;
+  bool found;
+  uint64_t nonce;
+};
 static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
                            const stwo_hip_fri_config &cfg, const uint32_t *const column[4], uint32_t log_size,
                            const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly,
-                           stwo_hip_fri_prover *keep) {
+                           stwo_hip_fri_prover *keep, FriGrind *grind = nullptr) {
   int rc = check_fri_args(ctx, cfg, log_size, tw);
   if (rc) return rc;
   const uint32_t last_log = cfg.log_last_layer_degree_bound + cfg.log_blowup_factor;
@@ -672,7 +781,12 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
     rc = get_scratch(ctx, "fri_layer_b", ((size_t)4 << (log_size - 1)) * 4, &lay_b);
     if (rc) return rc;
   }
-  rc = get_scratch(ctx, "fri_state", (kRootsOff + 8 * kMaxRoots) * 4, &state_p);
+  // then flags[kFlagsOff], the grind's best nonce (8-byte aligned), the
+  // grind prefix and the last layer's coefficients (device end only)
+  constexpr uint32_t kFlagsOff = kRootsOff + 8 * kMaxRoots, kBestOff = kFlagsOff + 2, kPrefixOff = kBestOff + 2,
+                     kPolyOff = kPrefixOff + 8;
+  static_assert(kBestOff % 2 == 0, "best nonce alignment");
+  rc = get_scratch(ctx, "fri_state", (kPolyOff + (4u << kDevLastMax)) * 4, &state_p);
   if (rc) return rc;
   uint32_t *scratch_tree = (uint32_t *)tree_p;
   uint32_t *state = (uint32_t *)state_p;
@@ -834,6 +948,57 @@ static int fri_commit_impl(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32
   // The fold into the last layer when no tail ran (or the circle fold alone).
   rc = flush();
   if (rc) return rc;
+  if (grind) grind->found = false;
+  if (grind && lg <= kDevLastMax && grind->pow_bits + 6 < 20) {
+    // The last layer, the channel mix and the grind's first range on the
+    // device, then one synchronization.
+    const uint32_t bound_log = cfg.log_last_layer_degree_bound;
+    LastLayerArgs la{};
+    for (int k = 0; k < 4; k++) la.src[k] = cur[k];
+    la.itw_end = tw->d_itw + tw->len;
+    la.lg = lg;
+    la.bound_log = bound_log;
+    la.pow_bits = grind->pow_bits;
+    la.n_inv = m31_inv((uint32_t)(((uint64_t)1 << lg) % P));
+    la.chan = d_chan;
+    la.poly = state + kPolyOff;
+    la.prefixed = state + kPrefixOff;
+    la.flags = state + kFlagsOff;
+    la.best = (unsigned long long *)(state + kBestOff);
+    rc = launch(ctx, "fri_last_layer", fri_last_layer_kernel, dim3(1), dim3(256), 0, la);
+    if (rc) return rc;
+    rc = grind_first_range(ctx, state + kPrefixOff, grind->pow_bits, la.best);
+    if (rc) return rc;
+    const size_t bound = (size_t)1 << bound_log;
+    std::vector<uint32_t> st(kPolyOff + 4 * bound);
+    {
+      const DownloadReq req = {st.data(), state, st.size()};
+      rc = download_many(ctx, &req, 1);
+      if (rc) return rc;
+    }
+    if (st[9]) return set_error(ctx, STWO_HIP_EDEVICE, "fri_commit: channel draw did not terminate");
+    if (nr != sched.size()) return set_error(ctx, STWO_HIP_EDEVICE, "internal: FRI schedule mismatch");
+    if (st[kFlagsOff]) return set_error(ctx, STWO_HIP_EDEGREE, "invalid degree");
+    memcpy(roots, st.data() + kRootsOff, 32 * nr);
+    if (keep)
+      for (uint32_t k = 0; k < nr; k++) memcpy(keep->layers[k].root, roots + 8 * k, 32);
+    memcpy(last_layer_poly, st.data() + kPolyOff, 16 * bound);
+    memcpy(channel_digest, st.data(), 32);
+    *channel_n_draws = st[8];
+    uint64_t best;
+    memcpy(&best, st.data() + kBestOff, 8);
+    if (best != ~0ull) {
+      grind->found = true;
+      grind->nonce = best;
+    }
+    *n_roots = nr;
+    if (keep) {
+      keep->last_layer_poly.assign(last_layer_poly, last_layer_poly + 4 * bound);
+      keep->last_log = lg;
+      for (int k = 0; k < 4; k++) keep->last_layer_eval[k] = cur[k];
+    }
+    return STWO_HIP_OK;
+  }
   // One synchronization for the channel state, the roots and the last layer.
   std::vector<uint32_t> st(kRootsOff + 8 * nr);
   size_t n = (size_t)1 << lg;
@@ -904,7 +1069,7 @@ int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], ui
                                const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots,
                                uint32_t *last_layer_poly, stwo_hip_fri_prover **out) {
   return fri_prover_commit_mem(ctx, channel_digest, channel_n_draws, cfg, column, log_size, tw, roots, n_roots,
-                               last_layer_poly, nullptr, out);
+                               last_layer_poly, nullptr, out, -1, nullptr, nullptr);
 }
 
 }  // extern "C"
@@ -912,7 +1077,8 @@ int stwo_hip_fri_prover_commit(stwo_hip_ctx *ctx, uint8_t channel_digest[32], ui
 int stwo_hip::fri_prover_commit_mem(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
                                     stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
                                     const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots,
-                                    uint32_t *last_layer_poly, const char *scratch_key, stwo_hip_fri_prover **out) {
+                                    uint32_t *last_layer_poly, const char *scratch_key, stwo_hip_fri_prover **out,
+                                    int pow_bits, uint64_t *nonce, bool *nonce_found) {
   if (!ctx) return STWO_HIP_EINVAL;
   STWO_HIP_CHECK_ARG(ctx, channel_digest && channel_n_draws && column && roots && n_roots && last_layer_poly && out,
                      "fri_prover_commit: NULL argument");
@@ -958,8 +1124,14 @@ int stwo_hip::fri_prover_commit_mem(stwo_hip_ctx *ctx, uint8_t channel_digest[32
     pr->layers[k].tree = pr->mem + tree_off[k];
   }
   for (size_t f = 0; f < sched.size(); f++) pr->evals.push_back(pr->mem + eval_off[f]);
+  FriGrind g{};
+  if (pow_bits >= 0) g.pow_bits = (uint32_t)pow_bits;
   rc = fri_commit_impl(ctx, channel_digest, channel_n_draws, cfg, column, log_size, tw, roots, n_roots,
-                       last_layer_poly, pr);
+                       last_layer_poly, pr, pow_bits >= 0 ? &g : nullptr);
+  if (!rc && nonce_found) {
+    *nonce_found = g.found;
+    if (g.found) *nonce = g.nonce;
+  }
   if (rc) {
     if (pr->owns_mem) stwo_hip_free(ctx, pr->mem);
     delete pr;

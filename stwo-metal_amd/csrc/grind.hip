@@ -16,12 +16,13 @@
 #include <algorithm>
 
 #include "blake2s.hpp"
+#include "channel.hpp"
 #include "common.hpp"
 #include "field.hpp"
+#include "internal.hpp"
 
 namespace stwo_hip {
 
-constexpr uint32_t kPowPrefix = 0x12345678u;
 constexpr uint32_t kGrindLowBits = 20;
 
 // Lane gid hashes nonce (hi0 + (gid >> 20), lo0 + (gid & (2^20 - 1))).
@@ -31,13 +32,15 @@ struct GrindArgs {
   uint32_t pow_bits;
 };
 
-__global__ void __launch_bounds__(256) grind_kernel(GrindArgs a, unsigned long long *best) {
+// d_prefixed: the prefix in device memory, in place of a.prefixed, when not NULL.
+__global__ void __launch_bounds__(256) grind_kernel(GrindArgs a, const uint32_t *d_prefixed, unsigned long long *best) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t lo = a.lo0 + (gid & ((1u << kGrindLowBits) - 1));
   const uint32_t hi = a.hi0 + (gid >> kGrindLowBits);
+  const uint32_t *pre = d_prefixed ? d_prefixed : a.prefixed;
   uint32_t m[16];
 #pragma unroll
-  for (int k = 0; k < 8; k++) m[k] = a.prefixed[k];
+  for (int k = 0; k < 8; k++) m[k] = pre[k];
   m[8] = lo;
   m[9] = hi;
 #pragma unroll
@@ -52,6 +55,17 @@ __global__ void __launch_bounds__(256) grind_kernel(GrindArgs a, unsigned long l
 
 __global__ void grind_reset_kernel(unsigned long long *best) {
   if (threadIdx.x == 0) *best = ~0ull;
+}
+
+// The first search range of stwo_hip_grind: lo < 2^(pow_bits + 6), hi = 0.
+static uint32_t first_range_log(uint32_t pow_bits) { return std::max<uint32_t>(8, pow_bits + 6); }
+
+int grind_first_range(stwo_hip_ctx *ctx, const uint32_t *d_prefixed, uint32_t pow_bits, unsigned long long *best) {
+  STWO_HIP_CHECK_ARG(ctx, first_range_log(pow_bits) < kGrindLowBits, "internal: grind range for %u bits", pow_bits);
+  GrindArgs a{};
+  a.pow_bits = pow_bits;
+  return launch(ctx, "grind", grind_kernel, dim3((1u << first_range_log(pow_bits)) / 256), dim3(256), 0, a,
+                d_prefixed, best);
 }
 
 }  // namespace stwo_hip
@@ -87,7 +101,8 @@ int stwo_hip_grind(stwo_hip_ctx *ctx, const uint8_t channel_digest[32], uint32_t
     if (r) return r;
     a.hi0 = hi0;
     a.lo0 = lo0;
-    r = launch(ctx, "grind", grind_kernel, dim3(lanes / 256), dim3(256), 0, a, (unsigned long long *)d_best);
+    r = launch(ctx, "grind", grind_kernel, dim3(lanes / 256), dim3(256), 0, a, (const uint32_t *)nullptr,
+               (unsigned long long *)d_best);
     if (r) return r;
     uint64_t best = 0;
     r = stwo_hip_download(ctx, (uint32_t *)&best, (const uint32_t *)d_best, 2);
@@ -100,7 +115,7 @@ int stwo_hip_grind(stwo_hip_ctx *ctx, const uint8_t channel_digest[32], uint32_t
   };
   uint64_t h_start = 0;
   // lo < 2^k0, then the rest of hi = 0 (both multiples of 256 lanes).
-  const uint32_t k0 = std::max<uint32_t>(8, pow_bits + 6);
+  const uint32_t k0 = first_range_log(pow_bits);
   if (k0 < kGrindLowBits) {
     rc = search(0, 0, 1u << k0);
     if (rc) return rc == 1 ? STWO_HIP_OK : rc;

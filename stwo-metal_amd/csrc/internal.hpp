@@ -220,10 +220,22 @@ namespace stwo_hip {
 // stwo_hip_fri_prover_commit whose memory is the context's scratch buffer
 // `scratch_key` (kept by the context, reused by the next call with the same
 // key) when scratch_key is not NULL.  One such prover per key at a time.
+// With pow_bits >= 0 (the prove), the commit's end (last layer, its mix
+// into the channel) and the proof of work's first search range may run on
+// the device before the one synchronization: *nonce_found then says whether
+// *nonce holds the grind's result (otherwise stwo_hip_grind finds it).
 int fri_prover_commit_mem(stwo_hip_ctx *ctx, uint8_t channel_digest[32], uint32_t *channel_n_draws,
                           stwo_hip_fri_config cfg, const uint32_t *const column[4], uint32_t log_size,
                           const stwo_hip_twiddles *tw, uint32_t *roots, uint32_t *n_roots, uint32_t *last_layer_poly,
-                          const char *scratch_key, stwo_hip_fri_prover **out);
+                          const char *scratch_key, stwo_hip_fri_prover **out, int pow_bits = -1,
+                          uint64_t *nonce = nullptr, bool *nonce_found = nullptr);
+
+// The proof of work's first search range (the lowest 2^(pow_bits + 6) lo
+// words of hi = 0) with the prefix H(POW_PREFIX || 0^12 || digest ||
+// pow_bits) already on the device and *best reset to all ones: the smallest
+// valid nonce of the range is left in *best (all ones if none).  Requires
+// pow_bits + 6 < 20.
+int grind_first_range(stwo_hip_ctx *ctx, const uint32_t *d_prefixed, uint32_t pow_bits, unsigned long long *best);
 
 // stwo_hip_fri_prover_decommit in two halves around a caller's gather, so
 // that other reads (the trace trees' decommitments) share its launch and

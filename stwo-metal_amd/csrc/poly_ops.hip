@@ -321,7 +321,9 @@ static int barycentric_eval_impl(stwo_hip_ctx *ctx, const uint32_t *const *evals
   void *d[2];
   const void *srcs[2] = {evals, work.data()};
   const size_t bytes[2] = {n_cols * sizeof(uint32_t *), work.size() * sizeof(BaryWork)};
-  int rc = upload_tables(ctx, 2, srcs, bytes, d);
+  // with the values left on the device (the prove) the tables repeat from call to call
+  int rc = d_values ? upload_tables_cached(ctx, "barycentric_eval", 2, srcs, bytes, d)
+                    : upload_tables(ctx, 2, srcs, bytes, d);
   if (rc) return rc;
   void *part = nullptr, *out = d_values;
   rc = get_scratch(ctx, "bary_partials", n_cols * (size_t)n_blocks * 16, &part);

@@ -611,8 +611,10 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
   std::vector<uint32_t> roots(8 * (E + 1));
   uint32_t n_roots = 0;
   std::vector<uint32_t> last(4u << config.fri.log_last_layer_degree_bound);
+  uint64_t nonce = 0;
+  bool nonce_found = false;
   rc = fri_prover_commit_mem(ctx, ch.digest, &ch.n_draws, config.fri, quot, E, tw, roots.data(), &n_roots,
-                             last.data(), "wide_fib_prove_fri", &pm.fri);
+                             last.data(), "wide_fib_prove_fri", &pm.fri, (int)config.pow_bits, &nonce, &nonce_found);
   if (rc) return rc;
   proof->last_layer_poly = last;
   proof->last_layer_log_size = config.fri.log_last_layer_degree_bound;
@@ -620,7 +622,15 @@ int stwo_hip_wide_fib_prove_ex(stwo_hip_ctx *ctx, const uint32_t *a, const uint3
 
   // 4d. proof of work.
   t0 = Clock::now();
-  rc = stwo_hip_grind(ctx, ch.digest, config.pow_bits, &proof->proof_of_work);
+  // (its first range ran with the FRI commit's end when that went on the device)
+  if (nonce_found) {
+    // the reference asserts both halves are below P (SIMD/grind.rs:50-57)
+    if ((nonce >> 32) >= P || (uint32_t)nonce >= P)
+      return set_error(ctx, STWO_HIP_EINVAL, "grind: nonce halves not below P");
+    proof->proof_of_work = nonce;
+  } else {
+    rc = stwo_hip_grind(ctx, ch.digest, config.pow_bits, &proof->proof_of_work);
+  }
   if (rc) return rc;
   ch.mix_u64(proof->proof_of_work);
   proof->phase_ms[5] = ms_since(t0);

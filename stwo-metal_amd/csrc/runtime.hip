@@ -187,6 +187,37 @@ int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const si
   return STWO_HIP_OK;
 }
 
+int upload_tables_cached(stwo_hip_ctx *ctx, const char *key, size_t n, const void *const *srcs, const size_t *bytes,
+                         void **dptrs) {
+  if (ctx->job_base) return upload_tables(ctx, n, srcs, bytes, dptrs);
+  size_t total = 0;
+  for (size_t i = 0; i < n; i++) total += table_span(bytes[i]);
+  const std::string skey = std::string("table_cache:") + key;
+  std::vector<uint8_t> &shadow = ctx->table_shadow[key];
+  bool same = shadow.size() == total;
+  for (size_t i = 0, off = 0; same && i < n; off += table_span(bytes[i]), i++)
+    same = !bytes[i] || memcmp(shadow.data() + off, srcs[i], bytes[i]) == 0;
+  auto it = ctx->scratch.find(skey);
+  if (same && it != ctx->scratch.end() && it->second.second >= total) {
+    for (size_t i = 0, off = 0; i < n; off += table_span(bytes[i]), i++) dptrs[i] = (uint8_t *)it->second.first + off;
+    return STWO_HIP_OK;
+  }
+  // New tables: through the ring for this call, then into the kept buffer.
+  shadow.clear();
+  int rc = upload_tables(ctx, n, srcs, bytes, dptrs);
+  if (rc) return rc;
+  void *dev = nullptr;
+  rc = get_scratch(ctx, skey.c_str(), total, &dev);
+  if (rc) return rc;
+  rc = flush_uploads(ctx);
+  if (rc) return rc;
+  STWO_HIP_TRY(ctx, hipMemcpyAsync(dev, dptrs[0], total, hipMemcpyDeviceToDevice, ctx->stream));
+  shadow.assign(total, 0);
+  for (size_t i = 0, off = 0; i < n; off += table_span(bytes[i]), i++)
+    if (bytes[i]) memcpy(shadow.data() + off, srcs[i], bytes[i]);
+  return STWO_HIP_OK;
+}
+
 int arena_quiesce(stwo_hip_ctx *ctx) {
   int rc = flush_uploads(ctx);
   if (rc) return rc;

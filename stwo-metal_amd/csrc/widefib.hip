@@ -196,7 +196,9 @@ int wide_fib_constraint_quotients_impl(stwo_hip_ctx *ctx, const uint32_t *const 
     const void *srcs[3] = {cols, random_coeff ? (const void *)random_coeff : (const void *)unused, dinv};
     const size_t bytes[3] = {n_cols * sizeof(uint32_t *), 16, sizeof dinv};
     void *d[3];
-    int rc = upload_tables(ctx, 3, srcs, bytes, d);
+    // with alpha on the device (the prove) the tables repeat from call to call
+    int rc = alpha_dev ? upload_tables_cached(ctx, "wide_fib_constraint_quotients", 3, srcs, bytes, d)
+                       : upload_tables(ctx, 3, srcs, bytes, d);
     if (rc) return rc;
     d_cols = d[0];
     d_alpha = d[1];

@@ -63,6 +63,11 @@ CASES = [
     (8, 64, pcs(8, 1, 1, 7, 3), "rand"),
     (10, 256, pcs(), "ref"),
     (12, 16, pcs(12, 3, 3, 20, 4), "rand"),
+    # the FRI commit's end and the grind's first range run on the device for
+    # last layers of at most 2^6 values and pow_bits below 14; these two sit
+    # on either side (2^6 values and 13 bits; 2^7 values and 14 bits)
+    (7, 8, pcs(13, 1, 5, 4, 1), "rand"),
+    (8, 8, pcs(14, 1, 6, 3, 1), "rand"),
 ]
 
 
