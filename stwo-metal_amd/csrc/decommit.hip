@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) gather_kernel(const GatherOp *ops, uint32
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_ops) return;
   const GatherOp o = ops[i];
-  for (uint32_t w = 0; w < o.n; w++) out[o.dst + w] = o.src[w];
+  for (uint32_t w = 0; w < o.n; w++) as_global(out)[o.dst + w] = as_global(o.src)[w];
 }
 
 void Gatherer::reserve(size_t more_ops) { ops_.reserve(ops_.size() + more_ops); }

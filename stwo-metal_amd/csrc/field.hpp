@@ -199,8 +199,23 @@ typedef __attribute__((address_space(1))) uint8_t g_u8;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 gconst_u4;
 typedef __attribute__((address_space(1))) u32x4 g_u4;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const u32x2 gconst_u2;
+typedef __attribute__((address_space(1))) u32x2 g_u2;
 __device__ __forceinline__ gconst_u32 *as_global(const uint32_t *p) { return (gconst_u32 *)p; }
 __device__ __forceinline__ g_u32 *as_global(uint32_t *p) { return (g_u32 *)p; }
+// 8- and 16-byte global loads and stores (p aligned to their size).  Flat
+// loads would also count against the LDS/scalar-load counter, so every wait
+// for a scalar load (a table entry) would wait for them too.
+__device__ __forceinline__ uint2 ld_global2(const uint32_t *p) {
+  const u32x2 v = *(gconst_u2 *)p;
+  return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 ld_global4(const uint32_t *p) {
+  const u32x4 v = *(gconst_u4 *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_global2(uint32_t *p, uint32_t a, uint32_t b) { *(g_u2 *)p = u32x2{a, b}; }
 }  // namespace stwo_hip
 
 namespace stwo_hip {

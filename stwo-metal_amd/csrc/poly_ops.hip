@@ -33,7 +33,7 @@
 namespace stwo_hip {
 
 constexpr uint32_t kBaryBlockLog = 11;
-constexpr uint32_t kBaryCols = 8;     // columns per inner step
+constexpr uint32_t kBaryCols = 4;     // columns per inner step
 constexpr uint32_t kBaryWgCols = 32;  // columns per workgroup (the weights are loaded once for them)
 
 struct BaryPointArgs {
@@ -67,7 +67,18 @@ __global__ void __launch_bounds__(256) barycentric_weights_kernel(BaryPointArgs 
     x = m31_sub(m31_add(sq, sq), 1);
   }
   const uint32_t s = m31_mul(m31_mul(m31_neg(m31_add(q.y, q.y)), a.four_pow), prod);
-  const uint32_t *pp = d_pt ? d_pt : a.px;  // a.px, a.py, a.vn are adjacent
+  uint32_t pp[12];  // x, y, vn
+  if (d_pt) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) pp[k] = as_global(d_pt)[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      pp[k] = a.px[k];
+      pp[4 + k] = a.py[k];
+      pp[8 + k] = a.vn[k];
+    }
+  }
   const qm31 px = qm31_make(pp[0], pp[1], pp[2], pp[3]);
   const qm31 py = qm31_make(pp[4], pp[5], pp[6], pp[7]);
   const qm31 vn = qm31_make(pp[8], pp[9], pp[10], pp[11]);
@@ -76,10 +87,10 @@ __global__ void __launch_bounds__(256) barycentric_weights_kernel(BaryPointArgs 
   const qm31 hy = qm31_sub(qm31_mul_m31(py, q.x), qm31_mul_m31(px, q.y));
   const qm31 num = qm31_mul(vn, qm31_add(qm31_make(1, 0, 0, 0), hx));
   const qm31 w = qm31_mul(num, qm31_inv(qm31_mul_m31(hy, s)));
-  w0[i] = w.v[0];
-  w1[i] = w.v[1];
-  w2[i] = w.v[2];
-  w3[i] = w.v[3];
+  as_global(w0)[i] = w.v[0];
+  as_global(w1)[i] = w.v[1];
+  as_global(w2)[i] = w.v[2];
+  as_global(w3)[i] = w.v[3];
 }
 
 struct BaryWork {
@@ -88,7 +99,7 @@ struct BaryWork {
 
 // Block `w.block` (2^B rows, B = min(log, 11)) of columns col0 .. col0 + n - 1
 // (n <= kBaryWgCols).  VEC: every column and weight array is 16-byte aligned
-// and B >= 3, so a lane's 8 rows are two 16-byte loads.
+// and B = 11, so a lane's 8 rows are two 16-byte loads, all in the block.
 template <bool VEC>
 __global__ void __launch_bounds__(256) barycentric_dot_kernel(const uint32_t *const *cols, const BaryWork *work,
                                                               uint32_t log_size, const uint32_t *w0,
@@ -104,16 +115,12 @@ __global__ void __launch_bounds__(256) barycentric_dot_kernel(const uint32_t *co
   const uint32_t valid = threadIdx.x * 8 < nb ? min(8u, nb - threadIdx.x * 8) : 0u;
   auto load8 = [&](const uint32_t *src, uint32_t (&v)[8]) {
     if (VEC) {
-      if (valid) {
-        const uint4 a = *(const uint4 *)(src + r0), b = *(const uint4 *)(src + r0 + 4);
-        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = 0;
-      }
+      // whole blocks only: every lane's 8 rows are in the block
+      const uint4 a = ld_global4(src + r0), b = ld_global4(src + r0 + 4);
+      v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; e++) v[e] = (uint32_t)e < valid ? src[r0 + e] : 0u;
+      for (int e = 0; e < 8; e++) v[e] = (uint32_t)e < valid ? as_global(src)[r0 + e] : 0u;
     }
   };
   uint32_t w[4][8];
@@ -136,6 +143,15 @@ __global__ void __launch_bounds__(256) barycentric_dot_kernel(const uint32_t *co
 #pragma unroll
     for (uint32_t j = 0; j < kBaryCols; j++) {
       if (j >= gn) break;
+      // The weights and the column's words as fresh 32-bit values for each
+      // column: otherwise the compiler keeps their 64-bit zero extensions
+      // live across the columns, two registers per word.
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) asm volatile("" : "+v"(w[k][e]));
+#pragma unroll
+      for (int e = 0; e < 8; e++) asm volatile("" : "+v"(cv[j][e]));
       uint64_t acc[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int e = 0; e < 8; e++) {
@@ -147,6 +163,9 @@ __global__ void __launch_bounds__(256) barycentric_dot_kernel(const uint32_t *co
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) red[4 * j + k][threadIdx.x] = m31_reduce64(acc[k]);
+      // one column's products at a time: interleaving the columns would hold
+      // every column's accumulators at once and cut the waves per SIMD
+      __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
     const uint32_t r = threadIdx.x >> 3, seg = threadIdx.x & 7;
@@ -315,7 +334,7 @@ static int barycentric_eval_impl(stwo_hip_ctx *ctx, const uint32_t *const *evals
   for (uint32_t c = 0; c < n_cols; c += kBaryWgCols)
     for (uint32_t b = 0; b < n_blocks; b++)
       work.push_back({c, (uint32_t)std::min<size_t>(kBaryWgCols, n_cols - c), b});
-  bool vec = log_size >= 3;
+  bool vec = log_size >= kBaryBlockLog;  // whole 2^11-row blocks
   for (int k = 0; k < 4; k++) vec = vec && ((uintptr_t)weights[k] & 15) == 0;
   for (size_t c = 0; c < n_cols; c++) vec = vec && ((uintptr_t)evals[c] & 15) == 0;
   void *d[2];

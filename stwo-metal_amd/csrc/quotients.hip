@@ -124,10 +124,10 @@ __global__ void __launch_bounds__(256) quotient_kernel(const QuotPoint *pts, uin
     cm31_mul(num.v[2], num.v[3], i0, i1, r2, r3);
     q = qm31_add(q, qm31_make(r0, r1, r2, r3));
   }
-  o0[row] = q.v[0];
-  o1[row] = q.v[1];
-  o2[row] = q.v[2];
-  o3[row] = q.v[3];
+  as_global(o0)[row] = q.v[0];
+  as_global(o1)[row] = q.v[1];
+  as_global(o2)[row] = q.v[2];
+  as_global(o3)[row] = q.v[3];
 }
 
 // Every column at the lifting size (no lifting) and 8-byte aligned: a lane
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) quotient_pair_kernel(const QuotPoint *pts
     for (; e + 6 <= z.e_end; e += 6) {
       uint2 v[6];
 #pragma unroll
-      for (int u = 0; u < 6; u++) v[u] = *reinterpret_cast<const uint2 *>(ents[e + u].col + row);
+      for (int u = 0; u < 6; u++) v[u] = ld_global2(ents[e + u].col + row);
 #pragma unroll
       for (int u = 0; u < 6; u++) {
 #pragma unroll
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) quotient_pair_kernel(const QuotPoint *pts
       }
     }
     for (; e < z.e_end; e++) {
-      const uint2 v = *reinterpret_cast<const uint2 *>(ents[e].col + row);
+      const uint2 v = ld_global2(ents[e].col + row);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         acc[0][k] = m31_fold64(acc[0][k] + (uint64_t)v.x * ents[e].c[k]);
@@ -205,10 +205,10 @@ __global__ void __launch_bounds__(256) quotient_pair_kernel(const QuotPoint *pts
       q[rr] = qm31_add(q[rr], qm31_make(r0, r1, r2, r3));
     }
   }
-  *reinterpret_cast<uint2 *>(o0 + row) = make_uint2(q[0].v[0], q[1].v[0]);
-  *reinterpret_cast<uint2 *>(o1 + row) = make_uint2(q[0].v[1], q[1].v[1]);
-  *reinterpret_cast<uint2 *>(o2 + row) = make_uint2(q[0].v[2], q[1].v[2]);
-  *reinterpret_cast<uint2 *>(o3 + row) = make_uint2(q[0].v[3], q[1].v[3]);
+  st_global2(o0 + row, q[0].v[0], q[1].v[0]);
+  st_global2(o1 + row, q[0].v[1], q[1].v[1]);
+  st_global2(o2 + row, q[0].v[2], q[1].v[2]);
+  st_global2(o3 + row, q[0].v[3], q[1].v[3]);
 }
 
 static qm31 conj(const qm31 &v) { return qm31_make(v.v[0], v.v[1], m31_neg(v.v[2]), m31_neg(v.v[3])); }

@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t 
     for (int k = 0; k < 4; k++) a1[rr][k] = a2[rr][k] = 0;
   auto load = [&](const uint32_t *c, uint32_t (&x)[R]) {
     if constexpr (R == 2) {
-      const uint2 v = *reinterpret_cast<const uint2 *>(c + r0);
+      const uint2 v = ld_global2(c + r0);
       x[0] = v.x;
       x[1] = v.y;
     } else {
@@ -119,17 +119,9 @@ __global__ void __launch_bounds__(256) wide_fib_quotients_kernel(const uint32_t 
 #pragma unroll
     for (int k = 0; k < 4; k++) res.v[k] = m31_add(m31_reduce64(a1[rr][k]), m31_reduce64(a2[rr][k]));
     res = qm31_mul_m31(res, denom_inv[r >> log_n]);
-    if (accumulate) {
-      acc0[r] = m31_add(acc0[r], res.v[0]);
-      acc1[r] = m31_add(acc1[r], res.v[1]);
-      acc2[r] = m31_add(acc2[r], res.v[2]);
-      acc3[r] = m31_add(acc3[r], res.v[3]);
-    } else {
-      acc0[r] = res.v[0];
-      acc1[r] = res.v[1];
-      acc2[r] = res.v[2];
-      acc3[r] = res.v[3];
-    }
+    g_u32 *o[4] = {as_global(acc0), as_global(acc1), as_global(acc2), as_global(acc3)};
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k][r] = accumulate ? m31_add(o[k][r], res.v[k]) : res.v[k];
   }
 }
 
