@@ -82,6 +82,13 @@ struct stwo_hip_ctx {
   uint8_t *h_stage = nullptr;
   size_t stage_cap = 0;
 
+  // Pinned host buffer that the decommitment gather reads its op table from
+  // and writes its result to directly (no copy either way), and its device
+  // address.
+  uint8_t *h_gather = nullptr;
+  void *d_gather = nullptr;
+  size_t gather_cap = 0;
+
   // Scratch device buffers kept across calls, keyed by purpose.
   std::map<std::string, std::pair<void *, size_t>> scratch;
   // Host copies of the tables upload_tables_cached last placed in its

@@ -49,6 +49,20 @@ int Gatherer::run(stwo_hip_ctx *ctx) {
   out.assign(words, 0);
   if (ops_.empty()) return STWO_HIP_OK;
   STWO_HIP_CHECK_ARG(ctx, words < (1ull << 32) && ops_.size() < (1ull << 32), "decommit: gather too large");
+  const size_t ops_bytes = ops_.size() * sizeof(GatherOp), out_off = (ops_bytes + 255) & ~(size_t)255;
+  if (ctx->h_gather && out_off + words * 4 <= ctx->gather_cap) {
+    // The kernel reads the op table from pinned host memory and writes the
+    // result there: no copy on the stream in either direction.
+    GatherOp *h_ops = (GatherOp *)ctx->h_gather;
+    for (size_t i = 0; i < ops_.size(); i++) h_ops[i] = {(const uint32_t *)ops_[i].src, ops_[i].n, ops_[i].dst};
+    const uint32_t n_ops = (uint32_t)ops_.size();
+    int rc = launch(ctx, "decommit_gather", gather_kernel, dim3((n_ops + 255) / 256), dim3(256), 0,
+                    (const GatherOp *)ctx->d_gather, n_ops, (uint32_t *)((uint8_t *)ctx->d_gather + out_off));
+    if (rc) return rc;
+    STWO_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(out.data(), ctx->h_gather + out_off, words * 4);
+    return STWO_HIP_OK;
+  }
   std::vector<GatherOp> ops(ops_.size());
   for (size_t i = 0; i < ops_.size(); i++)
     ops[i] = {(const uint32_t *)ops_[i].src, ops_[i].n, ops_[i].dst};
@@ -57,7 +71,6 @@ int Gatherer::run(stwo_hip_ctx *ctx) {
   // staging buffer: one synchronization per gather.
   void *d_ops = nullptr, *d_out = nullptr;
   int rc;
-  const size_t ops_bytes = ops.size() * sizeof(GatherOp);
   if (ops_bytes <= ctx->arena_cap / 2) {
     rc = upload_table(ctx, ops.data(), ops_bytes, &d_ops);
     if (rc) return rc;

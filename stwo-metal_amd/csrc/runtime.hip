@@ -62,6 +62,7 @@ void prof_end(stwo_hip_ctx *ctx, const char *name, hipEvent_t start) {
 
 static const size_t kArenaBytes = 8u << 20;
 static const size_t kStageBytes = 1u << 20;
+static const size_t kGatherBytes = 2u << 20;
 static const int kEventPoolInit = 16;
 
 static int arena_init(stwo_hip_ctx *ctx) {
@@ -71,6 +72,11 @@ static int arena_init(stwo_hip_ctx *ctx) {
   ctx->arena_head = 0;
   STWO_HIP_TRY(ctx, hipHostMalloc((void **)&ctx->h_stage, kStageBytes, hipHostMallocDefault));
   ctx->stage_cap = kStageBytes;
+  // coherent (fine-grained): the gather kernel's reads and writes of it are
+  // not held in the GPU's caches across launches
+  STWO_HIP_TRY(ctx, hipHostMalloc((void **)&ctx->h_gather, kGatherBytes, hipHostMallocCoherent));
+  STWO_HIP_TRY(ctx, hipHostGetDevicePointer(&ctx->d_gather, ctx->h_gather, 0));
+  ctx->gather_cap = kGatherBytes;
   return STWO_HIP_OK;
 }
 
@@ -389,6 +395,7 @@ int stwo_hip_ctx_destroy(stwo_hip_ctx *ctx) {
   for (auto &kv : ctx->scratch) (void)hipFree(kv.second.first);
   if (ctx->h_arena) (void)hipHostFree(ctx->h_arena);
   if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+  if (ctx->h_gather) (void)hipHostFree(ctx->h_gather);
   if (ctx->d_arena) (void)hipFree(ctx->d_arena);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
