@@ -80,6 +80,13 @@ int stwo_hip_last_error(const stwo_hip_ctx *ctx, char *buf, size_t len);
 int stwo_hip_sync(stwo_hip_ctx *ctx);
 /* The context's hipStream_t, for callers that order their own work. */
 void *stwo_hip_stream(stwo_hip_ctx *ctx);
+/* Test hook for the error paths; no counterpart in the reference.  After
+ * this call the context's next `after` device allocations succeed and every
+ * later one fails as an exhausted device would (STWO_HIP_ENOMEM with a
+ * message), until it is called again with after < 0 (off, the default).
+ * Counted: the library's own scratch, tables, plans and twiddle trees, and
+ * stwo_hip_alloc; not allocations made before the call. */
+int stwo_hip_ctx_fail_allocs_after(stwo_hip_ctx *ctx, long long after);
 
 /* ---- device memory ------------------------------------------------------
  * Replaces stwo_metal_u32_buffer_{alloc_zeroed,alloc_uninitialized,destroy,

@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# STWO_ORACLE_LIB: another build of the same source (the sanitizer build).
+_LIB_PATH = os.environ.get("STWO_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 P = (1 << 31) - 1
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)

@@ -29,7 +29,8 @@ def lib(variant=None):
     if variant is None and _lib is not None:
         return _lib
     v = variant or ("v4" if host_has_avx512() else "v3")
-    path = os.path.join(HERE, "libsimdport_%s.so" % v)
+    # STWO_SIMDPORT_LIB: another build of the same source (the sanitizer build)
+    path = os.environ.get("STWO_SIMDPORT_LIB") or os.path.join(HERE, "libsimdport_%s.so" % v)
     if not os.path.exists(path):
         raise RuntimeError("%s is not built (make -C oracle)" % path)
     L = ctypes.CDLL(path)

@@ -104,6 +104,10 @@ struct stwo_hip_ctx {
   // (job_mem_end), so a job's device use stays inside its budget block.
   uint8_t *job_base = nullptr;
   size_t job_cap = 0, job_used = 0;
+
+  // Allocation-failure injection (stwo_hip_ctx_fail_allocs_after): device
+  // allocations left before they fail, or -1 (off).
+  long long fail_allocs_after = -1;
   std::vector<std::string> job_keys;  // scratch entries carved from the job block
 
   // Launch plans of the last prove shape, kept for the next prove of that
@@ -139,6 +143,9 @@ int upload_table(stwo_hip_ctx *ctx, const void *src, size_t bytes, void **dptr);
 // Every table a sequence of launches reads must come from one such call
 // made before the sequence's first launch.  Sets larger than half the ring
 // go through a scratch buffer with a synchronous copy.
+// hipMalloc through the context's allocation-failure injection: every
+// device allocation of the library goes through here.
+hipError_t ctx_malloc(stwo_hip_ctx *ctx, void **dptr, size_t bytes);
 int upload_tables(stwo_hip_ctx *ctx, size_t n, const void *const *srcs, const size_t *bytes, void **dptrs);
 // upload_tables for tables that repeat from call to call (column lists of
 // buffers a caller reuses): a call whose tables equal the previous call's

@@ -1178,90 +1178,6 @@ int stwo_hip_fri_prover_decommit(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uin
 
 }  // extern "C"
 
-int stwo_hip::fri_decommit_plan(stwo_hip_ctx *ctx, stwo_hip_fri_prover *pr, uint8_t channel_digest[32],
-                                uint32_t *channel_n_draws, Gatherer &g, FriDecommitPlan &plan) {
-  if (!ctx) return STWO_HIP_EINVAL;
-  STWO_HIP_CHECK_ARG(ctx, pr && channel_digest && channel_n_draws, "fri_prover_decommit: NULL argument");
-  STWO_HIP_CHECK_ARG(ctx, pr->device == ctx->device, "fri_prover_decommit: prover of another device");
-  STWO_HIP_CHECK_ARG(ctx, !pr->decommitted, "fri_prover_decommit: already decommitted");
-  const uint32_t n_queries = pr->cfg.n_queries;
-  STWO_HIP_CHECK_ARG(ctx, n_queries >= 1 && n_queries <= (1u << 20), "fri_prover_decommit: n_queries %u invalid",
-                     n_queries);
-  HostChannel ch;
-  memcpy(ch.digest, channel_digest, 32);
-  ch.n_draws = *channel_n_draws;
-  const uint32_t N = pr->log_size;
-  const uint32_t mask = (uint32_t)((1ull << N) - 1);
-  std::vector<uint32_t> &raw = plan.raw;
-  raw.clear();
-  while (raw.size() < n_queries) {
-    uint32_t w[8];
-    ch.draw_u32s(w);
-    for (int k = 0; k < 8 && raw.size() < n_queries; k++) raw.push_back(w[k] & mask);
-  }
-  std::vector<uint32_t> &queries = plan.queries;
-  queries = raw;
-  std::sort(queries.begin(), queries.end());
-  queries.erase(std::unique(queries.begin(), queries.end()), queries.end());
-
-  plan.layers.assign(pr->layers.size(), FriDecommitPlan::Layer());
-  std::vector<uint32_t> lq = queries, folded;
-  folded.reserve(lq.size());
-  for (size_t k = 0; k < pr->layers.size(); k++) {
-    const FriLayerKeep &layer = pr->layers[k];
-    const uint32_t step = layer.shape.fold_step;
-    FriDecommitPlan::Layer &pl = plan.layers[k];
-    // at most 2^step positions per query
-    const size_t max_pos = lq.size() << step;
-    pl.positions.reserve(max_pos);
-    pl.val_off.reserve(4 * max_pos);
-    pl.witness.reserve(max_pos);
-    g.reserve(4 * max_pos);
-    for (size_t i = 0; i < lq.size();) {
-      size_t j = i;
-      while (j < lq.size() && (lq[j] >> step) == (lq[i] >> step)) j++;
-      const uint32_t start = (lq[i] >> step) << step;
-      size_t next = i;
-      for (uint32_t pos = start; pos < start + (1u << step); pos++) {
-        pl.positions.push_back(pos);
-        for (int c = 0; c < 4; c++) pl.val_off.push_back(g.add(layer.cols[c] + pos, 1));
-        const bool queried = next < j && lq[next] == pos;
-        if (queried) next++;
-        pl.witness.push_back(queried ? 0 : 1);
-      }
-      i = j;
-    }
-    plan_merkle_decommit(g, layer.tree, layer.shape.log_size, pl.positions, pl.merkle);
-    folded.clear();
-    for (uint32_t q : lq) {
-      uint32_t f = q >> step;
-      if (folded.empty() || folded.back() != f) folded.push_back(f);
-    }
-    lq.swap(folded);
-  }
-  memcpy(channel_digest, ch.digest, 32);
-  *channel_n_draws = ch.n_draws;
-  return STWO_HIP_OK;
-}
-
-void stwo_hip::fri_decommit_finish(stwo_hip_fri_prover *pr, const Gatherer &g, FriDecommitPlan &plan) {
-  pr->proofs.assign(pr->layers.size(), FriLayerDecommit());
-  for (size_t k = 0; k < plan.layers.size(); k++) {
-    const FriDecommitPlan::Layer &pl = plan.layers[k];
-    FriLayerDecommit &d = pr->proofs[k];
-    d.positions = pl.positions;
-    for (size_t i = 0; i < pl.positions.size(); i++) {
-      for (int c = 0; c < 4; c++) d.values.push_back(g.out[pl.val_off[4 * i + c]]);
-      if (pl.witness[i])
-        for (int c = 0; c < 4; c++) d.fri_witness.push_back(g.out[pl.val_off[4 * i + c]]);
-    }
-    finish_merkle_decommit(g, pl.merkle, d.hash_witness, nullptr);
-  }
-  pr->query_positions = plan.queries;
-  pr->unsorted_queries = plan.raw;
-  pr->decommitted = true;
-}
-
 extern "C" {
 
 int stwo_hip_fri_prover_layer_proof(stwo_hip_ctx *ctx, const stwo_hip_fri_prover *pr, uint32_t layer,

@@ -263,10 +263,19 @@ void job_mem_end(stwo_hip_ctx *ctx) {
   ctx->job_cap = ctx->job_used = 0;
 }
 
+hipError_t ctx_malloc(stwo_hip_ctx *ctx, void **dptr, size_t bytes) {
+  if (ctx->fail_allocs_after == 0) {
+    *dptr = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  if (ctx->fail_allocs_after > 0) ctx->fail_allocs_after--;
+  return hipMalloc(dptr, bytes);
+}
+
 int device_alloc(stwo_hip_ctx *ctx, size_t bytes, void **dptr, bool *from_job) {
   *from_job = ctx->job_base != nullptr;
   if (*from_job) return job_carve(ctx, bytes, dptr, "device table");
-  if (hipMalloc(dptr, bytes ? bytes : 256) != hipSuccess) {
+  if (ctx_malloc(ctx, dptr, bytes ? bytes : 256) != hipSuccess) {
     (void)hipGetLastError();
     *dptr = nullptr;
     return set_error(ctx, STWO_HIP_ENOMEM, "device allocation of %zu bytes failed", bytes);
@@ -313,8 +322,10 @@ int get_scratch(stwo_hip_ctx *ctx, const char *key, size_t bytes, void **dptr) {
     ctx->scratch.erase(it);
   }
   void *p = nullptr;
-  if (hipMalloc(&p, bytes ? bytes : 256) != hipSuccess)
+  if (ctx_malloc(ctx, &p, bytes ? bytes : 256) != hipSuccess) {
+    (void)hipGetLastError();
     return set_error(ctx, STWO_HIP_ENOMEM, "scratch '%s' of %zu bytes: out of device memory", key, bytes);
+  }
   ctx->scratch[key] = {p, bytes};
   *dptr = p;
   return STWO_HIP_OK;
@@ -420,11 +431,17 @@ int stwo_hip_sync(stwo_hip_ctx *ctx) {
 
 void *stwo_hip_stream(stwo_hip_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
+int stwo_hip_ctx_fail_allocs_after(stwo_hip_ctx *ctx, long long after) {
+  if (!ctx) return STWO_HIP_EINVAL;
+  ctx->fail_allocs_after = after < 0 ? -1 : after;
+  return STWO_HIP_OK;
+}
+
 int stwo_hip_alloc(stwo_hip_ctx *ctx, size_t n_words, uint32_t **out) {
   if (!ctx || !out) return STWO_HIP_EINVAL;
   *out = nullptr;
   if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, STWO_HIP_EDEVICE, "hipSetDevice failed");
-  hipError_t e = hipMalloc((void **)out, (n_words ? n_words : 1) * sizeof(uint32_t));
+  hipError_t e = ctx_malloc(ctx, (void **)out, (n_words ? n_words : 1) * sizeof(uint32_t));
   if (e != hipSuccess) {
     (void)hipGetLastError();
     *out = nullptr;

@@ -85,7 +85,7 @@ int stwo_hip_twiddles_create(stwo_hip_ctx *ctx, stwo_hip_coset root, stwo_hip_tw
   tw->root = {root.initial_index & IDX_MASK, root.log_size};
   tw->len = (size_t)1 << root.log_size;
   // One allocation: tree | inverse tree | both doubled.
-  if (hipMalloc((void **)&tw->d_tw, tw->len * 16) != hipSuccess) {
+  if (ctx_malloc(ctx, (void **)&tw->d_tw, tw->len * 16) != hipSuccess) {
     (void)hipGetLastError();  // clear the sticky error of the failed allocation
     delete tw;
     return set_error(ctx, STWO_HIP_ENOMEM, "twiddle tree of %zu entries: out of device memory", (size_t)1 << root.log_size);

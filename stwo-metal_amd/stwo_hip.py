@@ -220,6 +220,7 @@ def lib():
             f.restype = ctypes.c_int
         L.stwo_hip_stream.argtypes = [_vp]
         L.stwo_hip_stream.restype = _vp
+        L.stwo_hip_ctx_fail_allocs_after.argtypes = [_vp, ctypes.c_longlong]
         _lib = L
     return _lib
 
@@ -325,6 +326,11 @@ class Context:
     def _check(self, rc):
         if rc != 0:
             raise StwoHipError(rc, self.last_error())
+
+    def fail_allocs_after(self, n):
+        """Test hook: the next n device allocations succeed, later ones fail
+        (STWO_HIP_ENOMEM); n < 0 turns it off."""
+        self._check(lib().stwo_hip_ctx_fail_allocs_after(self.h, n))
 
     # -- memory ---------------------------------------------------------------
     def alloc(self, n):
