@@ -84,14 +84,15 @@ def uneven_logs(n_cols=1024, lo=10, hi=22, seed=3):
     return [int(lo + j) for j in rng.choice(len(w), size=n_cols, p=w / w.sum())]
 
 
-def batched_vs_plain(ctx, steps=3):
+def batched_vs_plain(ctx, steps=3, warm=6):
     """1,024 uneven columns (log 10..22), LDE at blowup 2, one mixed-degree tree:
     the plain per-column form (stwo_hip_lde_commit_per_column, a C loop of one
     interpolate and one evaluate per column, then the commit) against one
     batched plan replayed as a hipGraph (coefficients not stored, the PCS
     default), with the planner's defaults and after stwo_hip_fft_autotune
     timed every candidate plan of the set's 26 FFT shapes on the device; the
-    tuned plan also issued launch by launch.  batched_ms is the fastest."""
+    tuned plan also issued launch by launch.  Each form: `warm` untimed runs,
+    then the fastest of `steps`; batched_ms is the fastest batched form."""
     logs = uneven_logs()
     b = 1
     max_log = max(logs) + b
@@ -118,13 +119,15 @@ def batched_vs_plain(ctx, steps=3):
         ctx.sync()
 
     def timed(fn):
+        # `warm` untimed runs first: the device's clocks take a few runs of
+        # this size to settle, so whichever plan ran first would be slower
         ts = []
-        for it in range(steps + 1):
+        for it in range(warm + steps):
             restore()
             t0 = time.perf_counter()
             fn()
             ctx.sync()
-            if it:
+            if it >= warm:
                 ts.append(time.perf_counter() - t0)
         return min(ts), layers.download(8)
 
@@ -161,6 +164,7 @@ def batched_vs_plain(ctx, steps=3):
         "log_size_histogram": hist,
         "plain_ms": round(tp * 1e3, 3),
         "plain_form": "C loop: per column stwo_hip_interpolate + stwo_hip_evaluate, then stwo_hip_merkle_commit",
+        "timing": "each form: %d untimed runs, then the fastest of %d" % (warm, steps),
         "batched_ms": round(best * 1e3, 3),
         "batched_default_ms": round(tb * 1e3, 3),
         "batched_tuned_ms": round(tt * 1e3, 3),
