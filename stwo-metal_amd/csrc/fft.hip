@@ -1584,6 +1584,15 @@ int stwo_hip_fft_autotune(stwo_hip_ctx *ctx, const uint32_t *log_sizes, size_t n
           best_t = ck.second;
         }
       }
+      // Another plan must beat the default by kTuneMargin: timed alone a shape
+      // gains a few percent from a run-time schedule, but inside a batched plan
+      // the default schedules share compile-time kernels with the other sizes
+      // of their level and run as fewer launches.
+      constexpr float kTuneMargin = 0.05f;
+      if (def_ms > 0 && best > def_ms * (1.0f - kTuneMargin)) {
+        best = def_ms;
+        best_t = def;
+      }
       ctx->fft_tune[fft_tune_key(n, Nd, inverse)] = best_t;
       if (report && nr < max_report) {
         uint32_t *o = report + 8 * nr;
