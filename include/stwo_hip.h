@@ -658,7 +658,9 @@ int stwo_hip_debug_fft_plan_tuned(uint32_t n, uint32_t N, int inverse, uint32_t 
  * most 5..8 layers (default 8), the tile's own workgroup size or 512 lanes.
  * Each runs on as many synthetic columns as the set has of that size (at most
  * 2^26 words, in a context scratch buffer); the fastest of 3 timed runs after
- * a warm-up counts.  report (optional): 8 words per tuned shape, up to
+ * a warm-up counts.  Another plan replaces the default only when it is at
+ * least 5% faster (the default schedules batch into fewer launches).
+ * report (optional): 8 words per tuned shape, up to
  * max_report: n, N, inverse, t0, smax, wg, best microseconds, the default
  * plan's microseconds; *n_report = shapes tuned.  Synchronizes. */
 int stwo_hip_fft_autotune(stwo_hip_ctx *ctx, const uint32_t *log_sizes, size_t n_cols, uint32_t log_blowup,
